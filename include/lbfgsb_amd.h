@@ -1,0 +1,153 @@
+/*
+ * C ABI of the MI355X L-BFGS-B engine (liblbfgsb_amd.so).
+ *
+ * Plain pointers and sizes only.  "Device" pointers are HIP device memory
+ * (hipMalloc); `stream` is a hipStream_t passed as void* (NULL = the null
+ * stream).  The C++ drop-in API in culbfgsb.h is a header-only layer over
+ * these entry points.
+ *
+ * Precision: the _f64 entry points compute all n-length work in FP64; the
+ * _f32 entry points keep the n-length vectors and the S/Y memory in FP32
+ * with FP64-accumulated (compensated) reductions, while the 2m x 2m matrix
+ * algebra, the Cauchy-search scalars and the line search stay in FP64.
+ */
+#ifndef LBFGSB_AMD_H_
+#define LBFGSB_AMD_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Modes, as LBFGSB_CUDA_MODE (culbfgsb/culbfgsb.h:31).  Only the GPU mode
+ * is provided by this library; the CPU mode returns info = -1. */
+enum { LBFGSB_AMD_MODE_CPU = 0, LBFGSB_AMD_MODE_GPU = 1 };
+
+/* LBFGSB_CUDA_OPTION<real> (culbfgsb/culbfgsb.h:33-44), same field order. */
+typedef struct {
+  double machine_epsilon, machine_maximum, step_scaling, eps_g, eps_f, eps_x;
+  int hessian_approximate_dimension, max_iteration, mode;
+} lbfgsb_options_f64;
+typedef struct {
+  float machine_epsilon, machine_maximum, step_scaling, eps_g, eps_f, eps_x;
+  int hessian_approximate_dimension, max_iteration, mode;
+} lbfgsb_options_f32;
+
+/* LBFGSB_CUDA_SUMMARY<real> (culbfgsb/culbfgsb.h:46-53).
+ * info: -2 internal error, -1 bad input, 0 stopped by the user callback,
+ *        1 eps_f, 2 eps_x, 4 eps_g, 5 max_iteration. */
+typedef struct {
+  double residual_g, residual_f, residual_x;
+  int num_iteration, info;
+} lbfgsb_summary_f64;
+typedef struct {
+  float residual_g, residual_f, residual_x;
+  int num_iteration, info;
+} lbfgsb_summary_f32;
+
+/* Callback of LBFGSB_CUDA_STATE (culbfgsb/culbfgsb.h:55-67): x and g are
+ * device pointers, f is a host scalar the callback writes; stream is the
+ * solve's hipStream_t.  funcgrad must have finished its device work when it
+ * returns.  The return value of funcgrad and after_iteration is ignored; a
+ * non-zero return of customized_stopping stops with info = 0. */
+typedef int (*lbfgsb_callback_f64)(void* ctx, double* x, double* f, double* g, void* stream,
+                                   const lbfgsb_summary_f64* summary);
+typedef int (*lbfgsb_callback_f32)(void* ctx, float* x, float* f, float* g, void* stream,
+                                   const lbfgsb_summary_f32* summary);
+
+typedef struct {
+  lbfgsb_callback_f64 funcgrad;
+  void* funcgrad_ctx;
+  lbfgsb_callback_f64 after_iteration;
+  void* after_iteration_ctx;
+  lbfgsb_callback_f64 customized_stopping;
+  void* customized_stopping_ctx;
+} lbfgsb_callbacks_f64;
+typedef struct {
+  lbfgsb_callback_f32 funcgrad;
+  void* funcgrad_ctx;
+  lbfgsb_callback_f32 after_iteration;
+  void* after_iteration_ctx;
+  lbfgsb_callback_f32 customized_stopping;
+  void* customized_stopping_ctx;
+} lbfgsb_callbacks_f32;
+
+/* lbfgsbdefaultoption<double|float> (culbfgsb/lbfgsb.cpp:44-66). */
+void lbfgsb_default_options_f64(lbfgsb_options_f64* opt);
+void lbfgsb_default_options_f32(lbfgsb_options_f32* opt);
+
+/* lbfgsbminimize<double|float> (culbfgsb/culbfgsb.h:157-163,
+ * lbfgsb.cpp:24-42).  x (in/out), nbd, l, u are device arrays of length n.
+ * Blocking; returns summary->info. */
+int lbfgsb_minimize_f64(int n, const lbfgsb_callbacks_f64* cb, const lbfgsb_options_f64* opt,
+                        double* x, const int* nbd, const double* l, const double* u,
+                        lbfgsb_summary_f64* summary, void* stream);
+int lbfgsb_minimize_f32(int n, const lbfgsb_callbacks_f32* cb, const lbfgsb_options_f32* opt,
+                        float* x, const int* nbd, const float* l, const float* u,
+                        lbfgsb_summary_f32* summary, void* stream);
+
+/* ---- Stepping engine: the per-step wrappers of the reference GPU driver
+ * (culbfgsb/lbfgsbcuda.cpp:443-770: lbfgsbactive, projgr, cauchy, freev,
+ * formk, cmprlb, subsm, lnsrlb, matupd, formt) become programs of fused
+ * kernels over a device-resident state.  dtype: 0 = fp32 bulk, 1 = fp64.
+ * Used by the parity tests, the batched form and the serving loop. */
+typedef struct lbfgsb_engine lbfgsb_engine;
+
+lbfgsb_engine* lbfgsb_engine_create(int dtype, long long n, int m, void* x, const int* nbd,
+                                    const void* l, const void* u, void* stream);
+void lbfgsb_engine_destroy(lbfgsb_engine* e);
+/* errclb + active projection (lbfgsbcpu.cpp:258-266); returns the status. */
+int lbfgsb_engine_setup_begin(lbfgsb_engine* e, double eps_g, double eps_f, double eps_x,
+                              int max_iteration);
+/* projgr after the first evaluation at x (lbfgsbcpu.cpp:271-278). */
+int lbfgsb_engine_setup_finish(lbfgsb_engine* e, double f);
+/* Enqueue program `program` from step `start_step` (-1 = first) up to, not
+ * including, `stop_step` (16 = to the end), then read the status.
+ * Programs: 0 iteration start, 1 after an evaluation (f = its value),
+ * 2 after an accepted step, 3 after the stopping check, 4 Cauchy-point tail. */
+int lbfgsb_engine_run(lbfgsb_engine* e, int program, int start_step, int stop_step, double f);
+/* Device pointers of the accepted (which = 0) or pending trial (1) x / g. */
+void* lbfgsb_engine_x(lbfgsb_engine* e, int which);
+void* lbfgsb_engine_g(lbfgsb_engine* e, int which);
+/* Named scalar of the device state (see DevScalars), as double. */
+int lbfgsb_engine_get_scalar(lbfgsb_engine* e, const char* name, double* value);
+int lbfgsb_engine_set_scalar(lbfgsb_engine* e, const char* name, double value);
+/* Named array copied to / from host memory.  Names: x, g, xb, gb, z, d, r,
+ * dsub (bulk type, n); iwhere (int32, n); iwbits (uint8, n); ws, wy (bulk
+ * type, n x m row-major); index, indx2 (int32, n, 1-based as the
+ * reference); sy, ss, wt (m x m), wn, snd (2m x 2m), p, c, mc, wv, pa
+ * (double). */
+int lbfgsb_engine_get_array(lbfgsb_engine* e, const char* name, void* out, long long count);
+int lbfgsb_engine_set_array(lbfgsb_engine* e, const char* name, const void* in,
+                            long long count);
+/* Leave the accepted iterate in the caller's x buffer. */
+int lbfgsb_engine_finalize(lbfgsb_engine* e);
+
+/* ---- Built-in objectives (the callback side; no counterpart in the
+ * reference library -- they restate its example callback shape,
+ * examples/dsscfg/dsscfg_cuda.cu:43-154, for the benchmark workloads).
+ * kind: 1 grid quadratic, 2 grid cos-energy, 3 extended Rosenbrock,
+ * 4 pentadiagonal QP.  Arrays are device arrays of the bulk type. */
+typedef struct lbfgsb_problem lbfgsb_problem;
+lbfgsb_problem* lbfgsb_problem_create(int dtype, int kind, long long n, long long nx,
+                                      long long ny, double h, const void* b, const void* c,
+                                      const void* a0, const void* a1, const void* a2);
+void lbfgsb_problem_destroy(lbfgsb_problem* p);
+/* Usable directly as funcgrad (ctx = the problem). */
+int lbfgsb_problem_funcgrad_f64(void* ctx, double* x, double* f, double* g, void* stream,
+                                const lbfgsb_summary_f64* summary);
+int lbfgsb_problem_funcgrad_f32(void* ctx, float* x, float* f, float* g, void* stream,
+                                const lbfgsb_summary_f32* summary);
+/* Device-side evaluation: writes g and the value to f_dev (double, device). */
+int lbfgsb_problem_eval_device(lbfgsb_problem* p, const void* x, void* g, double* f_dev,
+                               void* stream);
+
+/* Library version string. */
+const char* lbfgsb_amd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBFGSB_AMD_H_ */

@@ -1,0 +1,120 @@
+// Device-side helpers shared by the L-BFGS-B kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbfgsb {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;           // threads per block for bulk passes
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kMaxM = 8;              // L-BFGS memory (the reference fixes m <= 8)
+constexpr int kTile = 64;             // W is stored as [n/64][16][64] tiles
+constexpr int kMaxBlocks = 2048;      // grid cap for the bulk passes
+
+// ---------------------------------------------------------------------------
+// W = [Y | S] in HBM: element i, slot j (0..7 = Y, 8..15 = S) lives at
+//   ((i / 64) * 16 + j) * 64 + (i % 64)
+// so one wave reading one slot of 64 consecutive elements touches one
+// contiguous 256 B (fp32) / 512 B (fp64) segment, and a new slot is written
+// with fully coalesced stores.
+__host__ __device__ inline size_t widx(size_t i, int j) {
+  return ((i >> 6) * 16 + (size_t)j) * 64 + (i & 63);
+}
+__host__ __device__ inline size_t wsize(size_t n) { return ((n + 63) / 64) * 16 * 64; }
+
+// ---------------------------------------------------------------------------
+// iwhere is kept as one byte per variable:
+//   bits 0..3: iwhere + 4   (values -3, -1, 0, 1, 2, 3 -> 1, 3, 4, 5, 6, 7)
+//   bit 4    : free in the last free/active partition
+//   bit 5    : free in the partition before it (entering/leaving sets)
+__device__ __host__ inline int iw_get(uint8_t b) { return (int)(b & 15) - 4; }
+__device__ __host__ inline bool iw_wasfree(uint8_t b) { return (b >> 4) & 1; }
+__device__ __host__ inline bool iw_prevfree(uint8_t b) { return (b >> 5) & 1; }
+__device__ __host__ inline uint8_t iw_make(int iw, bool freelast, bool freeprev) {
+  return (uint8_t)((iw + 4) | (freelast ? 16 : 0) | (freeprev ? 32 : 0));
+}
+
+// ---------------------------------------------------------------------------
+// Wave / block reductions.  All cross-lane traffic goes through
+// __shfl_xor (ds_swizzle / dpp on gfx950); 64 lanes.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+__device__ inline int wave_sum_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ inline int wave_or_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, kWave);
+  return v;
+}
+// (value, index) minimum with ties to the smallest index.
+__device__ inline void wave_argmin(double& v, long long& idx) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(v, o, kWave);
+    long long oi = __shfl_xor(idx, o, kWave);
+    if (ov < v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
+  }
+}
+
+// Error-free transformation a + b = s + e (Knuth two-sum).
+__device__ __host__ inline void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  double bp = s - a;
+  e = (a - (s - bp)) + (b - bp);
+}
+
+// Compensated sum of n doubles laid out with stride `stride`.
+__device__ inline double neumaier_sum(const double* p, int n, int stride) {
+  double s = 0.0, c = 0.0;
+  for (int i = 0; i < n; i++) {
+    double e;
+    two_sum(s, p[(size_t)i * stride], s, e);
+    c += e;
+  }
+  return s + c;
+}
+
+// Reduce NV per-thread values over a block of kBlock threads; writes the
+// block totals to out[v * out_stride].  `sh` must hold kWavesPerBlock*NV
+// doubles.  Deterministic (fixed tree).
+template <int NV>
+__device__ inline void block_sum_store(double (&vals)[NV], double* sh, double* out,
+                                       size_t out_stride) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    double s = wave_sum(vals[v]);
+    if (lane == 0) sh[wid * NV + v] = s;
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < NV; v += blockDim.x) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; w++) s += sh[w * NV + v];
+    out[(size_t)v * out_stride] = s;
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ inline T ld_nt(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+}  // namespace lbfgsb

@@ -1,0 +1,441 @@
+// Host engine of the L-BFGS-B solver (see engine.hpp).
+#include "engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "objectives.hpp"
+
+namespace lbfgsb {
+
+#define HIPCHK(expr)                       \
+  do {                                     \
+    hipError_t e_ = (expr);                \
+    if (e_ != hipSuccess) {                \
+      last_error = e_;                     \
+      return -1;                           \
+    }                                      \
+  } while (0)
+
+static const int kTail[] = {K_P4, K_FORMK, K_P5A, K_ALPHA, K_P5B, K_LSINIT, K_TRIAL};
+
+static std::vector<int> program_steps(int prog) {
+  std::vector<int> s;
+  switch (prog) {
+    case PROG_ITER_START:
+      s = {K_RESUME_RESET, K_PASSA_SA, K_UPD_REDUCE};
+      break;
+    case PROG_AFTER_EVAL:
+      s = {K_GD, K_LSSTEP, K_TRIAL, K_PASSA_FUSED, K_STOP, K_UPD};
+      break;
+    case PROG_AFTER_ACCEPT:
+      s = {K_RESUME_RESET, K_PASSA_FUSED, K_STOP, K_UPD};
+      break;
+    case PROG_AFTER_CHECK:
+      s = {K_RESUME, K_UPD};
+      break;
+    default:
+      break;
+  }
+  for (int k : kTail) s.push_back(k);
+  return s;
+}
+
+template <typename T>
+struct Engine final : EngineBase {
+  long long n;
+  int m;
+  int G;
+  hipStream_t st;
+  T* X[2];
+  T* Gr[2];
+  T** dX = nullptr;
+  T** dG = nullptr;
+  const T* l;
+  const T* u;
+  const int* nbd;
+  T *W = nullptr, *d = nullptr, *z = nullptr, *r = nullptr, *dsub = nullptr, *bp_t = nullptr;
+  int* bp_i = nullptr;
+  uint8_t* iwhere = nullptr;
+  DevScalars* ds = nullptr;
+  DevScalars* hs = nullptr;
+  double *part = nullptr, *pmax = nullptr, *pmin = nullptr;
+  long long* pmini = nullptr;
+  Cand *ca = nullptr, *cb = nullptr;
+  double *samp = nullptr, *tsum = nullptr, *tpre = nullptr, *total = nullptr;
+  double *tile_state = nullptr, *last_state = nullptr;
+  long long* tile_stop = nullptr;
+  int* index = nullptr;
+  int* indx2 = nullptr;
+  unsigned* exp_counts = nullptr;
+  unsigned long long* exp_totals = nullptr;
+  bool ok = false;
+
+  Engine(long long n_, int m_, T* x, const int* nbd_, const T* l_, const T* u_, hipStream_t s)
+      : n(n_), m(m_), st(s), l(l_), u(u_), nbd(nbd_) {
+    X[0] = x;
+    X[1] = Gr[0] = Gr[1] = nullptr;
+    long long blocks = (n + kBlock - 1) / kBlock;
+    G = (int)std::max<long long>(1, std::min<long long>(blocks, kMaxBlocks));
+    ok = alloc() == 0;
+  }
+
+  template <typename P>
+  int dmalloc(P*& p, size_t count) {
+    HIPCHK(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(P)));
+    return 0;
+  }
+
+  int alloc() {
+    const size_t nn = (size_t)n;
+    if (dmalloc(X[1], nn) || dmalloc(Gr[0], nn) || dmalloc(Gr[1], nn)) return -1;
+    if (dmalloc(W, wsize(nn)) || dmalloc(d, nn) || dmalloc(z, nn) || dmalloc(r, nn) ||
+        dmalloc(dsub, nn) || dmalloc(bp_t, nn) || dmalloc(bp_i, nn) || dmalloc(iwhere, nn))
+      return -1;
+    HIPCHK(hipMemsetAsync(W, 0, wsize(nn) * sizeof(T), st));
+    if (dmalloc(dX, 2) || dmalloc(dG, 2) || dmalloc(ds, 1)) return -1;
+    HIPCHK(hipHostMalloc((void**)&hs, sizeof(DevScalars), hipHostMallocDefault));
+    std::memset(hs, 0, sizeof(DevScalars));
+    const size_t nv = std::max<int>(P4_COUNT, PA_COUNT);
+    if (dmalloc(part, nv * G) || dmalloc(pmax, G) || dmalloc(pmin, G) || dmalloc(pmini, G))
+      return -1;
+    const size_t ntiles = nn / kBlock + 2;
+    if (dmalloc(ca, nn) || dmalloc(cb, nn) || dmalloc(samp, 2048) || dmalloc(tsum, 33 * ntiles) ||
+        dmalloc(tpre, 33 * ntiles) || dmalloc(total, 40) || dmalloc(tile_state, 33 * ntiles) ||
+        dmalloc(last_state, 40) || dmalloc(tile_stop, ntiles))
+      return -1;
+    T* hp[2] = {X[0], X[1]};
+    HIPCHK(hipMemcpyAsync(dX, hp, sizeof(hp), hipMemcpyHostToDevice, st));
+    T* hg[2] = {Gr[0], Gr[1]};
+    HIPCHK(hipMemcpyAsync(dG, hg, sizeof(hg), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  }
+
+  ~Engine() override {
+    hipStreamSynchronize(st);
+    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, z, r, dsub, bp_t};
+    for (T* p : bufs) hipFree(p);
+    hipFree(bp_i);
+    hipFree(iwhere);
+    hipFree(dX);
+    hipFree(dG);
+    hipFree(ds);
+    hipHostFree(hs);
+    hipFree(part);
+    hipFree(pmax);
+    hipFree(pmin);
+    hipFree(pmini);
+    hipFree(ca);
+    hipFree(cb);
+    hipFree(samp);
+    hipFree(tsum);
+    hipFree(tpre);
+    hipFree(total);
+    hipFree(tile_state);
+    hipFree(last_state);
+    hipFree(tile_stop);
+    hipFree(index);
+    hipFree(indx2);
+    hipFree(exp_counts);
+    hipFree(exp_totals);
+  }
+
+  int dtype() const override { return sizeof(T) == 8 ? 1 : 0; }
+  hipStream_t stream() const override { return st; }
+  long long size() const override { return n; }
+  DevScalars& host_scalars() override { return *hs; }
+
+  int read_scalars() override {
+    HIPCHK(hipMemcpyAsync(hs, ds, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  }
+  int write_scalars() override {
+    HIPCHK(hipMemcpyAsync(ds, hs, sizeof(DevScalars), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  }
+
+  int cur() const { return hs->phase & 1; }
+  void* x_ptr(int which) override {
+    if (which == 0) return X[cur()];
+    return X[hs->trial_dst];
+  }
+  void* g_ptr(int which) override {
+    if (which == 0) return Gr[cur()];
+    return Gr[hs->trial_dst];
+  }
+
+  // errclb (device part) and the projection of x0 (lbfgsbcpu.cpp:258-266).
+  int setup_begin() override {
+    launch_errclb<T>(st, G, n, nbd, l, u, part);
+    launch_setup_errclb(st, ds, part, G);
+    HIPCHK(hipGetLastError());
+    if (read_scalars()) return -1;
+    if (hs->status == ST_DONE) return ST_DONE;  // x is left untouched on bad input
+    launch_active<T>(st, G, n, X[0], l, u, nbd, iwhere, part);
+    launch_setup_active(st, ds, part, G);
+    HIPCHK(hipGetLastError());
+    return read_scalars() ? -1 : hs->status;
+  }
+  // after the first evaluation at x0 (lbfgsbcpu.cpp:271-278)
+  int setup_finish(double f) override {
+    launch_projgr<T>(st, G, n, X[0], Gr[0], l, u, nbd, pmax);
+    launch_setup_projgr(st, ds, pmax, G, f);
+    HIPCHK(hipGetLastError());
+    return read_scalars() ? -1 : hs->status;
+  }
+
+  void enqueue(int step, double f) {
+    switch (step) {
+      case K_RESUME_RESET:
+        launch_resume(st, ds, 1);
+        break;
+      case K_RESUME:
+        launch_resume(st, ds, 0);
+        break;
+      case K_PASSA_SA:
+        launch_pass_a<T>(st, G, false, n, dX, dG, l, u, nbd, iwhere, W, d, bp_t, bp_i,
+                         &ds->bp_count, ds, part, pmax, pmin, pmini);
+        break;
+      case K_UPD_REDUCE:
+        launch_update_gcp(st, ds, part, pmin, pmini, G, 1);
+        break;
+      case K_PASSA_FUSED:
+        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, nbd, iwhere, W, d, bp_t, bp_i,
+                         &ds->bp_count, ds, part, pmax, pmin, pmini);
+        break;
+      case K_STOP:
+        launch_stop_tests(st, ds, part, pmax, pmin, pmini, G);
+        break;
+      case K_UPD:
+        launch_update_gcp(st, ds, part, pmin, pmini, G, 0);
+        break;
+      case K_P4:
+        launch_p4<T>(st, G, n, dX, dG, l, u, nbd, iwhere, W, z, r, ds, part);
+        break;
+      case K_FORMK:
+        launch_formk_subsm(st, ds, part, G);
+        break;
+      case K_P5A:
+        launch_p5a<T>(st, G, n, l, u, nbd, iwhere, W, z, r, dsub, ds, pmin, pmini);
+        break;
+      case K_ALPHA:
+        launch_alpha(st, ds, pmin, pmini, G);
+        break;
+      case K_P5B:
+        launch_p5b<T>(st, G, n, dX, dG, l, u, nbd, iwhere, z, dsub, d, ds, part, pmin);
+        break;
+      case K_LSINIT:
+        launch_ls_init(st, ds, part, pmin, G);
+        break;
+      case K_TRIAL:
+        launch_trial<T>(st, G, n, dX, z, d, ds);
+        break;
+      case K_GD:
+        launch_gd<T>(st, G, n, dG, d, ds, part);
+        break;
+      case K_LSSTEP:
+        launch_ls_step(st, ds, part, G, f);
+        break;
+      default:
+        break;
+    }
+  }
+
+  // Sorted breakpoint rounds of the Cauchy search (gcp.hip).
+  int run_rounds() {
+    launch_gcp_prepare<T>(st, ds, bp_t, samp, &ds->cand_count);
+    for (int guard = 0; guard < 64; guard++) {
+      const long long nb = hs->gcp_nbreak;
+      launch_gcp_select<T>(st, G, bp_t, bp_i, nb, ds, ca, &ds->cand_count);
+      HIPCHK(hipGetLastError());
+      if (read_scalars()) return -1;
+      const long long K = (long long)hs->cand_count;
+      Cand* sorted = sort_candidates(st, ca, cb, K);
+      launch_gcp_scan<T>(st, sorted, K, dG, W, ds, tsum, tpre, total, tile_stop, tile_state,
+                         last_state, samp, &ds->cand_count);
+      HIPCHK(hipGetLastError());
+      if (read_scalars()) return -1;
+      if (hs->status != ST_GCP_ROUNDS) return 0;
+    }
+    return -1;
+  }
+
+  int run(int program, int start_step, int stop_step, double f) override {
+    const std::vector<int> steps = program_steps(program);
+    size_t i = 0;
+    if (start_step >= 0 && start_step != K_END) {
+      while (i < steps.size() && steps[i] != start_step) i++;
+      if (i == steps.size()) i = 0;
+    }
+    size_t j = i;
+    while (j < steps.size() && steps[j] != stop_step) j++;
+    for (size_t k = i; k < j; k++) enqueue(steps[k], f);
+    HIPCHK(hipGetLastError());
+    if (read_scalars()) return -1;
+    if (hs->status == ST_GCP_ROUNDS) {
+      if (run_rounds()) return -1;
+      // resume the tail from the Cauchy point pass, up to stop_step
+      size_t p = 0;
+      while (p < steps.size() && steps[p] != K_P4) p++;
+      if (p < j) {
+        for (size_t k = p; k < j; k++) enqueue(steps[k], f);
+        HIPCHK(hipGetLastError());
+        if (read_scalars()) return -1;
+      }
+    }
+    return hs->status;
+  }
+
+  int finalize_x() override {
+    if (read_scalars()) return -1;
+    if (cur() != 0) {
+      HIPCHK(hipMemcpyAsync(X[0], X[1], (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    return 0;
+  }
+
+  // ---- named arrays (host copies) for inspection and state loading
+  int get_array(const std::string& name, void* out, long long count) override {
+    if (read_scalars()) return -1;
+    const size_t nn = (size_t)n;
+    auto dcopy = [&](const void* src, size_t bytes) -> int {
+      HIPCHK(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      return 0;
+    };
+    const int c = cur();
+    if (name == "x") return dcopy(X[c], nn * sizeof(T));
+    if (name == "g") return dcopy(Gr[c], nn * sizeof(T));
+    if (name == "xb") return dcopy(X[c ^ 1], nn * sizeof(T));
+    if (name == "gb") return dcopy(Gr[c ^ 1], nn * sizeof(T));
+    if (name == "z") return dcopy(z, nn * sizeof(T));
+    if (name == "d") return dcopy(d, nn * sizeof(T));
+    if (name == "r") return dcopy(r, nn * sizeof(T));
+    if (name == "dsub") return dcopy(dsub, nn * sizeof(T));
+    if (name == "iwbits") return dcopy(iwhere, nn);
+    if (name == "iwhere") {
+      std::vector<uint8_t> b(nn);
+      HIPCHK(hipMemcpyAsync(b.data(), iwhere, nn, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      int* o = (int*)out;
+      for (size_t i = 0; i < nn; i++) o[i] = iw_get(b[i]);
+      return 0;
+    }
+    if (name == "ws" || name == "wy") {
+      std::vector<T> w(wsize(nn));
+      HIPCHK(hipMemcpyAsync(w.data(), W, w.size() * sizeof(T), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      const int off = name == "ws" ? 8 : 0;
+      T* o = (T*)out;
+      for (size_t i = 0; i < nn; i++)
+        for (int j = 0; j < m; j++) o[i * m + j] = w[widx(i, off + j)];
+      return 0;
+    }
+    if (name == "index" || name == "indx2") {
+      if (!index) {
+        if (dmalloc(index, nn) || dmalloc(indx2, nn) ||
+            dmalloc(exp_counts, 4 * (size_t)export_count_blocks(n) + 4) ||
+            dmalloc(exp_totals, 4))
+          return -1;
+      }
+      const int track = hs->iter > 0 && hs->cnstnd && !hs->shortcut;
+      launch_export_index(st, iwhere, n, track, exp_counts, exp_totals, index, indx2);
+      HIPCHK(hipGetLastError());
+      return dcopy(name == "index" ? index : indx2, nn * sizeof(int));
+    }
+    double* o = (double*)out;
+    auto small = [&](const double* a, int ld, int dim) {
+      for (int i = 0; i < dim; i++)
+        for (int j = 0; j < dim; j++) o[i * dim + j] = a[i * ld + j];
+      return 0;
+    };
+    if (name == "sy") return small(hs->sy, 8, m);
+    if (name == "ss") return small(hs->ss, 8, m);
+    if (name == "wt") return small(hs->wt, 8, m);
+    if (name == "wn") return small(hs->wn, 16, 2 * m);
+    if (name == "snd") return small(hs->wn1, 16, 2 * m);
+    auto vec = [&](const double* a, long long cnt) {
+      for (long long i = 0; i < cnt; i++) o[i] = a[i];
+      return 0;
+    };
+    if (name == "p") return vec(hs->p, std::min<long long>(count, 16));
+    if (name == "c") return vec(hs->c, std::min<long long>(count, 16));
+    if (name == "mc") return vec(hs->mc, std::min<long long>(count, 16));
+    if (name == "wv") return vec(hs->wv, std::min<long long>(count, 16));
+    if (name == "pa") return vec(hs->pa, std::min<long long>(count, PA_COUNT));
+    return -2;
+  }
+
+  int set_array(const std::string& name, const void* in, long long count) override {
+    if (read_scalars()) return -1;
+    const size_t nn = (size_t)n;
+    auto hcopy = [&](void* dst, size_t bytes) -> int {
+      HIPCHK(hipMemcpyAsync(dst, in, bytes, hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+      return 0;
+    };
+    const int c = cur();
+    if (name == "x") return hcopy(X[c], nn * sizeof(T));
+    if (name == "g") return hcopy(Gr[c], nn * sizeof(T));
+    if (name == "xb") return hcopy(X[c ^ 1], nn * sizeof(T));
+    if (name == "gb") return hcopy(Gr[c ^ 1], nn * sizeof(T));
+    if (name == "z") return hcopy(z, nn * sizeof(T));
+    if (name == "d") return hcopy(d, nn * sizeof(T));
+    if (name == "iwbits") return hcopy(iwhere, nn);
+    if (name == "ws" || name == "wy") {
+      std::vector<T> w(wsize(nn));
+      HIPCHK(hipMemcpyAsync(w.data(), W, w.size() * sizeof(T), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      const int off = name == "ws" ? 8 : 0;
+      const T* src = (const T*)in;
+      for (size_t i = 0; i < nn; i++)
+        for (int j = 0; j < m; j++) w[widx(i, off + j)] = src[i * m + j];
+      HIPCHK(hipMemcpyAsync(W, w.data(), w.size() * sizeof(T), hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+      return 0;
+    }
+    const double* s = (const double*)in;
+    auto small = [&](double* a, int ld, int dim) {
+      for (int i = 0; i < dim; i++)
+        for (int j = 0; j < dim; j++) a[i * ld + j] = s[i * dim + j];
+      return write_scalars();
+    };
+    if (name == "sy") return small(hs->sy, 8, m);
+    if (name == "ss") return small(hs->ss, 8, m);
+    if (name == "wt") return small(hs->wt, 8, m);
+    if (name == "wn") return small(hs->wn, 16, 2 * m);
+    if (name == "snd") return small(hs->wn1, 16, 2 * m);
+    auto vec = [&](double* a, long long cnt) {
+      for (long long i = 0; i < cnt; i++) a[i] = s[i];
+      return write_scalars();
+    };
+    if (name == "c") return vec(hs->c, std::min<long long>(count, 16));
+    return -2;
+  }
+};
+
+EngineBase* make_engine(int dtype, long long n, int m, void* x, const int* nbd, const void* l,
+                        const void* u, hipStream_t stream) {
+  EngineBase* e;
+  if (dtype == 1)
+    e = new Engine<double>(n, m, (double*)x, nbd, (const double*)l, (const double*)u, stream);
+  else
+    e = new Engine<float>(n, m, (float*)x, nbd, (const float*)l, (const float*)u, stream);
+  bool ok = dtype == 1 ? static_cast<Engine<double>*>(e)->ok : static_cast<Engine<float>*>(e)->ok;
+  if (!ok) {
+    delete e;
+    return nullptr;
+  }
+  return e;
+}
+
+}  // namespace lbfgsb

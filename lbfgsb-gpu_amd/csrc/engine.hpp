@@ -1,0 +1,70 @@
+// Host engine: owns the device state of one solve and enqueues the kernel
+// chain of an iteration.  The chain is speculative: every kernel checks the
+// device halt flag, so a whole iteration (or everything between two function
+// evaluations) is enqueued at once and the host reads one status block per
+// evaluation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "launch.hpp"
+#include "state.hpp"
+
+namespace lbfgsb {
+
+// Kernel steps of the chain.
+enum Step : int {
+  K_RESUME_RESET = 0,
+  K_RESUME,
+  K_PASSA_SA,   // standalone Cauchy classification pass
+  K_UPD_REDUCE, // controller after the standalone pass
+  K_PASSA_FUSED,
+  K_STOP,
+  K_UPD,
+  K_P4,
+  K_FORMK,
+  K_P5A,
+  K_ALPHA,
+  K_P5B,
+  K_LSINIT,
+  K_TRIAL,
+  K_GD,
+  K_LSSTEP,
+  K_END
+};
+
+// Programs (sequences of steps).
+enum Program : int {
+  PROG_ITER_START = 0,  // iteration 0 and after every memory reset
+  PROG_AFTER_EVAL = 1,  // a function value for the pending trial is known
+  PROG_AFTER_ACCEPT = 2,
+  PROG_AFTER_CHECK = 3,
+  PROG_TAIL = 4
+};
+
+struct EngineBase {
+  virtual ~EngineBase() {}
+  virtual int dtype() const = 0;  // 0 = fp32 bulk, 1 = fp64 bulk
+  virtual int setup_begin() = 0;  // errclb + active; returns status
+  virtual int setup_finish(double f) = 0;
+  virtual int run(int program, int start_step, int stop_step, double f) = 0;
+  virtual DevScalars& host_scalars() = 0;
+  virtual int read_scalars() = 0;
+  virtual int write_scalars() = 0;
+  virtual void* x_ptr(int which) = 0;  // 0 = accepted x, 1 = trial x
+  virtual void* g_ptr(int which) = 0;
+  virtual int get_array(const std::string& name, void* out, long long count) = 0;
+  virtual int set_array(const std::string& name, const void* in, long long count) = 0;
+  virtual int finalize_x() = 0;  // leaves the accepted x in the caller's buffer
+  virtual hipStream_t stream() const = 0;
+  virtual long long size() const = 0;
+  hipError_t last_error = hipSuccess;
+};
+
+EngineBase* make_engine(int dtype, long long n, int m, void* x, const int* nbd, const void* l,
+                        const void* u, hipStream_t stream);
+
+}  // namespace lbfgsb

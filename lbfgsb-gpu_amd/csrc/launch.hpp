@@ -1,0 +1,83 @@
+// Host-side launch wrappers of the engine's kernels (one definition per
+// .hip file; all launches are asynchronous on the given stream).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "state.hpp"
+
+namespace lbfgsb {
+
+// Breakpoint candidate of the Cauchy search: sorted by (t, idx).
+struct Cand {
+  double t;
+  int idx;
+  int pad;
+};
+
+// bulk.hip
+template <typename T>
+void launch_errclb(hipStream_t s, int G, long long n, const int* nbd, const T* l, const T* u,
+                   double* part);
+template <typename T>
+void launch_active(hipStream_t s, int G, long long n, T* x, const T* l, const T* u,
+                   const int* nbd, uint8_t* iwhere, double* part);
+template <typename T>
+void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g, const T* l,
+                   const T* u, const int* nbd, double* pmax);
+template <typename T>
+void launch_pass_a(hipStream_t s, int G, bool fused, long long n, T* const* X, T* const* Gv,
+                   const T* l, const T* u, const int* nbd, uint8_t* iwhere, T* W, const T* d,
+                   T* bp_t, int* bp_i, unsigned long long* bp_count, DevScalars* ds,
+                   double* part, double* pmax, double* pmin, long long* pmini);
+template <typename T>
+void launch_p4(hipStream_t s, int G, long long n, T* const* X, T* const* Gv, const T* l,
+               const T* u, const int* nbd, uint8_t* iwhere, const T* W, T* z, T* r,
+               DevScalars* ds, double* part);
+template <typename T>
+void launch_p5a(hipStream_t s, int G, long long n, const T* l, const T* u, const int* nbd,
+                const uint8_t* iwhere, const T* W, const T* z, const T* r, T* dsub,
+                DevScalars* ds, double* pmin, long long* pmini);
+template <typename T>
+void launch_p5b(hipStream_t s, int G, long long n, T* const* X, T* const* Gv, const T* l,
+                const T* u, const int* nbd, const uint8_t* iwhere, T* z, const T* dsub, T* d,
+                DevScalars* ds, double* part, double* pmin);
+template <typename T>
+void launch_trial(hipStream_t s, int G, long long n, T* const* X, const T* z, const T* d,
+                  const DevScalars* ds);
+template <typename T>
+void launch_gd(hipStream_t s, int G, long long n, T* const* Gv, const T* d,
+               const DevScalars* ds, double* part);
+
+// small.hip
+void launch_resume(hipStream_t s, DevScalars* ds, int reset_bp);
+void launch_setup_errclb(hipStream_t s, DevScalars* ds, const double* part, int G);
+void launch_setup_active(hipStream_t s, DevScalars* ds, const double* part, int G);
+void launch_setup_projgr(hipStream_t s, DevScalars* ds, const double* pmax, int G, double f);
+void launch_stop_tests(hipStream_t s, DevScalars* ds, const double* part, const double* pmax,
+                       const double* pmin, const long long* pmini, int G);
+void launch_update_gcp(hipStream_t s, DevScalars* ds, const double* part, const double* pmin,
+                       const long long* pmini, int G, int reduce);
+void launch_formk_subsm(hipStream_t s, DevScalars* ds, const double* part, int G);
+void launch_alpha(hipStream_t s, DevScalars* ds, const double* pmin, const long long* pmini,
+                  int G);
+void launch_ls_init(hipStream_t s, DevScalars* ds, const double* part, const double* pmin,
+                    int G);
+void launch_ls_step(hipStream_t s, DevScalars* ds, const double* part, int G, double f);
+
+// gcp.hip
+template <typename T>
+void launch_gcp_prepare(hipStream_t s, DevScalars* ds, const T* bp_t, double* samp,
+                        unsigned long long* cnt);
+template <typename T>
+void launch_gcp_select(hipStream_t s, int G, const T* bp_t, const int* bp_i, long long nb,
+                       const DevScalars* ds, Cand* cand, unsigned long long* cnt);
+Cand* sort_candidates(hipStream_t s, Cand* a, Cand* b, long long K);
+template <typename T>
+void launch_gcp_scan(hipStream_t s, const Cand* cand, long long K, T* const* Gv, const T* W,
+                     DevScalars* ds, double* tsum, double* tpre, double* total,
+                     long long* tile_stop, double* tile_state, double* last_state,
+                     const double* samp, unsigned long long* cnt);
+
+}  // namespace lbfgsb

@@ -1,0 +1,128 @@
+// Device-resident solver state shared by the host driver and the kernels.
+//
+// Everything the per-iteration control flow needs lives in one DevScalars
+// block in HBM, so the small single-workgroup kernels can make every
+// decision of the reference driver (lbfgsbcpu.cpp:279-462) on the device and
+// the host only has to read a status word once per function evaluation.
+// Small dense algebra and all line-search scalars are FP64 regardless of the
+// bulk type.
+#pragma once
+
+#include <stdint.h>
+
+namespace lbfgsb {
+
+// Host-visible status (DevScalars::status).
+enum Status : int {
+  ST_RUNNING = 0,     // chain continues on the device
+  ST_NEED_EVAL = 1,   // a trial point is ready: call funcgrad(xb, f, gb)
+  ST_ACCEPTED = 2,    // line search accepted a step (host may run callbacks)
+  ST_CHECKED = 3,     // stopping tests passed, custom stop may be asked
+  ST_GCP_ROUNDS = 4,  // breakpoint search needs sorted rounds
+  ST_RESTART = 5,     // memory reset: the iteration restarts at the Cauchy step
+  ST_DONE = 6         // solve ended, info holds the reason
+};
+
+// Slot layout of per-block partial sums (value-major: part[v * G + block]).
+enum PartA : int {
+  PA_TF = 0,        // sum (xold - x)^2
+  PA_RR = 1,        // sum y^2
+  PA_F1 = 2,        // sum over GCP-free variables of g^2
+  PA_NBREAK = 3,    // number of breakpoints
+  PA_NFREEGCP = 4,  // free variables that are not breakpoints
+  PA_UNBND = 5,     // free non-breakpoint with g != 0 (reference `bnded` false)
+  PA_SY = 6,        // 8: s . y_j for the retained slots
+  PA_SS = 14,       // 8: s_j . s for the retained slots
+  PA_P = 22,        // 16: p = [Y'd ; S'd] over GCP-free variables
+  PA_COUNT = 38
+};
+enum Part4 : int {
+  P4_NFREE = 0,
+  P4_NENTER = 1,
+  P4_NLEAVE = 2,
+  P4_NC = 3,      // 32: new-column inner products (yy, ss, sy_active, sy_free)
+  P4_WV = 35,     // 16: [Y'r ; S'r] over free variables
+  P4_CORR = 51,   // 136: signed entering/leaving inner products
+  P4_COUNT = 187
+};
+enum Part5 : int {
+  P5_DTD = 0,
+  P5_GD = 1,
+  P5_COUNT = 2
+};
+
+struct DevScalars {
+  // ---- control / host interface
+  int status;
+  int halt;          // when set, bulk kernels in the speculative chain no-op
+  int phase;
+  int info;          // summary info code
+  int internalinfo;
+
+  // ---- sizes and ring state (head/itail are 0-based slots)
+  long long n;
+  int m, col, head, itail, iupdat, iter, nfgv, nint, nintol, nskip;
+  int nfree, nenter, ileave, nleave, wrk, updatd, iword, ifun, iback, task, csave;
+  int cnstnd, boxed, prjctd;
+  int isave[2];
+  int max_iteration;
+  int restarts;
+  int skip_subsm;    // nfree == 0 || col == 0 for this iteration
+  int shortcut;      // unconstrained shortcut (no Cauchy search)
+
+  // ---- line-search / driver scalars
+  double f, fold, theta, dr, rr, dnrm, xstep, sbgnrm, ddum, dtd, gd, gdold;
+  double stp, stpmx, tf;
+  double eps_g, eps_f, eps_x;
+  double residual_g, residual_f, residual_x;
+  double dsave[13];
+
+  // ---- generalized Cauchy point
+  int gcp_mode;      // 0 none passed, 1 threshold (passed set), 2 trivial, 3 all-n passed
+  int gcp_nbreak;
+  int gcp_nfreegcp;
+  int gcp_unbnd;
+  long long gcp_kpassed;   // breakpoints passed so far
+  long long gcp_last_idx;  // (t, idx) of the last passed breakpoint
+  double gcp_last_t;
+  double gcp_f1, gcp_f2, gcp_f2org, gcp_dtm, gcp_tsum, gcp_bkmin;
+  long long gcp_ibkmin;
+  double gcp_D0;           // sum of d^2 over GCP-free variables
+  double gcp_carry[40];    // running [P(2m) ; Q(2m) ; D] over passed breakpoints
+  double gcp_tau_lo, gcp_tau_hi;
+  int gcp_round;
+  int gcp_floor_hits;      // f2 floor engaged (closed form may deviate)
+  long long gcp_ncand;
+  long long gcp_stop;      // first stop position in the current round (or -1)
+  long long gcp_nsample;
+  int gcp_last_round;      // the current round covers all remaining breakpoints
+  int gcp_cand_in_b;       // sorted candidates ended in the second buffer
+  double gcp_t_prev;       // t of the last passed breakpoint (0 before any)
+  long long gcp_idx_prev;
+
+  // ---- small dense algebra (row-major, leading dimension 8 or 16)
+  double sy[64], ss[64], wt[64];
+  double wn[256], wn1[256];
+  double Mx[256];          // explicit middle matrix M (2col x 2col, ld 16)
+  double p[16], c[16], v[16], wv[16], mc[16];
+  double newcol[32];
+  double pending_s_ss_diag;
+
+  // ---- reduced pass-A values (see PartA)
+  double pa[PA_COUNT];
+
+  // ---- subspace step
+  double alpha;
+  long long ibd;
+
+  // ---- trial point bookkeeping: k_trial writes X[trial_dst] from
+  // X[trial_src] when write_trial is set
+  int write_trial, trial_src, trial_dst;
+  int has_after_iteration, has_custom_stop;
+
+  // ---- append counters (breakpoint list, round candidates)
+  unsigned long long bp_count;
+  unsigned long long cand_count;
+};
+
+}  // namespace lbfgsb
