@@ -1,0 +1,394 @@
+"""Pins of the CPU oracle against independent statements of the mathematics.
+
+The reference's CPU path cannot be compiled in this image (it includes CUDA
+and cuBLAS headers), and the reference ships no golden vectors, so every
+oracle component is checked here against something that does not share its
+code: LINPACK identities, the explicit inverse of the middle matrix, a dense
+brute-force Cauchy point, a dense subspace Newton step, MINPACK-2's dcsrch as
+published (scipy's port), heap order, Gram definitions and projected
+Gauss-Seidel solutions of strictly convex box QPs.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_helpers as H
+from lbfgsb_amd import problems as pb
+
+RNG = np.random.default_rng(20241203)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ----------------------------------------------------------------- dense
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 16])
+def test_dpofa_reconstructs(oracle, n):
+    fl = oracle.load()[0]
+    A0 = RNG.standard_normal((n, n))
+    A = A0 @ A0.T + n * np.eye(n)
+    R = A.copy()
+    assert fl.dpofa(_ptr(R), n, n) == 1
+    R = np.triu(R)
+    assert np.allclose(R.T @ R, A, rtol=1e-13, atol=1e-13)
+    assert np.allclose(R, np.linalg.cholesky(A).T, rtol=1e-12, atol=1e-12)
+
+
+def test_dpofa_rejects_indefinite(oracle):
+    fl = oracle.load()[0]
+    A = np.array([[1.0, 2.0], [2.0, 1.0]])
+    assert fl.dpofa(_ptr(A), 2, 2) == 0
+
+
+@pytest.mark.parametrize("job", [1, 11])
+@pytest.mark.parametrize("n", [1, 4, 9, 16])
+def test_dtrsl_solves(oracle, job, n):
+    fl = oracle.load()[0]
+    T = np.triu(RNG.standard_normal((n, n))) + 3 * np.eye(n)
+    b = RNG.standard_normal(n)
+    x = b.copy()
+    assert fl.dtrsl(_ptr(T), n, n, _ptr(x), job) == 0
+    M = T if job == 1 else T.T
+    assert np.allclose(M @ x, b, rtol=1e-12, atol=1e-12)
+
+
+def test_dtrsl_zero_pivot(oracle):
+    fl = oracle.load()[0]
+    T = np.array([[1.0, 1.0], [0.0, 0.0]])
+    b = np.ones(2)
+    assert fl.dtrsl(_ptr(T), 2, 2, _ptr(b), 11) == 2
+
+
+def _memory(n, col, theta=1.7):
+    S = RNG.standard_normal((n, col))
+    A = RNG.standard_normal((n, n))
+    A = A @ A.T + n * np.eye(n)
+    Y = A @ S
+    return S, Y, theta
+
+
+@pytest.mark.parametrize("col", [1, 2, 5, 8])
+def test_formt_and_bmv_against_explicit_inverse(oracle, col):
+    """wt = chol(theta S'S + L D^-1 L') and bmv(v) = M v with the explicit
+    inverse of [[-D, L'], [L, theta S'S]]."""
+    fl = oracle.load()[0]
+    m = 8
+    S, Y, theta = _memory(30, col)
+    SY, SS = S.T @ Y, S.T @ S
+    st = fl.create(30, m)
+    s = st.contents
+    sy = np.ctypeslib.as_array(s.sy, shape=(m * m,))
+    ss = np.ctypeslib.as_array(s.ss, shape=(m * m,))
+    wt = np.ctypeslib.as_array(s.wt, shape=(m * m,))
+    for i in range(col):
+        for j in range(col):
+            sy[i * m + j] = SY[i, j]
+            ss[i * m + j] = SS[i, j]
+    s.col, s.theta, s.m = col, theta, m
+    fl.formt(st)
+    assert s.internalinfo == 0
+    D = np.diag(np.diag(SY))
+    L = np.tril(SY, -1)
+    Tm = theta * SS + L @ np.linalg.inv(D) @ L.T
+    R = np.triu(wt.reshape(m, m)[:col, :col])
+    assert np.allclose(R.T @ R, Tm, rtol=1e-11)
+    M = H.middle_matrix(S, Y, theta)
+    for _ in range(3):
+        v = RNG.standard_normal(2 * col)
+        p = np.zeros(2 * col)
+        assert fl.bmv(m, s.sy, s.wt, col, _ptr(v), _ptr(p)) == 0
+        assert np.allclose(p, M @ v, rtol=1e-9, atol=1e-11)
+    fl.destroy(st)
+
+
+def test_hpsolb_extracts_in_order(oracle):
+    fl = oracle.load()[0]
+    n = 97
+    t = RNG.random(n)
+    order = np.arange(1, n + 1, dtype=np.int32)
+    t0 = t.copy()
+    out = []
+    for it in range(n):
+        nleft = n - it
+        fl.hpsolb(nleft, _ptr(t), order.ctypes.data_as(C.POINTER(C.c_int)), it)
+        out.append((t[nleft - 1], order[nleft - 1]))
+        # remaining part keeps the heap property
+        h = t[: nleft - 1]
+        for i in range(1, len(h)):
+            assert h[(i - 1) // 2] <= h[i]
+    vals = [v for v, _ in out]
+    assert vals == sorted(t0)
+    assert all(t0[i - 1] == v for v, i in out)
+
+
+# ----------------------------------------------------------- line search
+def _scipy_dcsrch():
+    try:
+        from scipy.optimize._dcsrch import DCSRCH
+        return DCSRCH
+    except Exception:
+        return None
+
+
+PHIS = [
+    # (phi, dphi, stp0): smooth 1-D functions with phi'(0) < 0
+    (lambda a: (a - 2.0) ** 2, lambda a: 2 * (a - 2.0), 1.0),
+    (lambda a: -a / (a * a + 2.0), lambda a: (a * a - 2.0) / (a * a + 2.0) ** 2, 1.0),
+    (lambda a: (a + 0.004) ** 5 - 2 * (a + 0.004) ** 4,
+     lambda a: 5 * (a + 0.004) ** 4 - 8 * (a + 0.004) ** 3, 0.1),
+    (lambda a: np.exp(-3 * a) + 0.05 * a * a, lambda a: -3 * np.exp(-3 * a) + 0.1 * a, 10.0),
+    (lambda a: 1 - np.cos(a) - 0.3 * a, lambda a: np.sin(a) - 0.3, 30.0),
+]
+
+
+@pytest.mark.parametrize("k", range(len(PHIS)))
+def test_dcsrch_matches_minpack2(oracle, k):
+    """The oracle's dcsrch/dcstep against MINPACK-2 dcsrch as published
+    (scipy's line-by-line port), on the same 1-D functions: identical
+    step sequence and exit task."""
+    DCSRCH = _scipy_dcsrch()
+    if DCSRCH is None:
+        pytest.skip("scipy.optimize._dcsrch not available")
+    fl = oracle.load()[0]
+    phi, dphi, stp0 = PHIS[k]
+    ftol, gtol, xtol, stpmax = 1e-3, 0.9, 0.1, 1e10
+    ref = DCSRCH(phi, dphi, ftol, gtol, xtol, 0.0, stpmax)
+    stp_r, f_r, g_r, task_r = stp0, phi(0.0), dphi(0.0), b"START"
+    stp_o = C.c_double(stp0)
+    task_o = C.c_int(0)
+    isave = (C.c_int * 2)()
+    dsave = (C.c_double * 13)()
+    f_o, g_o = phi(0.0), dphi(0.0)
+    seq_r, seq_o = [], []
+    for _ in range(40):
+        stp_r, f_r, g_r, task_r = ref._iterate(stp_r, f_r, g_r, task_r)
+        fl.dcsrch(f_o, g_o, C.byref(stp_o), ftol, gtol, xtol, 0.0, stpmax, C.byref(task_o),
+                  isave, dsave)
+        seq_r.append(stp_r)
+        seq_o.append(stp_o.value)
+        if task_r[:2] != b"FG":
+            break
+        assert task_o.value == 1
+        f_r, g_r = phi(stp_r), dphi(stp_r)
+        f_o, g_o = phi(stp_o.value), dphi(stp_o.value)
+    assert np.allclose(seq_o, seq_r, rtol=1e-14, atol=0), (seq_o, seq_r)
+    code = {b"CONV": 4, b"WARN": 3, b"ERRO": 2}[task_r[:4]]
+    assert task_o.value == code
+    if code == 4:
+        a = seq_o[-1]
+        assert phi(a) <= phi(0) + ftol * a * dphi(0)
+        assert abs(dphi(a)) <= gtol * abs(dphi(0))
+
+
+# ------------------------------------------------- solver-state pins
+def _states(oracle, problem, dtype=np.float64, iters=(0, 1, 2, 3, 5, 8)):
+    """Oracle states at the top of the given iterations."""
+    s = oracle.Solver(problem, dtype=dtype, eps=1e-10, max_iter=200)
+    out = []
+    k = 0
+    while s.more and k <= max(iters):
+        if k in iters:
+            out.append((k, s.clone()))
+        s.step()
+        k += 1
+    return out
+
+
+def _small_problems():
+    return [pb.grid_cos(6, 5, seed=11), pb.penta_qp(40, seed=12),
+            pb.rosenbrock(30, seed=13), pb.grid_quadratic(6, 6, seed=14)]
+
+
+def _dense_state(s):
+    st = s.st
+    n, m = st.n, st.m
+    x = s.arr("x").astype(np.float64).copy()
+    g = s.arr("g").astype(np.float64).copy()
+    ws = s.arr("ws").reshape(n, m).astype(np.float64)
+    wy = s.arr("wy").reshape(n, m).astype(np.float64)
+    S, Y = H.ring_columns(ws, wy, st.head, st.col, m)
+    B = H.dense_B(S, Y, st.theta, n)
+    return x, g, B
+
+
+@pytest.mark.parametrize("pi", range(4))
+def test_cauchy_matches_bruteforce(oracle, pi):
+    p = _small_problems()[pi]
+    checked = 0
+    for k, s in _states(oracle, p):
+        st = s.st
+        if not st.cnstnd:
+            continue
+        x, g, B = _dense_state(s)
+        iw0 = s.arr("iwhere").copy()
+        nbd = p.nbd
+        xcp_ref, iw_ref, passed = H.cauchy_bruteforce(x, g, p.l, p.u, nbd, iw0, B)
+        s.fl.cauchy(s.s)
+        assert st.internalinfo == 0
+        xcp = s.arr("z").copy()
+        assert np.array_equal(s.arr("iwhere"), iw_ref), (k, pi)
+        # the reference's incremental f1/f2 recurrences accumulate rounding
+        # over the passed breakpoints (~1e-10 relative after 29 of them);
+        # the brute force evaluates the model exactly
+        assert np.allclose(xcp, xcp_ref, rtol=1e-8, atol=1e-12), (k, pi)
+        assert st.nint == passed + 1 or (passed == p.n and st.nint == p.n)
+        checked += 1
+    assert checked >= 3
+
+
+@pytest.mark.parametrize("pi", range(4))
+def test_freev_formk_subsm_match_dense_step(oracle, pi):
+    """freev index order by definition; the subspace step (formk + cmprlb +
+    subsm) equals the dense Newton step on the free variables."""
+    p = _small_problems()[pi]
+    checked = 0
+    for k, s in _states(oracle, p):
+        st = s.st
+        x, g, B = _dense_state(s)
+        n = st.n
+        old_index = s.arr("index").copy()
+        old_nfree = st.nfree
+        s.fl.cauchy(s.s)
+        xcp = s.arr("z").copy()
+        iw = s.arr("iwhere").copy()
+        s.fl.freev(s.s)
+        free = iw <= 0
+        idx = s.arr("index")
+        assert st.nfree == int(free.sum())
+        assert np.array_equal(idx[: st.nfree], np.where(free)[0] + 1)
+        assert np.array_equal(idx[st.nfree:], (np.where(~free)[0] + 1)[::-1])
+        if st.iter > 0 and st.cnstnd:
+            was_free = np.zeros(n, bool)
+            was_free[old_index[:old_nfree] - 1] = True
+            enter = np.where(~was_free & free)[0] + 1
+            leave = np.where(was_free & ~free)[0] + 1
+            ix2 = s.arr("indx2")
+            assert st.nenter == len(enter) and st.ileave == n + 1 - len(leave)
+            assert np.array_equal(ix2[: st.nenter], enter[::-1])
+            assert np.array_equal(ix2[st.ileave - 1:], leave[::-1])
+        if st.nfree == 0 or st.col == 0:
+            continue
+        if st.wrk:
+            s.fl.formk(s.s)
+        assert st.internalinfo == 0
+        s.fl.cmprlb(s.s)
+        s.fl.subsm(s.s)
+        assert st.internalinfo == 0
+        z_ref, alpha = H.subspace_dense(x, xcp, g, p.l, p.u, p.nbd, free, B)
+        z = s.arr("z")
+        assert np.allclose(z, z_ref, rtol=1e-8, atol=1e-10), (k, pi)
+        assert st.iword == (1 if alpha < 1 else 0)
+        checked += 1
+    assert checked >= 2
+
+
+@pytest.mark.parametrize("pi", range(4))
+def test_projgr_and_memory_match_definitions(oracle, pi):
+    p = _small_problems()[pi]
+    for k, s in _states(oracle, p, iters=(1, 3, 6, 9, 12)):
+        st = s.st
+        n, m, col = st.n, st.m, st.col
+        x = s.arr("x").astype(np.float64)
+        g = s.arr("g").astype(np.float64)
+        # projgr by definition
+        gp = g.copy()
+        lo = (p.nbd == 1) | (p.nbd == 2)
+        hi = (p.nbd == 2) | (p.nbd == 3)
+        neg = gp < 0
+        gp = np.where(neg & hi, np.maximum(x - p.u, gp), gp)
+        gp = np.where(~neg & lo, np.minimum(x - p.l, gp), gp)
+        assert st.sbgnrm == np.max(np.abs(gp))
+        if col == 0:
+            continue
+        ws = s.arr("ws").reshape(n, m)
+        wy = s.arr("wy").reshape(n, m)
+        S, Y = H.ring_columns(ws, wy, st.head, col, m)
+        sy = s.arr("sy").reshape(m, m)[:col, :col]
+        ss = s.arr("ss").reshape(m, m)[:col, :col]
+        SY, SS = S.T @ Y, S.T @ S
+        assert np.allclose(np.tril(sy), np.tril(SY), rtol=1e-10, atol=1e-13)
+        assert np.allclose(np.triu(ss), np.triu(SS), rtol=1e-10, atol=1e-13)
+        ynew, snew = Y[:, -1], S[:, -1]
+        assert np.isclose(st.theta, ynew @ ynew / (snew @ ynew), rtol=1e-10)
+
+
+# ----------------------------------------------------------- end to end
+@pytest.mark.parametrize("seed", [21, 22])
+def test_penta_qp_matches_projected_gauss_seidel(oracle, seed):
+    p = pb.penta_qp(3000, seed=seed)
+    xs = H.projected_gauss_seidel_penta(p)
+    r = oracle.minimize(p, eps=1e-13, max_iter=2000)
+    assert r["info"] in (1, 2, 4)
+    assert np.max(np.abs(r["x"] - xs)) < 1e-6
+    fs, _ = pb.objective_numpy(p, xs)
+    assert abs(r["f"] - fs) <= 1e-10 * abs(fs)
+
+
+def test_grid_quadratic_matches_projected_gauss_seidel(oracle):
+    p = pb.grid_quadratic(24, 20, seed=31)
+    xs = H.projected_gauss_seidel_grid(p)
+    r = oracle.minimize(p, eps=1e-14, max_iter=3000)
+    fs, _ = pb.objective_numpy(p, xs)
+    assert abs(r["f"] - fs) <= 1e-9 * abs(fs)
+    assert np.max(np.abs(r["x"] - xs)) < 1e-5
+
+
+def test_rosenbrock_reaches_interior_minimiser(oracle):
+    p = pb.rosenbrock(200, seed=41)
+    r = oracle.minimize(p, eps=1e-14, max_iter=5000)
+    assert r["f"] < 1e-9
+    assert np.max(np.abs(r["x"] - 1.0)) < 1e-4
+
+
+def test_kkt_at_termination(oracle):
+    p = pb.grid_cos(20, 20, seed=51)
+    r = oracle.minimize(p, eps=1e-12, max_iter=3000)
+    _, g = pb.objective_numpy(p, r["x"])
+    x = r["x"]
+    at_l = ((p.nbd == 1) | (p.nbd == 2)) & (x <= p.l)
+    at_u = ((p.nbd == 2) | (p.nbd == 3)) & (x >= p.u)
+    free = ~(at_l | at_u)
+    assert np.all(g[at_l] >= -1e-6) and np.all(g[at_u] <= 1e-6)
+    assert np.max(np.abs(g[free])) < 1e-5
+
+
+def test_objectives_match_numpy(oracle):
+    for p in [pb.grid_quadratic(9, 7), pb.grid_cos(8, 8), pb.rosenbrock(20), pb.penta_qp(33)]:
+        fl = oracle.load()[0]
+        obj = oracle.Objective(fl, p)
+        x = RNG.uniform(-0.5, 0.5, p.n)
+        f = C.c_double(0)
+        g = np.zeros(p.n)
+        assert fl.evaluate(C.byref(obj.c), p.n, _ptr(x), C.byref(f), _ptr(g)) == 0
+        fr, gr = pb.objective_numpy(p, x)
+        assert np.isclose(f.value, fr, rtol=1e-12)
+        assert np.allclose(g, gr, rtol=1e-12, atol=1e-14)
+
+
+# ------------------------------------------------------- failure paths
+def test_bad_input_codes(oracle):
+    p = pb.penta_qp(50)
+    q = pb.penta_qp(50)
+    q.nbd = q.nbd.copy()
+    q.nbd[7] = 5
+    assert oracle.minimize(q)["info"] == -1
+    q2 = pb.penta_qp(50)
+    q2.l = q2.l.copy()
+    q2.l[3] = q2.u[3] + 1
+    assert oracle.minimize(q2)["info"] == -1
+    # m > n
+    assert oracle.minimize(pb.penta_qp(5), m=8)["info"] == -1
+    assert oracle.minimize(p, max_iter=3)["info"] == 5
+
+
+def test_initial_projected_gradient_stop(oracle):
+    # x0 at the unconstrained minimiser of a separable quadratic
+    p = pb.penta_qp(20)
+    p.a1[:] = 0
+    p.a2[:] = 0
+    p.x0 = np.clip(p.b / p.a0, p.l, p.u)
+    p.b = p.a0 * p.x0
+    r = oracle.minimize(p, eps=1e-12)
+    assert r["info"] == 4 and r["iters"] == 0
