@@ -1,0 +1,197 @@
+"""L-BFGS-B benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-fp64]
+
+A step is one L-BFGS-B iteration (Cauchy point, subspace step, line search
+with its objective evaluations, memory update) of the plain solver on the
+mixed-precision workload: extended Rosenbrock, n = 16,777,216 (4096 x 4096),
+m = 8, bounds [-1.5, 2] / [-0.5, 1.5], x0 ~ U(-1.2, 1.2); n-length work in
+FP32 with FP64-accumulated reductions, 2m x 2m algebra and line search in
+FP64.  W warm-up iterations bring the memory to col = m; then exactly K
+iterations are timed between two host-synchronised status reads.  Inputs
+are resident in HBM before the timed region; the objective is evaluated by
+the library's device kernel through the funcgrad callback (host f, as in the
+reference API).  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [REPO, os.path.join(REPO, "lbfgsb-gpu_amd")]
+
+METRIC = "L-BFGS-B iters/s at n=16.8M fp32/fp64; batched solves/s + p99 at 1e-6 rel"
+HBM_PEAK_GBS = 8000.0
+
+# step ids of lbfgsb_engine_run / lbfgsb_engine_step_time
+STEPS = {"pass_a_fused": 4, "p4": 7, "formk_subsm": 8, "p5a": 9, "alpha": 10, "p5b": 11,
+         "ls_init": 12, "trial": 13, "gd": 14, "ls_step": 15, "gcp_rounds": 16,
+         "stop_tests": 5, "update_gcp": 6}
+
+
+def kernel_bytes_per_elem(s: int) -> dict:
+    """Algorithmic HBM bytes per element of each bulk pass at steady state
+    (col = m = 8, one memory update per iteration; nbd int32, iwhere 1 byte).
+      pass_a_fused: read x, x_old, g, g_old, l, u, d (7s), nbd, iwhere, the 14
+                    retained W entries (14s); write the new S/Y pair (2s),
+                    iwhere
+      p4:           read x, g, l, u (4s), nbd, iwhere, W (16s); write z, r
+                    (2s), iwhere
+      p5a:          read r, z, l, u (4s), nbd, iwhere, W (16s); write d_sub (s)
+      p5b:          read x, g, z, d_sub, l, u (6s), nbd, iwhere; write z, d (2s)
+      trial:        read z, write x_trial (2s, unit step)
+      gd:           read g, d (2s)"""
+    return {"pass_a_fused": 23 * s + 6, "p4": 22 * s + 6, "p5a": 21 * s + 5,
+            "p5b": 8 * s + 5, "trial": 2 * s, "gd": 2 * s}
+
+
+def b_alg_per_iter(n: int, s: int) -> float:
+    """Canonical compulsory bytes of one iteration, n (76 s + 32)."""
+    return float(n) * (76 * s + 32)
+
+
+def run_gpu(n: int, f64: bool, warmup: int, steps: int):
+    import numpy as np
+    import torch
+    from lbfgsb_amd import _lib, solver
+    from lbfgsb_amd import problems as pb
+
+    L = _lib.lib()
+    dtype = np.float64 if f64 else np.float32
+    p = pb.rosenbrock(n)
+    dp = solver.DeviceProblem.create(p, dtype)
+    x = dp.tensors["x0"].clone()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    t = dp.tensors
+    eng = L.lbfgsb_engine_create(1 if f64 else 0, n, 8, C.c_void_p(x.data_ptr()),
+                                 C.c_void_p(t["nbd"].data_ptr()), C.c_void_p(t["l"].data_ptr()),
+                                 C.c_void_p(t["u"].data_ptr()), stream)
+    if not eng:
+        raise RuntimeError("engine create failed")
+    L.lbfgsb_engine_set_profiling(eng, warmup)
+    opt = _lib.OptionsF64() if f64 else _lib.OptionsF32()
+    (L.lbfgsb_default_options_f64 if f64 else L.lbfgsb_default_options_f32)(C.byref(opt))
+    opt.eps_g = opt.eps_f = opt.eps_x = 0.0
+    opt.max_iteration = warmup + steps
+    cbs = _lib.CallbacksF64()
+    cbs.funcgrad = C.cast(L.lbfgsb_problem_funcgrad_f64 if f64 else
+                          L.lbfgsb_problem_funcgrad_f32, C.c_void_p)
+    cbs.funcgrad_ctx = C.c_void_p(dp.handle)
+    summ = _lib.SummaryF64() if f64 else _lib.SummaryF32()
+    cap = warmup + steps + 4
+    clock = (C.c_double * cap)()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solve = L.lbfgsb_engine_solve_f64 if f64 else L.lbfgsb_engine_solve_f32
+    solve(eng, C.byref(cbs), C.byref(opt), C.byref(summ), clock, cap)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    iters = summ.num_iteration
+    if iters < warmup + steps:
+        raise RuntimeError(f"solve stopped early: iters={iters} info={summ.info}")
+    timed = clock[warmup + steps - 1] - clock[warmup - 1]
+    kern = {}
+    for name, sid in STEPS.items():
+        ms, cnt = C.c_double(0), C.c_longlong(0)
+        L.lbfgsb_engine_step_time(eng, sid, C.byref(ms), C.byref(cnt))
+        kern[name] = (ms.value, cnt.value)
+    L.lbfgsb_engine_destroy(eng)
+    dp.close()
+    return dict(timed=timed, iters=iters, wall=wall, kern=kern, info=summ.info)
+
+
+def run_cpu_baseline(n_target: int, n_sample: int, warm: int = 9, timed: int = 3):
+    """The oracle (a 1-thread C restatement of the reference CPU path) on the
+    same generator at n_sample, steady-state iterations (col = m) timed and
+    the per-iteration cost scaled linearly to n_target."""
+    import numpy as np
+    from lbfgsb_amd import problems as pb
+    from oracle import pyoracle
+
+    p = pb.rosenbrock(n_sample)
+    s = pyoracle.Solver(p, dtype=np.float32, eps=0.0, max_iter=warm + timed + 2)
+    for _ in range(warm):
+        s.step()
+    t0 = time.perf_counter()
+    for _ in range(timed):
+        s.step()
+    per_iter = (time.perf_counter() - t0) / timed
+    scaled = per_iter * (n_target / n_sample)
+    return dict(value=1.0 / scaled, unit="iters/s", cores=1, kind="port",
+                sample=(f"oracle fp32, rosenbrock n={n_sample:,}, iterations "
+                        f"{warm + 1}-{warm + timed} (col=8) timed: {per_iter * 1e3:.1f} ms/iter, "
+                        f"scaled x{n_target / n_sample:g} to n={n_target:,}"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=4096 * 4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=2 ** 21)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(each solve runs on a single MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if a.warmup < 1 or a.steps < 1:
+        print("--warmup and --steps must be >= 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(2)
+    n = a.n
+    r32 = run_gpu(n, False, a.warmup, a.steps)
+    s = 4
+    value = a.steps / r32["timed"]
+    t_iter = r32["timed"] / a.steps
+    kb = kernel_bytes_per_elem(s)
+    # dominant bulk pass by measured time
+    dom = max(kb, key=lambda k: r32["kern"][k][0])
+    ms, cnt = r32["kern"][dom]
+    avg_s = ms / 1e3 / max(cnt, 1)
+    bytes_launch = kb[dom] * n
+    achieved = bytes_launch / avg_s / 1e9
+    out = {
+        "metric": METRIC, "value": value, "unit": "iters/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": t_iter * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"plain L-BFGS-B, extended Rosenbrock n={n:,} "
+                                f"(4096x4096 when default), m=8, fp32 bulk / fp64 small "
+                                f"dense + line search"), "n": n, "m": 8},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": bytes_launch,
+                     "avg_launch_us": avg_s * 1e6, "launches": cnt},
+        "iteration_roofline": {
+            "b_alg_bytes": b_alg_per_iter(n, s),
+            "achieved": b_alg_per_iter(n, s) / t_iter / 1e9, "unit": "GB/s",
+            "frac": b_alg_per_iter(n, s) / t_iter / 1e9 / HBM_PEAK_GBS},
+        "kernels_us": {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in r32["kern"].items()},
+        "kernel_launches": {k: v[1] for k, v in r32["kern"].items()},
+    }
+    if not a.no_fp64:
+        r64 = run_gpu(n, True, a.warmup, a.steps)
+        t64 = r64["timed"] / a.steps
+        out["fp64"] = {"value": a.steps / r64["timed"], "unit": "iters/s",
+                       "ms_per_step": t64 * 1e3,
+                       "iteration_roofline_frac": b_alg_per_iter(n, 8) / t64 / 1e9 / HBM_PEAK_GBS,
+                       "kernels_us": {k: round(v[0] * 1e3 / max(v[1], 1), 2)
+                                      for k, v in r64["kern"].items()}}
+    if not a.no_cpu:
+        out["cpu_baseline"] = run_cpu_baseline(n, min(a.cpu_sample, n))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

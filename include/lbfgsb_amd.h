@@ -123,6 +123,22 @@ int lbfgsb_engine_set_array(lbfgsb_engine* e, const char* name, const void* in,
                             long long count);
 /* Leave the accepted iterate in the caller's x buffer. */
 int lbfgsb_engine_finalize(lbfgsb_engine* e);
+/* A whole solve on an existing engine: the host loop of lbfgsb_minimize
+ * (lbfgsbcpu.cpp:117-465 control flow).  iter_clock (may be NULL) receives,
+ * for k < clock_cap, the host time in seconds at which the acceptance of
+ * iteration k+1 was observed. */
+int lbfgsb_engine_solve_f64(lbfgsb_engine* e, const lbfgsb_callbacks_f64* cb,
+                            const lbfgsb_options_f64* opt, lbfgsb_summary_f64* summary,
+                            double* iter_clock, int clock_cap);
+int lbfgsb_engine_solve_f32(lbfgsb_engine* e, const lbfgsb_callbacks_f32* cb,
+                            const lbfgsb_options_f32* opt, lbfgsb_summary_f32* summary,
+                            double* iter_clock, int clock_cap);
+/* Per-step kernel timing with HIP events on the engine's stream (no
+ * counterpart in the reference; used by the benchmark).  Launches are timed
+ * once the iteration count reaches from_iter (-1 = off; counters reset).
+ * step 16 = the sorted breakpoint rounds of the Cauchy search. */
+void lbfgsb_engine_set_profiling(lbfgsb_engine* e, int from_iter);
+int lbfgsb_engine_step_time(lbfgsb_engine* e, int step, double* total_ms, long long* count);
 
 /* ---- Built-in objectives (the callback side; no counterpart in the
  * reference library -- they restate its example callback shape,

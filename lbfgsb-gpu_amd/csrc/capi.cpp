@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
+#include <chrono>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -72,44 +73,27 @@ void fill_summary(const DevScalars& h, Summary* s) {
   s->info = h.info;
 }
 
-// Host driver of a plain solve (lbfgsbcpu.cpp:117-465 control flow; the
-// per-iteration decisions run on the device, see small.hip).
+double now_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// Host loop of a plain solve on an existing engine (lbfgsbcpu.cpp:117-465
+// control flow; the per-iteration decisions run on the device, see
+// small.hip).  iter_clock[k], if given, receives the host time at which the
+// acceptance of iteration k+1 was observed.
 template <typename R, typename Opt, typename Cb, typename Summary>
-int minimize_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x, const int* nbd,
-                  const R* l, const R* u, Summary* summary, void* stream) {
-  std::memset(summary, 0, sizeof(Summary));
-  if (!cb || !cb->funcgrad || !opt) {
-    summary->info = -1;
-    return -1;
-  }
-  if (opt->mode != LBFGSB_AMD_MODE_GPU) {
-    std::fprintf(stderr,
-                 "lbfgsb_amd: mode %d is not provided (this library implements the GPU mode "
-                 "only)\n",
-                 opt->mode);
-    summary->info = -1;
-    return -1;
-  }
-  const int m = opt->hessian_approximate_dimension < 8 ? opt->hessian_approximate_dimension : 8;
-  // errclb's scalar checks (lbfgsbcpu.cpp:879-890) and the driver's option
-  // checks (lbfgsbcpu.cpp:259-264)
-  if (n <= 0 || m <= 0 || m > n || opt->eps_f < 0 || opt->max_iteration < 0 ||
-      opt->eps_g < 0 || opt->eps_x < 0) {
-    summary->info = -1;
-    return -1;
-  }
-  hipStream_t st = (hipStream_t)stream;
-  EngineBase* e = make_engine(dtype, n, m, x, nbd, l, u, st);
-  if (!e) {
-    summary->info = -2;
-    return -2;
-  }
+int drive(EngineBase* e, const Cb* cb, const Opt* opt, Summary* summary, void* stream,
+          double* iter_clock, int clock_cap) {
   DevScalars& h = e->host_scalars();
+  if (e->read_scalars()) return -2;
+  const long long n = h.n;
+  const int m = h.m;
   std::memset(&h, 0, sizeof(DevScalars));
   h.n = n;
   h.m = m;
   h.theta = 1.0;
-  h.nfree = n;
+  h.nfree = (int)n;
   h.eps_g = (double)opt->eps_g;
   h.eps_f = (double)opt->eps_f;
   h.eps_x = (double)opt->eps_x;
@@ -117,6 +101,12 @@ int minimize_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x, const in
   h.has_after_iteration = cb->after_iteration != nullptr;
   h.has_custom_stop = cb->customized_stopping != nullptr;
   h.ibd = -1;
+  int seen_iter = 0;
+  auto clock_iter = [&]() {
+    if (!iter_clock) return;
+    const double t = now_seconds();
+    while (seen_iter < h.iter && seen_iter < clock_cap) iter_clock[seen_iter++] = t;
+  };
   int status = e->write_scalars();
   if (status == 0) status = e->setup_begin();
   R f = 0;
@@ -138,6 +128,7 @@ int minimize_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x, const in
     }
     status = e->run(prog, -1, K_END, fval);
     if (status < 0) break;
+    clock_iter();
     fill_summary(h, summary);
     if (status == ST_NEED_EVAL) {
       f = 0;
@@ -175,6 +166,44 @@ int minimize_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x, const in
   e->finalize_x();
   fill_summary(h, summary);
   summary->info = info;
+  return info;
+}
+
+template <typename R, typename Opt, typename Cb, typename Summary>
+int minimize_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x, const int* nbd,
+                  const R* l, const R* u, Summary* summary, void* stream) {
+  std::memset(summary, 0, sizeof(Summary));
+  if (!cb || !cb->funcgrad || !opt) {
+    summary->info = -1;
+    return -1;
+  }
+  if (opt->mode != LBFGSB_AMD_MODE_GPU) {
+    std::fprintf(stderr,
+                 "lbfgsb_amd: mode %d is not provided (this library implements the GPU mode "
+                 "only)\n",
+                 opt->mode);
+    summary->info = -1;
+    return -1;
+  }
+  const int m = opt->hessian_approximate_dimension < 8 ? opt->hessian_approximate_dimension : 8;
+  // errclb's scalar checks (lbfgsbcpu.cpp:879-890) and the driver's option
+  // checks (lbfgsbcpu.cpp:259-264)
+  if (n <= 0 || m <= 0 || m > n || opt->eps_f < 0 || opt->max_iteration < 0 ||
+      opt->eps_g < 0 || opt->eps_x < 0) {
+    summary->info = -1;
+    return -1;
+  }
+  EngineBase* e = make_engine(dtype, n, m, x, nbd, l, u, (hipStream_t)stream);
+  if (!e) {
+    summary->info = -2;
+    return -2;
+  }
+  DevScalars& h = e->host_scalars();
+  std::memset(&h, 0, sizeof(DevScalars));
+  h.n = n;
+  h.m = m;
+  e->write_scalars();
+  const int info = drive<R>(e, cb, opt, summary, stream, nullptr, 0);
   delete e;
   return info;
 }
@@ -315,6 +344,29 @@ int lbfgsb_engine_set_array(lbfgsb_engine* e, const char* name, const void* in,
   return e->e->set_array(name, in, count);
 }
 int lbfgsb_engine_finalize(lbfgsb_engine* e) { return e->e->finalize_x(); }
+
+int lbfgsb_engine_solve_f64(lbfgsb_engine* e, const lbfgsb_callbacks_f64* cb,
+                            const lbfgsb_options_f64* opt, lbfgsb_summary_f64* summary,
+                            double* iter_clock, int clock_cap) {
+  std::memset(summary, 0, sizeof(*summary));
+  if (!e || e->e->dtype() != 1 || !cb || !cb->funcgrad || !opt) return summary->info = -1;
+  return drive<double>(e->e, cb, opt, summary, (void*)e->e->stream(), iter_clock, clock_cap);
+}
+
+int lbfgsb_engine_solve_f32(lbfgsb_engine* e, const lbfgsb_callbacks_f32* cb,
+                            const lbfgsb_options_f32* opt, lbfgsb_summary_f32* summary,
+                            double* iter_clock, int clock_cap) {
+  std::memset(summary, 0, sizeof(*summary));
+  if (!e || e->e->dtype() != 0 || !cb || !cb->funcgrad || !opt) return summary->info = -1;
+  return drive<float>(e->e, cb, opt, summary, (void*)e->e->stream(), iter_clock, clock_cap);
+}
+
+void lbfgsb_engine_set_profiling(lbfgsb_engine* e, int from_iter) { e->e->set_profiling(from_iter); }
+
+int lbfgsb_engine_step_time(lbfgsb_engine* e, int step, double* total_ms, long long* count) {
+  e->e->step_time(step, total_ms, count);
+  return 0;
+}
 
 lbfgsb_problem* lbfgsb_problem_create(int dtype, int kind, long long n, long long nx,
                                       long long ny, double h, const void* b, const void* c,

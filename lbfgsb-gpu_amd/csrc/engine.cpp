@@ -76,6 +76,17 @@ struct Engine final : EngineBase {
   unsigned long long* exp_totals = nullptr;
   bool ok = false;
 
+  // ---- optional per-step timing (HIP events on st)
+  struct Pending {
+    int step;
+    hipEvent_t a, b;
+  };
+  int profile_from = -1;  // record kernels once the iteration count reaches this
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<Pending> pending;
+  double step_ms[K_END + 1] = {};
+  long long step_n[K_END + 1] = {};
+
   Engine(long long n_, int m_, T* x, const int* nbd_, const T* l_, const T* u_, hipStream_t s)
       : n(n_), m(m_), st(s), l(l_), u(u_), nbd(nbd_) {
     X[0] = x;
@@ -117,8 +128,54 @@ struct Engine final : EngineBase {
     return 0;
   }
 
+  hipEvent_t take_event() {
+    if (ev_pool.empty()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      return e;
+    }
+    hipEvent_t e = ev_pool.back();
+    ev_pool.pop_back();
+    return e;
+  }
+  // fold finished event pairs into the per-step totals (stream is idle)
+  void drain_events() {
+    for (const Pending& p : pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        step_ms[p.step] += ms;
+        step_n[p.step] += 1;
+      }
+      ev_pool.push_back(p.a);
+      ev_pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  void set_profiling(int from_iter) override {
+    hipStreamSynchronize(st);
+    drain_events();
+    profile_from = from_iter;
+    for (int i = 0; i <= K_END; i++) {
+      step_ms[i] = 0;
+      step_n[i] = 0;
+    }
+  }
+  void step_time(int step, double* ms, long long* count) override {
+    hipStreamSynchronize(st);
+    drain_events();
+    if (step < 0 || step > K_END) {
+      *ms = 0;
+      *count = 0;
+      return;
+    }
+    *ms = step_ms[step];
+    *count = step_n[step];
+  }
+
   ~Engine() override {
     hipStreamSynchronize(st);
+    drain_events();
+    for (hipEvent_t e : ev_pool) hipEventDestroy(e);
     T* bufs[] = {X[1], Gr[0], Gr[1], W, d, z, r, dsub, bp_t};
     for (T* p : bufs) hipFree(p);
     hipFree(bp_i);
@@ -154,6 +211,7 @@ struct Engine final : EngineBase {
   int read_scalars() override {
     HIPCHK(hipMemcpyAsync(hs, ds, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (!pending.empty()) drain_events();
     return 0;
   }
   int write_scalars() override {
@@ -193,6 +251,16 @@ struct Engine final : EngineBase {
   }
 
   void enqueue(int step, double f) {
+    if (profile_from < 0 || hs->iter < profile_from) return launch_step(step, f);
+    hipEvent_t a = take_event(), b = take_event();
+    if (!a || !b) return launch_step(step, f);
+    hipEventRecord(a, st);
+    launch_step(step, f);
+    hipEventRecord(b, st);
+    pending.push_back({step, a, b});
+  }
+
+  void launch_step(int step, double f) {
     switch (step) {
       case K_RESUME_RESET:
         launch_resume(st, ds, 1);
@@ -281,7 +349,18 @@ struct Engine final : EngineBase {
     HIPCHK(hipGetLastError());
     if (read_scalars()) return -1;
     if (hs->status == ST_GCP_ROUNDS) {
+      hipEvent_t ea = (profile_from >= 0 && hs->iter >= profile_from) ? take_event() : nullptr;
+      if (ea) hipEventRecord(ea, st);
       if (run_rounds()) return -1;
+      if (ea) {
+        hipEvent_t eb = take_event();
+        if (eb) {
+          hipEventRecord(eb, st);
+          pending.push_back({K_END, ea, eb});
+        } else {
+          ev_pool.push_back(ea);
+        }
+      }
       // resume the tail from the Cauchy point pass, up to stop_step
       size_t p = 0;
       while (p < steps.size() && steps[p] != K_P4) p++;

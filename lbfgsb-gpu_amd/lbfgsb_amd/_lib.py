@@ -78,6 +78,15 @@ SIGNATURES = {
     "lbfgsb_engine_get_array": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_longlong]),
     "lbfgsb_engine_set_array": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_longlong]),
     "lbfgsb_engine_finalize": (C.c_int, [C.c_void_p]),
+    "lbfgsb_engine_solve_f64": (C.c_int, [C.c_void_p, C.POINTER(CallbacksF64),
+                                          C.POINTER(OptionsF64), C.POINTER(SummaryF64),
+                                          C.c_void_p, C.c_int]),
+    "lbfgsb_engine_solve_f32": (C.c_int, [C.c_void_p, C.POINTER(CallbacksF32),
+                                          C.POINTER(OptionsF32), C.POINTER(SummaryF32),
+                                          C.c_void_p, C.c_int]),
+    "lbfgsb_engine_set_profiling": (None, [C.c_void_p, C.c_int]),
+    "lbfgsb_engine_step_time": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_longlong)]),
     "lbfgsb_problem_create": (C.c_void_p, [C.c_int, C.c_int, C.c_longlong, C.c_longlong,
                                            C.c_longlong, C.c_double, C.c_void_p, C.c_void_p,
                                            C.c_void_p, C.c_void_p, C.c_void_p]),
