@@ -30,8 +30,8 @@ HBM_PEAK_GBS = 8000.0
 
 # step ids of lbfgsb_engine_run / lbfgsb_engine_step_time
 STEPS = {"pass_a_fused": 4, "p4": 7, "formk_subsm": 8, "p5a": 9, "alpha": 10, "p5b": 11,
-         "ls_init": 12, "trial": 13, "gd": 14, "ls_step": 15, "gcp_rounds": 16,
-         "stop_tests": 5, "update_gcp": 6}
+         "ls_init": 12, "trial": 13, "gd": 14, "ls_step": 15, "gcp_rounds_host": 16,
+         "gcp_round": 17, "stop_tests": 5, "update_gcp": 6}
 
 
 def kernel_bytes_per_elem(s: int) -> dict:

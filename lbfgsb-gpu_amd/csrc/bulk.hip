@@ -185,9 +185,7 @@ __global__ void __launch_bounds__(kBlock)
 k_pass_a(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
          const T* __restrict__ u, const int* __restrict__ nbd,
          uint8_t* __restrict__ iwhere, T* __restrict__ W, const T* __restrict__ d,
-         T* __restrict__ bp_t, int* __restrict__ bp_i,
-         unsigned long long* __restrict__ bp_count, DevScalars* ds,
-         double* part, double* pmax, double* pmin, long long* pmini) {
+         DevScalars* ds, double* part, double* pmax, double* pmin, long long* pmini) {
   __shared__ double sh[kWavesPerBlock * PA_COUNT];
   if (ds->halt) return;
   const int cur = ds->phase & 1;
@@ -199,7 +197,6 @@ k_pass_a(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
   const bool upd = FUSED && ds->updatd;
   const T stp = (T)ds->stp;
   const bool stp1 = ds->stp == 1.0;
-  const int lane = threadIdx.x & 63;
 
   double acc[PA_COUNT];
 #pragma unroll
@@ -208,13 +205,11 @@ k_pass_a(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
   double bkmin = __builtin_inf();
   long long ibk = 0x7fffffffffffffffLL;
 
-  for (long long i0 = blockIdx.x * (long long)kBlock; i0 < n;
-       i0 += (long long)gridDim.x * kBlock) {
-    const long long i = i0 + threadIdx.x;
-    const bool in = i < n;
+  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
     bool isbrk = false;
     T tbrk = 0;
-    if (in) {
+    {
       const T xi = x[i], gi = g[i], li = l[i], ui = u[i];
       const int nb = nbd[i];
       const uint8_t iwb = iwhere[i];
@@ -286,19 +281,6 @@ k_pass_a(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
       }
       iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb));
     }
-    // wave-aggregated append of breakpoints (order is fixed later by sort)
-    const unsigned long long mask = __ballot(isbrk);
-    if (mask) {
-      const int cnt = __popcll(mask);
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(bp_count, (unsigned long long)cnt);
-      base = __shfl(base, 0, kWave);
-      const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
-      if (isbrk && slot < (unsigned long long)n) {
-        bp_t[slot] = tbrk;
-        bp_i[slot] = (int)i;
-      }
-    }
   }
   // PA_UNBND is an OR: keep it as a 0/1 value through the sum
   block_sum_store<PA_COUNT>(acc, sh, part + blockIdx.x, gridDim.x);
@@ -306,7 +288,7 @@ k_pass_a(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
   wave_argmin(bkmin, ibk);
   __shared__ double shv[kWavesPerBlock];
   __shared__ long long shi[kWavesPerBlock];
-  if (lane == 0) {
+  if ((threadIdx.x & 63) == 0) {
     shv[threadIdx.x >> 6] = bkmin;
     shi[threadIdx.x >> 6] = ibk;
   }
@@ -757,17 +739,14 @@ void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g,
 template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, T* const* X,
                    T* const* Gv, const T* l, const T* u, const int* nbd,
-                   uint8_t* iwhere, T* W, const T* d, T* bp_t, int* bp_i,
-                   unsigned long long* bp_count, DevScalars* ds, double* part,
+                   uint8_t* iwhere, T* W, const T* d, DevScalars* ds, double* part,
                    double* pmax, double* pmin, long long* pmini) {
   if (fused)
     hipLaunchKernelGGL((k_pass_a<T, true>), dim3(G), dim3(kBlock), 0, s, n, X, Gv,
-                       l, u, nbd, iwhere, W, d, bp_t, bp_i, bp_count, ds, part,
-                       pmax, pmin, pmini);
+                       l, u, nbd, iwhere, W, d, ds, part, pmax, pmin, pmini);
   else
     hipLaunchKernelGGL((k_pass_a<T, false>), dim3(G), dim3(kBlock), 0, s, n, X,
-                       Gv, l, u, nbd, iwhere, W, d, bp_t, bp_i, bp_count, ds,
-                       part, pmax, pmin, pmini);
+                       Gv, l, u, nbd, iwhere, W, d, ds, part, pmax, pmin, pmini);
 }
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, T* const* X, T* const* Gv,
@@ -812,8 +791,7 @@ void launch_gd(hipStream_t s, int G, long long n, T* const* Gv, const T* d,
                                  const T*, const T*, const int*, double*);          \
   template void launch_pass_a<T>(hipStream_t, int, bool, long long, T* const*,      \
                                  T* const*, const T*, const T*, const int*,         \
-                                 uint8_t*, T*, const T*, T*, int*,                  \
-                                 unsigned long long*, DevScalars*, double*,         \
+                                 uint8_t*, T*, const T*, DevScalars*, double*,      \
                                  double*, double*, long long*);                     \
   template void launch_p4<T>(hipStream_t, int, long long, T* const*, T* const*,     \
                              const T*, const T*, const int*, uint8_t*, const T*,    \

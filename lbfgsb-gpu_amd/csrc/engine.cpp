@@ -28,16 +28,16 @@ static std::vector<int> program_steps(int prog) {
   std::vector<int> s;
   switch (prog) {
     case PROG_ITER_START:
-      s = {K_RESUME_RESET, K_PASSA_SA, K_UPD_REDUCE};
+      s = {K_RESUME_RESET, K_PASSA_SA, K_UPD_REDUCE, K_GCP};
       break;
     case PROG_AFTER_EVAL:
-      s = {K_GD, K_LSSTEP, K_TRIAL, K_PASSA_FUSED, K_STOP, K_UPD};
+      s = {K_GD, K_LSSTEP, K_TRIAL, K_PASSA_FUSED, K_STOP, K_UPD, K_GCP};
       break;
     case PROG_AFTER_ACCEPT:
-      s = {K_RESUME_RESET, K_PASSA_FUSED, K_STOP, K_UPD};
+      s = {K_RESUME_RESET, K_PASSA_FUSED, K_STOP, K_UPD, K_GCP};
       break;
     case PROG_AFTER_CHECK:
-      s = {K_RESUME, K_UPD};
+      s = {K_RESUME, K_UPD, K_GCP};
       break;
     default:
       break;
@@ -59,15 +59,15 @@ struct Engine final : EngineBase {
   const T* l;
   const T* u;
   const int* nbd;
-  T *W = nullptr, *d = nullptr, *z = nullptr, *r = nullptr, *dsub = nullptr, *bp_t = nullptr;
-  int* bp_i = nullptr;
+  T *W = nullptr, *d = nullptr, *z = nullptr, *r = nullptr, *dsub = nullptr;
   uint8_t* iwhere = nullptr;
   DevScalars* ds = nullptr;
   DevScalars* hs = nullptr;
   double *part = nullptr, *pmax = nullptr, *pmin = nullptr;
   long long* pmini = nullptr;
   Cand *ca = nullptr, *cb = nullptr;
-  double *samp = nullptr, *tsum = nullptr, *tpre = nullptr, *total = nullptr;
+  double *tsum = nullptr, *tpre = nullptr, *total = nullptr;
+  int tstride = 0;
   double *tile_state = nullptr, *last_state = nullptr;
   long long* tile_stop = nullptr;
   int* index = nullptr;
@@ -84,8 +84,8 @@ struct Engine final : EngineBase {
   int profile_from = -1;  // record kernels once the iteration count reaches this
   std::vector<hipEvent_t> ev_pool;
   std::vector<Pending> pending;
-  double step_ms[K_END + 1] = {};
-  long long step_n[K_END + 1] = {};
+  double step_ms[kNumSteps] = {};
+  long long step_n[kNumSteps] = {};
 
   Engine(long long n_, int m_, T* x, const int* nbd_, const T* l_, const T* u_, hipStream_t s)
       : n(n_), m(m_), st(s), l(l_), u(u_), nbd(nbd_) {
@@ -106,7 +106,7 @@ struct Engine final : EngineBase {
     const size_t nn = (size_t)n;
     if (dmalloc(X[1], nn) || dmalloc(Gr[0], nn) || dmalloc(Gr[1], nn)) return -1;
     if (dmalloc(W, wsize(nn)) || dmalloc(d, nn) || dmalloc(z, nn) || dmalloc(r, nn) ||
-        dmalloc(dsub, nn) || dmalloc(bp_t, nn) || dmalloc(bp_i, nn) || dmalloc(iwhere, nn))
+        dmalloc(dsub, nn) || dmalloc(iwhere, nn))
       return -1;
     HIPCHK(hipMemsetAsync(W, 0, wsize(nn) * sizeof(T), st));
     if (dmalloc(dX, 2) || dmalloc(dG, 2) || dmalloc(ds, 1)) return -1;
@@ -116,7 +116,8 @@ struct Engine final : EngineBase {
     if (dmalloc(part, nv * G) || dmalloc(pmax, G) || dmalloc(pmin, G) || dmalloc(pmini, G))
       return -1;
     const size_t ntiles = nn / kBlock + 2;
-    if (dmalloc(ca, nn) || dmalloc(cb, nn) || dmalloc(samp, 2048) || dmalloc(tsum, 33 * ntiles) ||
+    tstride = (int)ntiles;
+    if (dmalloc(ca, nn) || dmalloc(cb, nn) || dmalloc(tsum, 33 * ntiles) ||
         dmalloc(tpre, 33 * ntiles) || dmalloc(total, 40) || dmalloc(tile_state, 33 * ntiles) ||
         dmalloc(last_state, 40) || dmalloc(tile_stop, ntiles))
       return -1;
@@ -155,7 +156,7 @@ struct Engine final : EngineBase {
     hipStreamSynchronize(st);
     drain_events();
     profile_from = from_iter;
-    for (int i = 0; i <= K_END; i++) {
+    for (int i = 0; i < kNumSteps; i++) {
       step_ms[i] = 0;
       step_n[i] = 0;
     }
@@ -163,7 +164,7 @@ struct Engine final : EngineBase {
   void step_time(int step, double* ms, long long* count) override {
     hipStreamSynchronize(st);
     drain_events();
-    if (step < 0 || step > K_END) {
+    if (step < 0 || step >= kNumSteps) {
       *ms = 0;
       *count = 0;
       return;
@@ -176,9 +177,8 @@ struct Engine final : EngineBase {
     hipStreamSynchronize(st);
     drain_events();
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
-    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, z, r, dsub, bp_t};
+    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, z, r, dsub};
     for (T* p : bufs) hipFree(p);
-    hipFree(bp_i);
     hipFree(iwhere);
     hipFree(dX);
     hipFree(dG);
@@ -190,7 +190,6 @@ struct Engine final : EngineBase {
     hipFree(pmini);
     hipFree(ca);
     hipFree(cb);
-    hipFree(samp);
     hipFree(tsum);
     hipFree(tpre);
     hipFree(total);
@@ -269,15 +268,19 @@ struct Engine final : EngineBase {
         launch_resume(st, ds, 0);
         break;
       case K_PASSA_SA:
-        launch_pass_a<T>(st, G, false, n, dX, dG, l, u, nbd, iwhere, W, d, bp_t, bp_i,
-                         &ds->bp_count, ds, part, pmax, pmin, pmini);
+        launch_pass_a<T>(st, G, false, n, dX, dG, l, u, nbd, iwhere, W, d, ds, part, pmax, pmin,
+                         pmini);
         break;
       case K_UPD_REDUCE:
         launch_update_gcp(st, ds, part, pmin, pmini, G, 1);
         break;
       case K_PASSA_FUSED:
-        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, nbd, iwhere, W, d, bp_t, bp_i,
-                         &ds->bp_count, ds, part, pmax, pmin, pmini);
+        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, nbd, iwhere, W, d, ds, part, pmax, pmin,
+                         pmini);
+        break;
+      case K_GCP:
+        launch_gcp_select<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, ca, n);
+        launch_gcp_round_small<T>(st, ca, dG, W, ds);
         break;
       case K_STOP:
         launch_stop_tests(st, ds, part, pmax, pmin, pmini, G);
@@ -317,21 +320,27 @@ struct Engine final : EngineBase {
     }
   }
 
-  // Sorted breakpoint rounds of the Cauchy search (gcp.hip).
+  // Host-sized breakpoint rounds of the Cauchy search (gcp.hip): used when a
+  // round holds more candidates than one workgroup sorts, and for the rounds
+  // after the first.  The candidates of the current round are already
+  // selected when this is entered.
   int run_rounds() {
-    launch_gcp_prepare<T>(st, ds, bp_t, samp, &ds->cand_count);
+    GcpScratch w{tsum, tpre, tstride, total, tile_stop, tile_state, last_state};
     for (int guard = 0; guard < 64; guard++) {
-      const long long nb = hs->gcp_nbreak;
-      launch_gcp_select<T>(st, G, bp_t, bp_i, nb, ds, ca, &ds->cand_count);
-      HIPCHK(hipGetLastError());
       if (read_scalars()) return -1;
+      if (!hs->gcp_pending) return 0;
       const long long K = (long long)hs->cand_count;
-      Cand* sorted = sort_candidates(st, ca, cb, K);
-      launch_gcp_scan<T>(st, sorted, K, dG, W, ds, tsum, tpre, total, tile_stop, tile_state,
-                         last_state, samp, &ds->cand_count);
+      // the previous kernel halted the chain: clear it for the round kernels
+      launch_resume(st, ds, 0);
+      launch_gcp_sortscan<T>(st, dG, W, ds, ca, cb, K, w);
       HIPCHK(hipGetLastError());
       if (read_scalars()) return -1;
-      if (hs->status != ST_GCP_ROUNDS) return 0;
+      if (!hs->gcp_pending) return 0;
+      if (hs->status != ST_GCP_ROUNDS) return -1;
+      // next round: select the next t-range (sorted at its size above)
+      launch_resume(st, ds, 0);
+      launch_gcp_select<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, ca, n);
+      HIPCHK(hipGetLastError());
     }
     return -1;
   }

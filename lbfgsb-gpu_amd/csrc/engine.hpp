@@ -33,8 +33,10 @@ enum Step : int {
   K_TRIAL,
   K_GD,
   K_LSSTEP,
-  K_END
+  K_END,        // also the timing slot of host-sized breakpoint rounds
+  K_GCP = 17    // device-sized breakpoint round (select + one-workgroup round)
 };
+constexpr int kNumSteps = 18;
 
 // Programs (sequences of steps).
 enum Program : int {

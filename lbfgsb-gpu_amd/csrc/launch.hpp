@@ -29,8 +29,8 @@ void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g, co
 template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, T* const* X, T* const* Gv,
                    const T* l, const T* u, const int* nbd, uint8_t* iwhere, T* W, const T* d,
-                   T* bp_t, int* bp_i, unsigned long long* bp_count, DevScalars* ds,
-                   double* part, double* pmax, double* pmin, long long* pmini);
+                   DevScalars* ds, double* part, double* pmax, double* pmin,
+                   long long* pmini);
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, T* const* X, T* const* Gv, const T* l,
                const T* u, const int* nbd, uint8_t* iwhere, const T* W, T* z, T* r,
@@ -67,17 +67,24 @@ void launch_ls_init(hipStream_t s, DevScalars* ds, const double* part, const dou
 void launch_ls_step(hipStream_t s, DevScalars* ds, const double* part, int G, double f);
 
 // gcp.hip
+struct GcpScratch {
+  double* tsum;
+  double* tpre;
+  int tstride;
+  double* total;
+  long long* tile_stop;
+  double* tile_state;
+  double* last_state;
+};
 template <typename T>
-void launch_gcp_prepare(hipStream_t s, DevScalars* ds, const T* bp_t, double* samp,
-                        unsigned long long* cnt);
+void launch_gcp_select(hipStream_t s, int G, long long n, T* const* X, T* const* Gv,
+                       const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
+                       DevScalars* ds, Cand* ca, long long buf_cap);
 template <typename T>
-void launch_gcp_select(hipStream_t s, int G, const T* bp_t, const int* bp_i, long long nb,
-                       const DevScalars* ds, Cand* cand, unsigned long long* cnt);
-Cand* sort_candidates(hipStream_t s, Cand* a, Cand* b, long long K);
+void launch_gcp_round_small(hipStream_t s, const Cand* ca, T* const* Gv, const T* W,
+                            DevScalars* ds);
 template <typename T>
-void launch_gcp_scan(hipStream_t s, const Cand* cand, long long K, T* const* Gv, const T* W,
-                     DevScalars* ds, double* tsum, double* tpre, double* total,
-                     long long* tile_stop, double* tile_state, double* last_state,
-                     const double* samp, unsigned long long* cnt);
+void launch_gcp_sortscan(hipStream_t s, T* const* Gv, const T* W, DevScalars* ds, Cand* ca,
+                         Cand* cb, long long cap, const GcpScratch& w);
 
 }  // namespace lbfgsb

@@ -2,8 +2,9 @@
 // sums of the bulk passes and run the scalar logic of the reference driver
 // on the device (lbfgsbcpu.cpp:279-462), so that a whole iteration can be
 // enqueued without a host round trip.  One launch = one block of 256
-// threads; partial reductions are spread over the four waves, the scalar
-// logic runs on thread 0.
+// threads.  Each kernel copies the device state block into LDS, runs the
+// scalar logic there (thread 0) and writes it back, so the dependent chains
+// of the small dense algebra see LDS rather than HBM latency.
 #include "common.hpp"
 #include "dense.hpp"
 #include "launch.hpp"
@@ -17,26 +18,53 @@ static constexpr double kMachEps = 5e-16;  // culbfgsb/ap.cpp:38, both precision
 static constexpr double kBig = 1.0e10;     // lbfgsbcpu.cpp:1259
 static constexpr double kFtol = 1.0e-3, kGtol = 0.9, kXtol = 0.1;
 
-// Compensated reduction of nv value rows (each G long) into out[0..nv).
+static_assert(sizeof(DevScalars) % 8 == 0, "DevScalars is copied as 8-byte words");
+constexpr int kDsWords = sizeof(DevScalars) / 8;
+
+__device__ void ds_load(DevScalars* sds, const DevScalars* ds) {
+  const unsigned long long* src = (const unsigned long long*)ds;
+  unsigned long long* dst = (unsigned long long*)sds;
+  for (int i = threadIdx.x; i < kDsWords; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+}
+__device__ void ds_store(DevScalars* ds, const DevScalars* sds) {
+  __syncthreads();
+  const unsigned long long* src = (const unsigned long long*)sds;
+  unsigned long long* dst = (unsigned long long*)ds;
+  for (int i = threadIdx.x; i < kDsWords; i += blockDim.x) dst[i] = src[i];
+}
+
+// Compensated reduction of value rows v0..v0+nv (each G partials long) into
+// out[0..nv): one wave per row, four independent accumulators per lane,
+// two-sum combination across accumulators and lanes.
 __device__ void reduce_rows(const double* part, int G, int v0, int nv, double* out) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int v = w; v < nv; v += kWavesPerBlock) {
     const double* row = part + (size_t)(v0 + v) * G;
-    double s = 0.0, c = 0.0;
-    for (int b = lane; b < G; b += kWave) {
-      double e;
-      two_sum(s, row[b], s, e);
-      c += e;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int b = lane;
+    for (; b + 3 * kWave < G; b += 4 * kWave) {
+      a0 += row[b];
+      a1 += row[b + kWave];
+      a2 += row[b + 2 * kWave];
+      a3 += row[b + 3 * kWave];
     }
+    for (; b < G; b += kWave) a0 += row[b];
+    double s, c = 0.0, e;
+    two_sum(a0, a1, s, e);
+    c += e;
+    two_sum(s, a2, s, e);
+    c += e;
+    two_sum(s, a3, s, e);
+    c += e;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-      double os = __shfl_xor(s, o, kWave), oc = __shfl_xor(c, o, kWave), e;
+      const double os = __shfl_xor(s, o, kWave), oc = __shfl_xor(c, o, kWave);
       two_sum(s, os, s, e);
       c += oc + e;
     }
     if (lane == 0) out[v] = s + c;
   }
-  __syncthreads();
 }
 
 __device__ double reduce_max(const double* pm, int G, double* sh) {
@@ -56,9 +84,11 @@ __device__ void reduce_argmin(const double* pv, const long long* pi, int G, doub
   double bv = __builtin_inf();
   long long bi = 0x7fffffffffffffffLL;
   for (int b = threadIdx.x; b < G; b += blockDim.x) {
-    if (pv[b] < bv || (pv[b] == bv && pi[b] < bi)) {
-      bv = pv[b];
-      bi = pi[b];
+    const double x = pv[b];
+    const long long k = pi[b];
+    if (x < bv || (x == bv && k < bi)) {
+      bv = x;
+      bi = k;
     }
   }
   wave_argmin(bv, bi);
@@ -119,12 +149,14 @@ __global__ void s_resume(DevScalars* ds, int reset_bp) {
 __global__ void s_setup_errclb(DevScalars* ds, const double* part, int G) {
   __shared__ double r[2];
   reduce_rows(part, G, 0, 2, r);
+  __syncthreads();
   if (threadIdx.x == 0 && (r[0] > 0 || r[1] > 0)) set_done(ds, -1);
 }
 
 __global__ void s_setup_active(DevScalars* ds, const double* part, int G) {
   __shared__ double r[3];
   reduce_rows(part, G, 0, 3, r);
+  __syncthreads();
   if (threadIdx.x == 0) {
     ds->prjctd = r[0] > 0;
     ds->cnstnd = r[1] > 0;
@@ -146,59 +178,11 @@ __global__ void s_setup_projgr(DevScalars* ds, const double* pmax, int G, double
   }
 }
 
-// Stopping tests after an accepted step (lbfgsbcpu.cpp:394-429).
-__global__ void s_stop_tests(DevScalars* ds, const double* part, const double* pmax,
-                             const double* pmin, const long long* pmini, int G) {
-  __shared__ double r[PA_COUNT];
-  __shared__ double sh[kWavesPerBlock];
-  __shared__ long long shi[kWavesPerBlock];
-  if (ds->halt) {
-    if (threadIdx.x == 0) ds->write_trial = 0;
-    return;
-  }
-  reduce_rows(part, G, 0, PA_COUNT, r);
-  if (threadIdx.x == 0) ds->write_trial = 0;
-  double sbg = reduce_max(pmax, G, sh);
-  double bk;
-  long long ibk;
-  reduce_argmin(pmin, pmini, G, sh, shi, bk, ibk);
-  if (threadIdx.x != 0) return;
-  for (int v = 0; v < PA_COUNT; v++) ds->pa[v] = r[v];
-  ds->gcp_bkmin = bk;
-  ds->gcp_ibkmin = ibk;
-  ds->sbgnrm = sbg;
-  ds->residual_g = sbg;
-  if (sbg <= ds->eps_g) return set_done(ds, 4);
-  double tf = sqrt(r[PA_TF]);
-  ds->tf = tf;
-  ds->residual_x = tf;
-  if (tf <= ds->eps_x) return set_done(ds, 2);
-  double ddum = dmax_(fabs(ds->fold), dmax_(fabs(ds->f), 1.0));
-  ds->ddum = ddum;
-  ds->residual_f = (ds->fold - ds->f) / ddum;
-  if (ds->fold - ds->f <= ds->eps_f * ddum) return set_done(ds, 1);
-  if (ds->iter > ds->max_iteration && ds->max_iteration > 0) return set_done(ds, 5);
-  if (ds->has_custom_stop) {
-    ds->status = ST_CHECKED;
-    ds->halt = 1;
-  }
-}
-
-// c <- M c for the reduced gradient (lbfgsbcpu.cpp:851-857); the same
-// factor already passed in the Cauchy search, so this cannot fail there.
-__device__ bool gcp_finish(DevScalars* ds) {
-  const int col = ds->col;
-  if (col > 0) {
-    if (d_bmv(ds->sy, ds->wt, col, ds->c, ds->mc) != 0) return false;
-  }
-  ds->nintol += ds->nint;
-  return true;
-}
-
 // Cauchy-search controller (lbfgsbcpu.cpp:620-724 and the first step of the
-// breakpoint loop :725-747).  Returns true if the sorted breakpoint rounds
-// are needed.
+// breakpoint loop :725-747).  Thread 0.  Sets gcp_pending when the sorted
+// breakpoint rounds must run.
 __device__ void gcp_controller(DevScalars* ds) {
+  ds->gcp_pending = 0;
   for (;;) {
     const int col = ds->col;
     ds->iword = -1;
@@ -207,7 +191,6 @@ __device__ void gcp_controller(DevScalars* ds) {
       ds->wrk = ds->updatd;
       ds->nint = 0;
       ds->gcp_mode = 2;
-      ds->status = ST_RUNNING;
       return;
     }
     const double theta = ds->theta;
@@ -225,11 +208,11 @@ __device__ void gcp_controller(DevScalars* ds) {
     if (nbreak == 0 && nfreegcp == 0) {
       // nothing moves: x_cp = x, c and nint keep their previous values
       ds->gcp_mode = 2;
-      if (!gcp_finish(ds)) {
+      if (col > 0 && d_bmv(ds->sy, ds->wt, col, ds->c, ds->mc) != 0) {
         reset_memory(ds);
         continue;
       }
-      ds->status = ST_RUNNING;
+      ds->nintol += ds->nint;
       return;
     }
     for (int j = 0; j < 2 * col; j++) ds->c[j] = 0.0;
@@ -256,24 +239,67 @@ __device__ void gcp_controller(DevScalars* ds) {
       ds->gcp_tsum = dtm;
       for (int j = 0; j < 2 * col; j++) ds->c[j] += dtm * ds->p[j];
       ds->gcp_mode = 0;
-      if (!gcp_finish(ds)) {
+      if (col > 0 && d_bmv(ds->sy, ds->wt, col, ds->c, ds->mc) != 0) {
         reset_memory(ds);
         continue;
       }
-      ds->status = ST_RUNNING;
+      ds->nintol += ds->nint;
       return;
     }
-    ds->status = ST_GCP_ROUNDS;
-    ds->halt = 1;
+    // at least the smallest breakpoint is passed: sorted rounds, the first
+    // over t in [0, 2 dtm) (dtm >= bkmin > 0 here)
+    ds->gcp_pending = 1;
+    ds->gcp_round = 0;
+    ds->gcp_floor_hits = 0;
+    for (int q = 0; q < 40; q++) ds->gcp_carry[q] = 0.0;
+    ds->gcp_tau_lo = -__builtin_inf();
+    const double t0 = 2.0 * dtm;
+    ds->gcp_tau_hi = t0 < 1e300 ? t0 : __builtin_inf();
+    ds->cand_count = 0;
     return;
   }
 }
 
-// matupd's small part + formt (lbfgsbcpu.cpp:1365-1386, 1105-1133), then
-// the next iteration's Cauchy controller.  `reduce` selects the standalone
-// path (iteration start / restart), which reduces the pass-A partials here.
-__global__ void s_update_gcp(DevScalars* ds, const double* part, const double* pmin,
-                             const long long* pmini, int G, int reduce) {
+// Columns M e_k of the middle matrix for the breakpoint rounds; threads
+// 0 .. 2col-1 (sy, wt in LDS).
+__device__ void build_middle(DevScalars* ds) {
+  const int col = ds->col;
+  if (ds->gcp_pending && (int)threadIdx.x < 2 * col) {
+    double e[16], out[16];
+    for (int i = 0; i < 16; i++) e[i] = 0.0;
+    e[threadIdx.x] = 1.0;
+    d_bmv(ds->sy, ds->wt, col, e, out);
+    for (int r = 0; r < 2 * col; r++) ds->Mx[r * 16 + threadIdx.x] = out[r];
+  }
+}
+
+// matupd's small part + formt (lbfgsbcpu.cpp:1365-1386, 1105-1133).
+__device__ void memory_update(DevScalars* ds) {
+  if (!ds->updatd) return;
+  const int m = ds->m, col = ds->col;
+  ds->rr = ds->pa[PA_RR];
+  ds->theta = ds->rr / ds->dr;
+  if (ds->iupdat > m) {
+    for (int j = 1; j <= col - 1; j++) {
+      for (int i = 1; i <= j; i++) A1(ds->ss, 8, i, j) = A1(ds->ss, 8, i + 1, j + 1);
+      for (int i = j; i <= col - 1; i++) A1(ds->sy, 8, i, j) = A1(ds->sy, 8, i + 1, j + 1);
+    }
+  }
+  for (int j = 1; j <= col - 1; j++) {
+    A1(ds->sy, 8, col, j) = ds->pa[PA_SY + j - 1];
+    A1(ds->ss, 8, j, col) = ds->pa[PA_SS + j - 1];
+  }
+  A1(ds->ss, 8, col, col) = ds->stp == 1.0 ? ds->dtd : ds->stp * ds->stp * ds->dtd;
+  A1(ds->sy, 8, col, col) = ds->dr;
+  if (!d_formt(ds->wt, ds->sy, ds->ss, col, ds->theta)) reset_memory(ds);
+}
+
+// Stopping tests after an accepted step (lbfgsbcpu.cpp:394-429); without a
+// customised stopping callback this kernel also runs the memory update and
+// the next Cauchy controller.
+__global__ void s_stop_tests(DevScalars* ds, const double* part, const double* pmax,
+                             const double* pmin, const long long* pmini, int G) {
+  __shared__ DevScalars sds;
   __shared__ double r[PA_COUNT];
   __shared__ double sh[kWavesPerBlock];
   __shared__ long long shi[kWavesPerBlock];
@@ -281,38 +307,89 @@ __global__ void s_update_gcp(DevScalars* ds, const double* part, const double* p
     if (threadIdx.x == 0) ds->write_trial = 0;
     return;
   }
-  if (reduce) {
-    reduce_rows(part, G, 0, PA_COUNT, r);
-    double bk;
-    long long ibk;
-    reduce_argmin(pmin, pmini, G, sh, shi, bk, ibk);
-    if (threadIdx.x == 0) {
-      for (int v = 0; v < PA_COUNT; v++) ds->pa[v] = r[v];
-      ds->gcp_bkmin = bk;
-      ds->gcp_ibkmin = ibk;
-    }
-  }
-  if (threadIdx.x != 0) return;
-  ds->write_trial = 0;
-  if (!reduce && ds->updatd) {
-    const int m = ds->m, col = ds->col;
-    ds->rr = ds->pa[PA_RR];
-    ds->theta = ds->rr / ds->dr;
-    if (ds->iupdat > m) {
-      for (int j = 1; j <= col - 1; j++) {
-        for (int i = 1; i <= j; i++) A1(ds->ss, 8, i, j) = A1(ds->ss, 8, i + 1, j + 1);
-        for (int i = j; i <= col - 1; i++) A1(ds->sy, 8, i, j) = A1(ds->sy, 8, i + 1, j + 1);
+  reduce_rows(part, G, 0, PA_COUNT, r);
+  const double sbg = reduce_max(pmax, G, sh);
+  double bk;
+  long long ibk;
+  reduce_argmin(pmin, pmini, G, sh, shi, bk, ibk);
+  ds_load(&sds, ds);
+  DevScalars* s = &sds;
+  if (threadIdx.x == 0) {
+    s->write_trial = 0;
+    s->upd_done = 0;
+    for (int v = 0; v < PA_COUNT; v++) s->pa[v] = r[v];
+    s->gcp_bkmin = bk;
+    s->gcp_ibkmin = ibk;
+    s->sbgnrm = sbg;
+    s->residual_g = sbg;
+    const double tf = sqrt(r[PA_TF]);
+    const double ddum = dmax_(fabs(s->fold), dmax_(fabs(s->f), 1.0));
+    if (sbg <= s->eps_g) {
+      set_done(s, 4);
+    } else {
+      s->tf = tf;
+      s->residual_x = tf;
+      if (tf <= s->eps_x) {
+        set_done(s, 2);
+      } else {
+        s->ddum = ddum;
+        s->residual_f = (s->fold - s->f) / ddum;
+        if (s->fold - s->f <= s->eps_f * ddum) {
+          set_done(s, 1);
+        } else if (s->iter > s->max_iteration && s->max_iteration > 0) {
+          set_done(s, 5);
+        } else if (s->has_custom_stop) {
+          s->status = ST_CHECKED;
+          s->halt = 1;
+        } else {
+          memory_update(s);
+          gcp_controller(s);
+          s->upd_done = 1;
+        }
       }
     }
-    for (int j = 1; j <= col - 1; j++) {
-      A1(ds->sy, 8, col, j) = ds->pa[PA_SY + j - 1];
-      A1(ds->ss, 8, j, col) = ds->pa[PA_SS + j - 1];
-    }
-    A1(ds->ss, 8, col, col) = ds->stp == 1.0 ? ds->dtd : ds->stp * ds->stp * ds->dtd;
-    A1(ds->sy, 8, col, col) = ds->dr;
-    if (!d_formt(ds->wt, ds->sy, ds->ss, col, ds->theta)) reset_memory(ds);
   }
-  gcp_controller(ds);
+  __syncthreads();
+  build_middle(s);
+  ds_store(ds, s);
+}
+
+// Memory update + Cauchy controller.  `reduce` selects the standalone path
+// (iteration start / restart) that reduces the classification partials here.
+__global__ void s_update_gcp(DevScalars* ds, const double* part, const double* pmin,
+                             const long long* pmini, int G, int reduce) {
+  __shared__ DevScalars sds;
+  __shared__ double r[PA_COUNT];
+  __shared__ double sh[kWavesPerBlock];
+  __shared__ long long shi[kWavesPerBlock];
+  if (ds->halt) {
+    if (threadIdx.x == 0) ds->write_trial = 0;
+    return;
+  }
+  if (!reduce && ds->upd_done) return;
+  double bk = 0.0;
+  long long ibk = 0;
+  if (reduce) {
+    reduce_rows(part, G, 0, PA_COUNT, r);
+    reduce_argmin(pmin, pmini, G, sh, shi, bk, ibk);
+  }
+  ds_load(&sds, ds);
+  DevScalars* s = &sds;
+  if (threadIdx.x == 0) {
+    s->write_trial = 0;
+    if (reduce) {
+      for (int v = 0; v < PA_COUNT; v++) s->pa[v] = r[v];
+      s->gcp_bkmin = bk;
+      s->gcp_ibkmin = ibk;
+    } else {
+      memory_update(s);
+    }
+    gcp_controller(s);
+    s->upd_done = 1;
+  }
+  __syncthreads();
+  build_middle(s);
+  ds_store(ds, s);
 }
 
 // formk (lbfgsbcpu.cpp:907-1103) from the inner products of k_p4.
@@ -341,20 +418,22 @@ __device__ bool formk_device(DevScalars* ds, const double* nc, const double* cor
   } else {
     upcl = col;
   }
-  for (int iy = 1; iy <= upcl; iy++) {
-    for (int jy = 1; jy <= iy; jy++) {
-      const int q = (iy - 1) * iy / 2 + (jy - 1);
-      A1(wn1, 16, iy, jy) = A1(wn1, 16, iy, jy) + corr[q];
-      A1(wn1, 16, m + iy, m + jy) = A1(wn1, 16, m + iy, m + jy) - corr[36 + q];
+  if (corr) {
+    for (int iy = 1; iy <= upcl; iy++) {
+      for (int jy = 1; jy <= iy; jy++) {
+        const int q = (iy - 1) * iy / 2 + (jy - 1);
+        A1(wn1, 16, iy, jy) = A1(wn1, 16, iy, jy) + corr[q];
+        A1(wn1, 16, m + iy, m + jy) = A1(wn1, 16, m + iy, m + jy) - corr[36 + q];
+      }
     }
-  }
-  for (int i = 1; i <= upcl; i++) {
-    for (int jy = 1; jy <= upcl; jy++) {
-      const double v = corr[72 + (i - 1) * 8 + (jy - 1)];
-      if (i <= jy)
-        A1(wn1, 16, m + i, jy) = A1(wn1, 16, m + i, jy) + v;
-      else
-        A1(wn1, 16, m + i, jy) = A1(wn1, 16, m + i, jy) - v;
+    for (int i = 1; i <= upcl; i++) {
+      for (int jy = 1; jy <= upcl; jy++) {
+        const double v = corr[72 + (i - 1) * 8 + (jy - 1)];
+        if (i <= jy)
+          A1(wn1, 16, m + i, jy) = A1(wn1, 16, m + i, jy) + v;
+        else
+          A1(wn1, 16, m + i, jy) = A1(wn1, 16, m + i, jy) - v;
+      }
     }
   }
   for (int iy = 1; iy <= col; iy++) {
@@ -398,50 +477,66 @@ __device__ bool formk_device(DevScalars* ds, const double* nc, const double* cor
 // After k_p4: free/active bookkeeping (lbfgsbcpu.cpp:1143-1161), formk,
 // and the two triangular solves of subsm (lbfgsbcpu.cpp:1459-1471).
 __global__ void s_formk_subsm(DevScalars* ds, const double* part, int G) {
+  __shared__ DevScalars sds;
   __shared__ double r[P4_COUNT];
+  __shared__ int flags[2];
   if (ds->halt) return;
-  reduce_rows(part, G, 0, P4_COUNT, r);
-  if (threadIdx.x != 0) return;
-  const long long n = ds->n;
-  const int col = ds->col;
-  if (!ds->shortcut) {
-    ds->nfree = (int)r[P4_NFREE];
-    if (ds->iter > 0 && ds->cnstnd) {
-      ds->nenter = (int)r[P4_NENTER];
-      ds->nleave = (int)r[P4_NLEAVE];
-    } else {
-      ds->nenter = 0;
-      ds->nleave = 0;
+  // counts first; the other rows only when they are used
+  reduce_rows(part, G, 0, 3, r);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const bool shortcut = ds->shortcut;
+    const bool track = ds->iter > 0 && ds->cnstnd && !shortcut;
+    flags[0] = track && (r[P4_NENTER] > 0 || r[P4_NLEAVE] > 0);
+    flags[1] = ds->updatd;
+  }
+  __syncthreads();
+  if (flags[1]) reduce_rows(part, G, P4_NC, 32, r + P4_NC);
+  reduce_rows(part, G, P4_WV, 16, r + P4_WV);
+  if (flags[0]) reduce_rows(part, G, P4_CORR, 136, r + P4_CORR);
+  ds_load(&sds, ds);
+  DevScalars* s = &sds;
+  if (threadIdx.x == 0) {
+    const long long n = s->n;
+    const int col = s->col;
+    if (!s->shortcut) {
+      s->nfree = (int)r[P4_NFREE];
+      if (s->iter > 0 && s->cnstnd) {
+        s->nenter = (int)r[P4_NENTER];
+        s->nleave = (int)r[P4_NLEAVE];
+      } else {
+        s->nenter = 0;
+        s->nleave = 0;
+      }
+      s->ileave = (int)(n + 1 - s->nleave);
+      s->wrk = s->nleave > 0 || s->nenter > 0 || s->updatd;
     }
-    ds->ileave = (int)(n + 1 - ds->nleave);
-    ds->wrk = ds->nleave > 0 || ds->nenter > 0 || ds->updatd;
-  }
-  ds->skip_subsm = (ds->nfree == 0 || col == 0);
-  if (ds->skip_subsm) return;
-  if (ds->wrk) {
-    if (!formk_device(ds, r + P4_NC, r + P4_CORR)) {
-      reset_memory(ds);
-      ds->status = ST_RESTART;
-      ds->halt = 1;
-      return;
+    s->skip_subsm = (s->nfree == 0 || col == 0);
+    bool fail = false;
+    if (!s->skip_subsm) {
+      if (s->wrk) fail = !formk_device(s, r + P4_NC, flags[0] ? r + P4_CORR : nullptr);
+      if (!fail) {
+        const double theta = s->theta;
+        double* wv = s->wv;
+        for (int i = 0; i < col; i++) {
+          wv[i] = r[P4_WV + i];
+          wv[col + i] = theta * r[P4_WV + 8 + i];
+        }
+        int info = d_dtrsl(s->wn, 16, 2 * col, wv, 11);
+        if (info == 0) {
+          for (int i = 0; i < col; i++) wv[i] = -wv[i];
+          info = d_dtrsl(s->wn, 16, 2 * col, wv, 1);
+        }
+        fail = info != 0;
+      }
+      if (fail) {
+        reset_memory(s);
+        s->status = ST_RESTART;
+        s->halt = 1;
+      }
     }
   }
-  const double theta = ds->theta;
-  double* wv = ds->wv;
-  for (int i = 0; i < col; i++) {
-    wv[i] = r[P4_WV + i];
-    wv[col + i] = theta * r[P4_WV + 8 + i];
-  }
-  int info = d_dtrsl(ds->wn, 16, 2 * col, wv, 11);
-  if (info == 0) {
-    for (int i = 0; i < col; i++) wv[i] = -wv[i];
-    info = d_dtrsl(ds->wn, 16, 2 * col, wv, 1);
-  }
-  if (info != 0) {
-    reset_memory(ds);
-    ds->status = ST_RESTART;
-    ds->halt = 1;
-  }
+  ds_store(ds, s);
 }
 
 // Backtrack length of subsm (lbfgsbcpu.cpp:1484-1538).
@@ -532,76 +627,87 @@ __device__ void ls_failure(DevScalars* ds) {
 
 // First call of lnsrlb for an iteration (lbfgsbcpu.cpp:1263-1339).
 __global__ void s_ls_init(DevScalars* ds, const double* part, const double* pmin, int G) {
+  __shared__ DevScalars sds;
   __shared__ double r[P5_COUNT];
   __shared__ double sh[kWavesPerBlock];
   if (ds->halt) return;
   reduce_rows(part, G, 0, P5_COUNT, r);
   double smin;
   reduce_min(pmin, G, sh, smin);
-  if (threadIdx.x != 0) return;
-  ds->dtd = r[P5_DTD];
-  ds->dnrm = sqrt(ds->dtd);
-  ds->stpmx = kBig;
-  if (ds->cnstnd) {
-    if (ds->iter == 0)
-      ds->stpmx = 1.0;
+  ds_load(&sds, ds);
+  DevScalars* s = &sds;
+  if (threadIdx.x == 0) {
+    s->dtd = r[P5_DTD];
+    s->dnrm = sqrt(s->dtd);
+    s->stpmx = kBig;
+    if (s->cnstnd) {
+      if (s->iter == 0)
+        s->stpmx = 1.0;
+      else
+        s->stpmx = dmin_(kBig, smin);
+    }
+    if (s->iter == 0 && !s->boxed)
+      s->stp = dmin_(1.0 / s->dnrm, s->stpmx);
     else
-      ds->stpmx = dmin_(kBig, smin);
+      s->stp = 1.0;
+    s->fold = s->f;
+    s->ifun = 0;
+    s->iback = 0;
+    s->csave = 0;
+    s->gd = r[P5_GD];
+    s->gdold = s->gd;
+    if (s->gd >= 0) {
+      s->internalinfo = -4;
+      ls_failure(s);
+    } else {
+      d_dcsrch(s->f, s->gd, s->stp, kFtol, kGtol, kXtol, 0.0, s->stpmx, s->csave, s->isave,
+               s->dsave);
+      s->xstep = s->stp * s->dnrm;
+      if (s->csave != 4 && s->csave != 3) {
+        request_trial(s);
+        s->status = ST_NEED_EVAL;
+        s->halt = 1;
+      } else {
+        s->task = 5;
+        accept_step(s);
+      }
+    }
   }
-  if (ds->iter == 0 && !ds->boxed)
-    ds->stp = dmin_(1.0 / ds->dnrm, ds->stpmx);
-  else
-    ds->stp = 1.0;
-  ds->fold = ds->f;
-  ds->ifun = 0;
-  ds->iback = 0;
-  ds->csave = 0;
-  ds->gd = r[P5_GD];
-  ds->gdold = ds->gd;
-  if (ds->gd >= 0) {
-    ds->internalinfo = -4;
-    return ls_failure(ds);
-  }
-  d_dcsrch(ds->f, ds->gd, ds->stp, kFtol, kGtol, kXtol, 0.0, ds->stpmx, ds->csave, ds->isave,
-           ds->dsave);
-  ds->xstep = ds->stp * ds->dnrm;
-  if (ds->csave != 4 && ds->csave != 3) {
-    request_trial(ds);
-    ds->status = ST_NEED_EVAL;
-    ds->halt = 1;
-  } else {
-    ds->task = 5;
-    accept_step(ds);
-  }
+  ds_store(ds, s);
 }
 
 // Later calls of lnsrlb, after an evaluation with value f
 // (lbfgsbcpu.cpp:1312-1339 and the loop exit at :352).
 __global__ void s_ls_step(DevScalars* ds, const double* part, int G, double f) {
+  __shared__ DevScalars sds;
   __shared__ double r[1];
   reduce_rows(part, G, 0, 1, r);
-  if (threadIdx.x != 0) return;
-  ds->halt = 0;
-  ds->status = ST_RUNNING;
-  ds->f = f;
-  ds->gd = r[0];
-  ds->write_trial = 0;
-  d_dcsrch(ds->f, ds->gd, ds->stp, kFtol, kGtol, kXtol, 0.0, ds->stpmx, ds->csave, ds->isave,
-           ds->dsave);
-  ds->xstep = ds->stp * ds->dnrm;
-  if (ds->csave != 4 && ds->csave != 3) {
-    request_trial(ds);
-    if (ds->iback >= 20) {
-      // the reference leaves the loop before evaluating this trial point
-      accept_step(ds);
+  ds_load(&sds, ds);
+  DevScalars* s = &sds;
+  if (threadIdx.x == 0) {
+    s->halt = 0;
+    s->status = ST_RUNNING;
+    s->f = f;
+    s->gd = r[0];
+    s->write_trial = 0;
+    d_dcsrch(s->f, s->gd, s->stp, kFtol, kGtol, kXtol, 0.0, s->stpmx, s->csave, s->isave,
+             s->dsave);
+    s->xstep = s->stp * s->dnrm;
+    if (s->csave != 4 && s->csave != 3) {
+      request_trial(s);
+      if (s->iback >= 20) {
+        // the reference leaves the loop before evaluating this trial point
+        accept_step(s);
+      } else {
+        s->status = ST_NEED_EVAL;
+        s->halt = 1;
+      }
     } else {
-      ds->status = ST_NEED_EVAL;
-      ds->halt = 1;
+      s->task = 5;
+      accept_step(s);
     }
-  } else {
-    ds->task = 5;
-    accept_step(ds);
   }
+  ds_store(ds, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -123,6 +123,9 @@ struct DevScalars {
   // ---- append counters (breakpoint list, round candidates)
   unsigned long long bp_count;
   unsigned long long cand_count;
+
+  int gcp_pending;   // sorted breakpoint rounds are in progress
+  int upd_done;      // the memory update of this iteration already ran
 };
 
 }  // namespace lbfgsb
