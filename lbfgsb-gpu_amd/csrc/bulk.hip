@@ -331,9 +331,7 @@ k_p4(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
      uint8_t* __restrict__ iwhere, const T* __restrict__ W, T* __restrict__ z,
      T* __restrict__ r, DevScalars* ds, double* part) {
   __shared__ double sh[kWavesPerBlock * 52];
-  __shared__ long long s_list[kBlock];
-  __shared__ signed char s_sign[kBlock];
-  __shared__ int s_wcnt[kWavesPerBlock];
+  __shared__ double s_corr[kWavesPerBlock][136];
   if (ds->halt) return;
   const int cur = ds->phase & 1;
   const T* __restrict__ x = X[cur];
@@ -362,15 +360,24 @@ k_p4(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
   double acc[51];
 #pragma unroll
   for (int v = 0; v < 51; v++) acc[v] = 0.0;
-  // one signed Gram pair per thread (threads < 136)
-  double corr = 0.0;
-  int ccat = 0, ca = 0, cb = 0;
-  const bool cthread = threadIdx.x < 136;
-  if (cthread) corr_pair(threadIdx.x, ccat, ca, cb);
-  const bool cactive = cthread && ca < upcl && cb < upcl;
-  int cslot_a = head + ca, cslot_b = head + cb;
-  if (cslot_a >= m) cslot_a -= m;
-  if (cslot_b >= m) cslot_b -= m;
+  // signed Gram pairs over entering (+1) / leaving (-1) variables: lane L of
+  // every wave owns pairs L, L+64, L+128 (< 136); wa/wb are the W columns
+  // (0..7 = Y slots, 8..15 = S slots) of the pair's two factors
+  double corr[3] = {0.0, 0.0, 0.0};
+  int wa[3], wb[3];
+  bool pact[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const int t = lane + 64 * q;
+    int cat = 0, a = 0, b = 0;
+    if (t < 136) corr_pair(t, cat, a, b);
+    pact[q] = t < 136 && a < upcl && b < upcl;
+    int sa = head + a, sb = head + b;
+    if (sa >= m) sa -= m;
+    if (sb >= m) sb -= m;
+    wa[q] = cat == 0 ? sa : 8 + sa;
+    wb[q] = cat == 1 ? 8 + sb : sb;
+  }
 
   for (long long i0 = blockIdx.x * (long long)kBlock; i0 < n;
        i0 += (long long)gridDim.x * kBlock) {
@@ -481,47 +488,39 @@ k_p4(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
         }
       }
     }
-    // cooperative signed Gram over entering (+) / leaving (-) variables
+    // signed Gram over this wave's entering / leaving variables, one
+    // flagged variable at a time in lane order (no block barrier)
     if (track && col > 0) {
-      const unsigned long long mask = __ballot(flag != 0);
-      if (lane == 0) s_wcnt[wid] = __popcll(mask);
-      __syncthreads();
-      int total = 0, before = 0;
+      unsigned long long fm = __ballot(flag != 0);
+      if (fm && lane == 0) ds->p4_flagged = 1;
+      while (fm) {
+        const int src = __ffsll((long long)fm) - 1;
+        fm &= fm - 1ull;
+        const long long e = __shfl(i, src, kWave);
+        const double sg = (double)__shfl(flag, src, kWave);
+        const double wrow = lane < 16 ? (double)W[widx(e, lane)] : 0.0;
 #pragma unroll
-      for (int w = 0; w < kWavesPerBlock; w++) {
-        if (w < wid) before += s_wcnt[w];
-        total += s_wcnt[w];
-      }
-      if (flag != 0) {
-        const int pos = before + __popcll(mask & ((1ull << lane) - 1ull));
-        s_list[pos] = i;
-        s_sign[pos] = (signed char)flag;
-      }
-      __syncthreads();
-      if (cactive) {
-        for (int q = 0; q < total; q++) {
-          const long long e = s_list[q];
-          const double sg = (double)s_sign[q];
-          double va, vb;
-          if (ccat == 0) {
-            va = (double)W[widx(e, cslot_a)];
-            vb = (double)W[widx(e, cslot_b)];
-          } else if (ccat == 1) {
-            va = (double)W[widx(e, 8 + cslot_a)];
-            vb = (double)W[widx(e, 8 + cslot_b)];
-          } else {
-            va = (double)W[widx(e, 8 + cslot_a)];
-            vb = (double)W[widx(e, cslot_b)];
-          }
-          corr += sg * va * vb;
+        for (int q = 0; q < 3; q++) {
+          const double va = __shfl(wrow, wa[q], kWave);
+          const double vb = __shfl(wrow, wb[q], kWave);
+          if (pact[q]) corr[q] += sg * va * vb;
         }
       }
-      __syncthreads();
     }
   }
   block_sum_store<51>(acc, sh, part + blockIdx.x, gridDim.x);
-  // correction pairs: one value per thread, no cross-thread reduction needed
-  if (cthread) part[(size_t)(P4_CORR + threadIdx.x) * gridDim.x + blockIdx.x] = corr;
+  if (track && col > 0) {
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (lane + 64 * q < 136) s_corr[wid][lane + 64 * q] = corr[q];
+    __syncthreads();
+    for (int t = threadIdx.x; t < 136; t += blockDim.x) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < kWavesPerBlock; w++) s += s_corr[w][t];
+      part[(size_t)(P4_CORR + t) * gridDim.x + blockIdx.x] = s;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

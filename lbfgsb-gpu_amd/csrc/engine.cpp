@@ -272,6 +272,7 @@ struct Engine final : EngineBase {
                          pmini);
         break;
       case K_UPD_REDUCE:
+        launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0);
         launch_update_gcp(st, ds, part, pmin, pmini, G, 1);
         break;
       case K_PASSA_FUSED:
@@ -283,6 +284,7 @@ struct Engine final : EngineBase {
         launch_gcp_round_small<T>(st, ca, dG, W, ds);
         break;
       case K_STOP:
+        launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0);
         launch_stop_tests(st, ds, part, pmax, pmin, pmini, G);
         break;
       case K_UPD:
@@ -292,6 +294,7 @@ struct Engine final : EngineBase {
         launch_p4<T>(st, G, n, dX, dG, l, u, nbd, iwhere, W, z, r, ds, part);
         break;
       case K_FORMK:
+        launch_rowsum(st, part, G, 0, P4_COUNT, ds, 1);
         launch_formk_subsm(st, ds, part, G);
         break;
       case K_P5A:

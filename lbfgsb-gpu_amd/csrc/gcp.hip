@@ -437,6 +437,7 @@ __device__ void round_conclude(DevScalars* ds, const Cand* cand, long long K, lo
   ds->gcp_last_idx = ik;
   ds->gcp_mode = kpass > 0 ? 1 : 0;
   ds->nint = (int)(kpass + 1);
+  ds->gcp_passed_total += kpass;
   gcp_done(ds);
 }
 

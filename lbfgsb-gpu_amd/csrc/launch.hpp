@@ -51,6 +51,8 @@ void launch_gd(hipStream_t s, int G, long long n, T* const* Gv, const T* d,
                const DevScalars* ds, double* part);
 
 // small.hip
+void launch_rowsum(hipStream_t s, const double* part, int G, int v0, int nrows, DevScalars* ds,
+                   int p4_rows);
 void launch_resume(hipStream_t s, DevScalars* ds, int reset_bp);
 void launch_setup_errclb(hipStream_t s, DevScalars* ds, const double* part, int G);
 void launch_setup_active(hipStream_t s, DevScalars* ds, const double* part, int G);

@@ -137,6 +137,54 @@ __device__ void set_done(DevScalars* ds, int info) {
 }
 
 // ---------------------------------------------------------------------------
+// Multi-block finish of per-block partials: block b sums row v0+b (G
+// entries) into ds->red[b] with a compensated block tree.  p4_rows skips the
+// rows of k_p4 that the iteration does not use (entering/leaving Gram when
+// nothing entered or left, the new formk column when no update happened).
+__global__ void __launch_bounds__(kBlock)
+k_rowsum(const double* part, int G, int v0, DevScalars* ds, int p4_rows) {
+  __shared__ double ss[kWavesPerBlock], sc[kWavesPerBlock];
+  if (ds->halt) return;
+  const int v = blockIdx.x;
+  if (p4_rows) {
+    const bool skip = (v >= P4_CORR && !ds->p4_flagged) ||
+                      (v >= P4_NC && v < P4_NC + 32 && !ds->updatd);
+    if (skip) {
+      if (threadIdx.x == 0) ds->red[v] = 0.0;
+      return;
+    }
+  }
+  const double* row = part + (size_t)(v0 + v) * G;
+  double a0 = 0.0, a1 = 0.0;
+  int b = threadIdx.x;
+  for (; b + kBlock < G; b += 2 * kBlock) {
+    a0 += row[b];
+    a1 += row[b + kBlock];
+  }
+  for (; b < G; b += kBlock) a0 += row[b];
+  double s, c, e;
+  two_sum(a0, a1, s, c);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double os = __shfl_xor(s, o, kWave), oc = __shfl_xor(c, o, kWave);
+    two_sum(s, os, s, e);
+    c += oc + e;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ss[threadIdx.x >> 6] = s;
+    sc[threadIdx.x >> 6] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = ss[0], tc = sc[0];
+    for (int w = 1; w < kWavesPerBlock; w++) {
+      two_sum(t, ss[w], t, e);
+      tc += sc[w] + e;
+    }
+    ds->red[v] = t + tc;
+  }
+}
+
 __global__ void s_resume(DevScalars* ds, int reset_bp) {
   if (threadIdx.x == 0) {
     ds->halt = 0;
@@ -183,6 +231,7 @@ __global__ void s_setup_projgr(DevScalars* ds, const double* pmax, int G, double
 // breakpoint rounds must run.
 __device__ void gcp_controller(DevScalars* ds) {
   ds->gcp_pending = 0;
+  ds->p4_flagged = 0;
   for (;;) {
     const int col = ds->col;
     ds->iword = -1;
@@ -249,6 +298,7 @@ __device__ void gcp_controller(DevScalars* ds) {
     // at least the smallest breakpoint is passed: sorted rounds, the first
     // over t in [0, 2 dtm) (dtm >= bkmin > 0 here)
     ds->gcp_pending = 1;
+    ds->gcp_rounds_run++;
     ds->gcp_round = 0;
     ds->gcp_floor_hits = 0;
     for (int q = 0; q < 40; q++) ds->gcp_carry[q] = 0.0;
@@ -300,20 +350,20 @@ __device__ void memory_update(DevScalars* ds) {
 __global__ void s_stop_tests(DevScalars* ds, const double* part, const double* pmax,
                              const double* pmin, const long long* pmini, int G) {
   __shared__ DevScalars sds;
-  __shared__ double r[PA_COUNT];
   __shared__ double sh[kWavesPerBlock];
   __shared__ long long shi[kWavesPerBlock];
   if (ds->halt) {
     if (threadIdx.x == 0) ds->write_trial = 0;
     return;
   }
-  reduce_rows(part, G, 0, PA_COUNT, r);
   const double sbg = reduce_max(pmax, G, sh);
   double bk;
   long long ibk;
   reduce_argmin(pmin, pmini, G, sh, shi, bk, ibk);
   ds_load(&sds, ds);
   DevScalars* s = &sds;
+  const double* r = s->red;  // row sums of the pass-A partials (k_rowsum)
+  (void)part;
   if (threadIdx.x == 0) {
     s->write_trial = 0;
     s->upd_done = 0;
@@ -359,7 +409,6 @@ __global__ void s_stop_tests(DevScalars* ds, const double* part, const double* p
 __global__ void s_update_gcp(DevScalars* ds, const double* part, const double* pmin,
                              const long long* pmini, int G, int reduce) {
   __shared__ DevScalars sds;
-  __shared__ double r[PA_COUNT];
   __shared__ double sh[kWavesPerBlock];
   __shared__ long long shi[kWavesPerBlock];
   if (ds->halt) {
@@ -369,16 +418,14 @@ __global__ void s_update_gcp(DevScalars* ds, const double* part, const double* p
   if (!reduce && ds->upd_done) return;
   double bk = 0.0;
   long long ibk = 0;
-  if (reduce) {
-    reduce_rows(part, G, 0, PA_COUNT, r);
-    reduce_argmin(pmin, pmini, G, sh, shi, bk, ibk);
-  }
+  if (reduce) reduce_argmin(pmin, pmini, G, sh, shi, bk, ibk);
   ds_load(&sds, ds);
   DevScalars* s = &sds;
+  (void)part;
   if (threadIdx.x == 0) {
     s->write_trial = 0;
     if (reduce) {
-      for (int v = 0; v < PA_COUNT; v++) s->pa[v] = r[v];
+      for (int v = 0; v < PA_COUNT; v++) s->pa[v] = s->red[v];
       s->gcp_bkmin = bk;
       s->gcp_ibkmin = ibk;
     } else {
@@ -478,24 +525,15 @@ __device__ bool formk_device(DevScalars* ds, const double* nc, const double* cor
 // and the two triangular solves of subsm (lbfgsbcpu.cpp:1459-1471).
 __global__ void s_formk_subsm(DevScalars* ds, const double* part, int G) {
   __shared__ DevScalars sds;
-  __shared__ double r[P4_COUNT];
-  __shared__ int flags[2];
+  __shared__ int flags[1];
   if (ds->halt) return;
-  // counts first; the other rows only when they are used
-  reduce_rows(part, G, 0, 3, r);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const bool shortcut = ds->shortcut;
-    const bool track = ds->iter > 0 && ds->cnstnd && !shortcut;
-    flags[0] = track && (r[P4_NENTER] > 0 || r[P4_NLEAVE] > 0);
-    flags[1] = ds->updatd;
-  }
-  __syncthreads();
-  if (flags[1]) reduce_rows(part, G, P4_NC, 32, r + P4_NC);
-  reduce_rows(part, G, P4_WV, 16, r + P4_WV);
-  if (flags[0]) reduce_rows(part, G, P4_CORR, 136, r + P4_CORR);
+  (void)part;
+  (void)G;
   ds_load(&sds, ds);
   DevScalars* s = &sds;
+  const double* r = s->red;  // row sums of the k_p4 partials (k_rowsum)
+  if (threadIdx.x == 0) flags[0] = s->p4_flagged;
+  __syncthreads();
   if (threadIdx.x == 0) {
     const long long n = s->n;
     const int col = s->col;
@@ -711,6 +749,10 @@ __global__ void s_ls_step(DevScalars* ds, const double* part, int G, double f) {
 }
 
 // ---------------------------------------------------------------------------
+void launch_rowsum(hipStream_t s, const double* part, int G, int v0, int nrows, DevScalars* ds,
+                   int p4_rows) {
+  hipLaunchKernelGGL(k_rowsum, dim3(nrows), dim3(kBlock), 0, s, part, G, v0, ds, p4_rows);
+}
 void launch_resume(hipStream_t s, DevScalars* ds, int reset_bp) {
   hipLaunchKernelGGL(s_resume, dim3(1), dim3(64), 0, s, ds, reset_bp);
 }

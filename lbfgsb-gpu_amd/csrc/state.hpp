@@ -126,6 +126,12 @@ struct DevScalars {
 
   int gcp_pending;   // sorted breakpoint rounds are in progress
   int upd_done;      // the memory update of this iteration already ran
+  int p4_flagged;    // k_p4 saw entering/leaving variables
+  int gcp_rounds_run;  // statistics: Cauchy searches that needed sorted rounds
+  long long gcp_passed_total;  // statistics: breakpoints passed in those
+
+  // ---- finished row sums of the last multi-block reduction (k_rowsum)
+  double red[P4_COUNT + 1];
 };
 
 }  // namespace lbfgsb
