@@ -116,6 +116,7 @@ ORC_DEFINE_TYPES(float, f32)
   void orc_formt_##SFX(orc_state_##SFX* s);                                    \
   void orc_projgr_##SFX(orc_state_##SFX* s);                                   \
   void orc_active_##SFX(orc_state_##SFX* s);                                   \
+  int orc_lnsrlb_step_##SFX(orc_state_##SFX* s);                               \
   /* pure kernels with explicit arguments */                                   \
   int orc_dpofa_##SFX(REAL* a, int lda, int n);                                \
   int orc_dtrsl_##SFX(const REAL* t, int ldt, int n, REAL* b, int job);        \

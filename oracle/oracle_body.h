@@ -1203,6 +1203,13 @@ static void S(orc_lnsrlb)(S(orc_state)* s) {
   }
 }
 
+/* One lnsrlb call on a state (task, d, x, g as the driver leaves them);
+ * returns the new task. */
+int S(orc_lnsrlb_step)(S(orc_state)* s) {
+  S(orc_lnsrlb)(s);
+  return s->task;
+}
+
 /* lbfgsbmatupd (lbfgsbcpu.cpp:1341-1387): append s = d, y = r to the ring,
  * update theta, shift sy/ss when the ring is full, new row of sy and new
  * column of ss. */

