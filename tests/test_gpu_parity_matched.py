@@ -7,8 +7,12 @@ both then run the same iteration phase by phase:
   subspace step + first line-search call     -> z, d, stp, d'd, g'd
   first trial point                          -> x_trial
   line search to acceptance + memory update  -> x_{k+1}, sy, ss, theta
-FP tolerances: fp64 rtol 1e-9; fp32 (FP32 bulk with FP64-accumulated dots on
-the GPU vs an all-float oracle) rtol 2e-4 on vectors.
+FP tolerances: fp64 rtol 1e-9.  For fp32 the engine keeps n-vectors in FP32
+but runs the Cauchy-search scalars, the small dense algebra and the line
+search in FP64, so its reference is the FP64 oracle applied to the same
+float-rounded state (the all-float oracle's f1/f2 recurrences drift over
+thousands of passed breakpoints: 3930 vs 3954 passed on Rosenbrock
+iteration 0); sets are still compared exactly, vectors with rtol 2e-4.
 """
 import numpy as np
 import pytest
@@ -63,6 +67,39 @@ def _load(engine, s, p):
         engine.set_scalar(k, v)
 
 
+_SCALARS_I = ["col", "head", "itail", "iupdat", "iter", "nfgv", "nint", "nintol", "nskip",
+              "nfree", "nact", "ileave", "nenter", "iword", "ifun", "iback", "info", "task",
+              "csave", "internalinfo", "prjctd", "cnstnd", "boxed", "updatd", "wrk", "k_err",
+              "restarts"]
+_SCALARS_R = ["f", "fold", "theta", "dr", "rr", "dnrm", "xstep", "sbgnrm", "ddum", "dtd", "gd",
+              "gdold", "stp", "stpmx", "tf"]
+
+
+def _as_f64(oracle, s32, p):
+    """The float32 oracle state widened to an FP64 oracle state: the
+    reference for the mixed-precision engine (FP32 data, FP64 scalar and
+    small-dense logic)."""
+    p64 = p.cast(np.float32).cast(np.float64)
+    s = oracle.Solver(p64, dtype=np.float64, eps=1e-10, max_iter=200)
+    a, b = s32.st, s.st
+    for k in _SCALARS_I:
+        setattr(b, k, getattr(a, k))
+    for k in _SCALARS_R:
+        setattr(b, k, float(getattr(a, k)))
+    for i in range(2):
+        b.isave[i] = a.isave[i]
+    for i in range(13):
+        b.dsave[i] = float(a.dsave[i])
+    for i in range(64):
+        b.wa[i] = float(a.wa[i])
+    for name in ("x", "g", "l", "u", "xold", "xdiff", "z", "r", "d", "t", "ws", "wy", "sy",
+                 "ss", "wt", "wn", "snd"):
+        s.arr(name)[:] = s32.arr(name).astype(np.float64)
+    for name in ("nbd", "index", "iwhere", "indx2"):
+        s.arr(name)[:] = s32.arr(name)
+    return s
+
+
 def _states(oracle, p, dtype, iters):
     s = oracle.Solver(p, dtype=dtype, eps=1e-10, max_iter=200)
     out, k = [], 0
@@ -97,8 +134,9 @@ def test_matched_iteration(oracle, pname, dtype):
         _load(eng, s, p)
         st = s.st
         n = st.n
-        # ---- oracle: Cauchy point, free/active sets, formk
-        ref = s.clone()
+        # ---- oracle: Cauchy point, free/active sets, formk (for fp32 the
+        # FP64 oracle on the same float state, see _as_f64)
+        ref = s.clone() if dtype == np.float64 else _as_f64(oracle, s, p)
         rs = ref.st
         shortcut = (not rs.cnstnd) and rs.col > 0
         old_index = ref.arr("index").copy()
@@ -115,7 +153,11 @@ def test_matched_iteration(oracle, pname, dtype):
         assert status == 0, status
         assert int(eng.scalar("restarts")) == 0
         if not shortcut:
-            assert np.array_equal(eng.array("iwhere"), ref.arr("iwhere")), "iwhere"
+            iw_g, iw_r = eng.array("iwhere"), ref.arr("iwhere")
+            bad = np.where(iw_g != iw_r)[0]
+            print(f"  nint gpu {int(eng.scalar('nint'))} oracle {rs.nint}; "
+                  f"iwhere mismatches {len(bad)} {bad[:8]}", flush=True)
+            assert np.array_equal(iw_g, iw_r), "iwhere"
             assert int(eng.scalar("nfree")) == rs.nfree
             assert int(eng.scalar("nint")) == rs.nint, (eng.scalar("nint"), rs.nint)
             if rs.iter > 0 and rs.cnstnd:
