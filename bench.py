@@ -50,6 +50,14 @@ def kernel_bytes_per_elem(s: int) -> dict:
             "p5b": 8 * s + 5, "trial": 2 * s, "gd": 2 * s}
 
 
+# HBM bytes per launch at steady state for the default workload (n = 16,777,216,
+# fp32), from two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE, KiB per
+# dispatch) scaled by what the same counters read for a 64 MiB copy kernel
+# (fetch x1.999, write x1.000); see profiles/plain_rosenbrock_16M_fp32_hbm_traffic.md.
+MEASURED_TRAFFIC_16M_FP32 = {"pass_a_fused": 1647023985, "p4": 1588828545, "p5a": 1492505298,
+                             "p5b": 620730337, "trial": 134205575, "gd": 134233373}
+
+
 def b_alg_per_iter(n: int, s: int) -> float:
     """Canonical compulsory bytes of one iteration, n (76 s + 32)."""
     return float(n) * (76 * s + 32)
@@ -105,7 +113,7 @@ def run_gpu(n: int, f64: bool, warmup: int, steps: int):
     return dict(timed=timed, iters=iters, wall=wall, kern=kern, info=summ.info)
 
 
-def run_cpu_baseline(n_target: int, n_sample: int, warm: int = 9, timed: int = 3):
+def run_cpu_baseline(n_target: int, n_sample: int, warm: int = 9, timed: int = 12):
     """The oracle (a 1-thread C restatement of the reference CPU path) on the
     same generator at n_sample, steady-state iterations (col = m) timed and
     the per-iteration cost scaled linearly to n_target."""
@@ -136,7 +144,7 @@ def main():
     ap.add_argument("--n", type=int, default=4096 * 4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=2 ** 21)
+    ap.add_argument("--cpu-sample", type=int, default=2 ** 22)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -171,7 +179,9 @@ def main():
                                 f"dense + line search"), "n": n, "m": 8},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": bytes_launch,
+                     "traffic": (MEASURED_TRAFFIC_16M_FP32.get(dom) if n == 4096 * 4096
+                                 else None),
+                     "bytes_per_launch": bytes_launch,
                      "avg_launch_us": avg_s * 1e6, "launches": cnt},
         "iteration_roofline": {
             "b_alg_bytes": b_alg_per_iter(n, s),

@@ -3,7 +3,9 @@
     python lbfgsb-gpu_amd/build.py [--jobs N]
 
 Every source is compiled on every call; objects go to lbfgsb-gpu_amd/_build
-and the library to lbfgsb-gpu_amd/lib/liblbfgsb_amd.so.
+and the library to lbfgsb-gpu_amd/lib/liblbfgsb_amd.so.  The example of the
+C++ drop-in API (examples/dsscfg.hip, over include/culbfgsb.h) is linked
+against it as lbfgsb-gpu_amd/lib/dsscfg_example.
 """
 from __future__ import annotations
 
@@ -19,6 +21,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "liblbfgsb_amd.so")
+EXAMPLE_SRC = os.path.join(HERE, "examples", "dsscfg.hip")
+EXAMPLE = os.path.join(LIBDIR, "dsscfg_example")
 
 SOURCES = ["bulk.hip", "small.hip", "gcp.hip", "freev.hip", "objectives.hip",
            "engine.cpp", "capi.cpp"]
@@ -60,8 +64,13 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    cmd = [HIPCC] + COMMON + DEVICE + [EXAMPLE_SRC, "-x", "none", "-L" + LIBDIR,
+                                       "-llbfgsb_amd", "-Wl,-rpath,$ORIGIN", "-o", EXAMPLE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"example build failed:\n{r.stderr}")
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", EXAMPLE)
     return LIB
 
 
