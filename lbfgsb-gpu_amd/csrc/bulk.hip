@@ -71,6 +71,96 @@ __device__ inline void classify(T x, T g, T l, T u, int nb, int iw_in, int& iw,
 
 __device__ inline int grid_blocks() { return gridDim.x; }
 
+// Ring slot of each memory position j = 0..kMaxM-1 (j = 0 oldest).
+__device__ inline void ring_slots(int head, int m, int (&slot)[kMaxM]) {
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    slot[j] = head + j;
+    if (slot[j] >= m) slot[j] -= m;
+  }
+}
+
+// Which W slots the kernel reads for each memory position j (uniform over
+// the grid).  Positions that are not used (j >= col, or the slot that is
+// being overwritten, skip_slot) read the slot of the first used position
+// instead: an address the same wave is reading anyway, so the load hits in
+// cache and the loop needs no branch around it.
+struct RingReads {
+  int rs[kMaxM];
+  bool use[kMaxM];
+  int newest;  // slot of position col - 1 (0 when the memory is empty)
+};
+__device__ inline RingReads ring_reads(int head, int m, int col, int skip_slot) {
+  RingReads r;
+  int slot[kMaxM];
+  ring_slots(head, m, slot);
+  r.newest = col > 0 ? (head + col - 1) % m : 0;
+  int fb = -1;
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    r.use[j] = j < col && slot[j] != skip_slot;
+    if (r.use[j] && fb < 0) fb = slot[j];
+  }
+  if (fb < 0) fb = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) r.rs[j] = r.use[j] ? slot[j] : fb;
+  return r;
+}
+
+// The retained Y and S entries of element i in ring order (0 for unused
+// positions).  The 16 loads are unconditional and issued ahead of any use,
+// so a wave has the whole W row in flight at once (one 256 B segment per
+// slot in FP32) instead of waiting on each pair.
+template <typename T>
+__device__ inline void load_ring(const T* __restrict__ W, long long i, const RingReads& rr,
+                                 T (&yv)[kMaxM], T (&sv)[kMaxM]) {
+  const T* __restrict__ row = W + widx((size_t)i, 0);
+  T ly[kMaxM], ls[kMaxM];
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    ly[j] = row[rr.rs[j] * kTile];
+    ls[j] = row[(8 + rr.rs[j]) * kTile];
+  }
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    yv[j] = rr.use[j] ? ly[j] : (T)0;
+    sv[j] = rr.use[j] ? ls[j] : (T)0;
+  }
+}
+
+// Everything the Cauchy-point pass reads for one element; yn / sn repeat
+// the newest pair (position col - 1, a cache hit) so that no per-element
+// select over positions is needed.
+template <typename T>
+struct BulkIn {
+  T x, g, l, u;
+  int nb;
+  uint8_t iwb;
+  T yv[kMaxM], sv[kMaxM];
+  T yn, sn;
+};
+template <typename T>
+__device__ inline void load_bulk_in(BulkIn<T>& e, long long i, const T* __restrict__ x,
+                                    const T* __restrict__ g, const T* __restrict__ l,
+                                    const T* __restrict__ u, const int* __restrict__ nbd,
+                                    const uint8_t* __restrict__ iwhere,
+                                    const T* __restrict__ W, const RingReads& rr) {
+  e.x = x[i];
+  e.g = g[i];
+  e.l = l[i];
+  e.u = u[i];
+  e.nb = nbd[i];
+  e.iwb = iwhere[i];
+  load_ring(W, i, rr, e.yv, e.sv);
+  const T* __restrict__ row = W + widx((size_t)i, 0);
+  e.yn = row[rr.newest * kTile];
+  e.sn = row[(8 + rr.newest) * kTile];
+}
+template <typename T>
+__device__ inline void keep_bulk_in(const BulkIn<T>& e) {
+  keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.yv, e.sv, e.yn, e.sn);
+}
+
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -182,7 +272,7 @@ k_projgr(long long n, const T* __restrict__ x, const T* __restrict__ g,
 //          s'y_j, s_j's over the retained slots is formed.
 template <typename T, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
-k_pass_a(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
+k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
          const T* __restrict__ u, const int* __restrict__ nbd,
          uint8_t* __restrict__ iwhere, T* __restrict__ W, const T* __restrict__ d,
          DevScalars* ds, double* part, double* pmax, double* pmin, long long* pmini) {
@@ -197,89 +287,128 @@ k_pass_a(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
   const bool upd = FUSED && ds->updatd;
   const T stp = (T)ds->stp;
   const bool stp1 = ds->stp == 1.0;
+  const RingReads rr = ring_reads(head, m, col, upd ? itail : -1);
 
   double acc[PA_COUNT];
 #pragma unroll
   for (int v = 0; v < PA_COUNT; v++) acc[v] = 0.0;
   double sbg = 0.0;
+  double pny = 0.0, pns = 0.0;  // new pair's share of p (position col - 1)
   double bkmin = __builtin_inf();
   long long ibk = 0x7fffffffffffffffLL;
 
-  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
-       i += (long long)gridDim.x * kBlock) {
-    bool isbrk = false;
-    T tbrk = 0;
-    {
-      const T xi = x[i], gi = g[i], li = l[i], ui = u[i];
-      const int nb = nbd[i];
-      const uint8_t iwb = iwhere[i];
-      T ys = 0, ss = 0;
-      if (FUSED) {
-        const T gp = projected_gi(xi, gi, li, ui, nb);
-        sbg = fmax(sbg, (double)fabs(gp));
-        const T xd = xold[i] - xi;
-        acc[PA_TF] += (double)xd * (double)xd;
-        if (upd) {
-          ys = gi - gold[i];
-          ss = stp1 ? d[i] : d[i] * stp;
-          acc[PA_RR] += (double)ys * (double)ys;
-          W[widx(i, itail)] = ys;
-          W[widx(i, 8 + itail)] = ss;
-        }
-      }
-      // ring slots in order from head; position col-1 is the newest slot
-      T yv[kMaxM], sv[kMaxM];
-#pragma unroll
-      for (int j = 0; j < kMaxM; j++) {
-        yv[j] = 0;
-        sv[j] = 0;
-        if (j < col) {
-          int slot = head + j;
-          if (slot >= m) slot -= m;
-          if (upd && slot == itail) {
-            yv[j] = ys;
-            sv[j] = ss;
-          } else {
-            yv[j] = W[widx(i, slot)];
-            sv[j] = W[widx(i, 8 + slot)];
-          }
-        }
-      }
+  // everything the pass reads for one element; the slot being replaced by
+  // the new pair is not read
+  struct In {
+    T x, g, l, u, xo, go, dd;
+    int nb;
+    uint8_t iwb;
+    T yv[kMaxM], sv[kMaxM];
+  };
+  auto load = [&](In& e, const long long i) __attribute__((always_inline)) {
+    e.x = x[i];
+    e.g = g[i];
+    e.l = l[i];
+    e.u = u[i];
+    e.nb = nbd[i];
+    e.iwb = iwhere[i];
+    e.xo = FUSED ? xold[i] : (T)0;
+    e.go = upd ? gold[i] : (T)0;
+    e.dd = upd ? d[i] : (T)0;
+    load_ring(W, i, rr, e.yv, e.sv);
+  };
+  auto keep = [&](const In& e) __attribute__((always_inline)) {
+    keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.xo, e.go, e.dd, e.yv, e.sv);
+  };
+  auto element = [&](const long long i, In& e) __attribute__((always_inline)) {
+    const T xi = e.x, gi = e.g, li = e.l, ui = e.u;
+    const int nb = e.nb;
+    const uint8_t iwb = e.iwb;
+    T ys = 0, ss = 0;
+    if (FUSED) {
+      const T gp = projected_gi(xi, gi, li, ui, nb);
+      sbg = fmax(sbg, (double)fabs(gp));
+      const T xd = e.xo - xi;
+      acc_prod(acc[PA_TF], xd, xd);
       if (upd) {
-#pragma unroll
-        for (int j = 0; j < kMaxM - 1; j++) {
-          if (j < col - 1) {
-            acc[PA_SY + j] += (double)ss * (double)yv[j];
-            acc[PA_SS + j] += (double)sv[j] * (double)ss;
-          }
-        }
+        ys = gi - e.go;
+        ss = stp1 ? e.dd : e.dd * stp;
+        acc_prod(acc[PA_RR], ys, ys);
+        W[widx(i, itail)] = ys;
+        W[widx(i, 8 + itail)] = ss;
       }
-      int iw;
-      T neggi;
-      bool isfree;
-      classify(xi, gi, li, ui, nb, iw_get(iwb), iw, neggi, isfree, isbrk, tbrk);
-      if (isfree) {
-        acc[PA_F1] += (double)neggi * (double)neggi;
+    }
+    // Unused positions hold zeros (load_ring), and with an update the
+    // newest position col - 1 (slot itail, not read) is zero too: the sums
+    // run over all positions without guards, and the new pair's share of p
+    // goes to pny / pns (added at position col - 1 after the loop).
+    if (upd) {
 #pragma unroll
-        for (int j = 0; j < kMaxM; j++) {
-          if (j < col) {
-            acc[PA_P + j] += (double)yv[j] * (double)neggi;
-            acc[PA_P + 8 + j] += (double)sv[j] * (double)neggi;
-          }
-        }
-        if (isbrk) {
-          acc[PA_NBREAK] += 1.0;
-          double tt = (double)tbrk;
-          if (tt < bkmin || (tt == bkmin && i < ibk)) {
-            bkmin = tt;
-            ibk = i;
-          }
-        } else {
-          acc[PA_NFREEGCP] += 1.0;
-          if (fabs(neggi) > 0) acc[PA_UNBND] = 1.0;
-        }
+      for (int j = 0; j < kMaxM - 1; j++) {
+        acc_prod(acc[PA_SY + j], ss, e.yv[j]);
+        acc_prod(acc[PA_SS + j], e.sv[j], ss);
       }
-      iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb));
+    }
+    int iw;
+    T neggi, tbrk;
+    bool isfree, isbrk;
+    classify(xi, gi, li, ui, nb, iw_get(iwb), iw, neggi, isfree, isbrk, tbrk);
+    // GCP-free variables feed f1 and p; the others add an exact zero
+    const T nf = isfree ? neggi : (T)0;
+    acc_prod(acc[PA_F1], nf, nf);
+#pragma unroll
+    for (int j = 0; j < kMaxM; j++) {
+      acc_prod(acc[PA_P + j], e.yv[j], nf);
+      acc_prod(acc[PA_P + 8 + j], e.sv[j], nf);
+    }
+    if (upd) {
+      acc_prod(pny, ys, nf);
+      acc_prod(pns, ss, nf);
+    }
+    if (isfree) {
+      if (isbrk) {
+        acc[PA_NBREAK] += 1.0;
+        const double tt = (double)tbrk;
+        if (tt < bkmin || (tt == bkmin && i < ibk)) {
+          bkmin = tt;
+          ibk = i;
+        }
+      } else {
+        acc[PA_NFREEGCP] += 1.0;
+        if (fabs(neggi) > 0) acc[PA_UNBND] = 1.0;
+      }
+    }
+    iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb));
+  };
+
+  if constexpr (sizeof(T) == 4) {
+    // FP32: two consecutive runs of 256 elements per block step, loads of
+    // both issued before either is processed
+    for (long long i0 = blockIdx.x * (long long)(2 * kBlock); i0 < n;
+         i0 += (long long)gridDim.x * (2 * kBlock)) {
+      const long long ia = i0 + threadIdx.x, ib = ia + kBlock;
+      In ea, eb;
+      load(ea, ia < n ? ia : n - 1);
+      load(eb, ib < n ? ib : n - 1);
+      keep(ea);
+      keep(eb);
+      if (ia < n) element(ia, ea);
+      if (ib < n) element(ib, eb);
+    }
+  } else {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kBlock) {
+      In e;
+      load(e, i);
+      keep(e);
+      element(i, e);
+    }
+  }
+  if (upd) {
+#pragma unroll
+    for (int j = 0; j < kMaxM; j++) {
+      acc[PA_P + j] += j == col - 1 ? pny : 0.0;
+      acc[PA_P + 8 + j] += j == col - 1 ? pns : 0.0;
     }
   }
   // PA_UNBND is an OR: keep it as a 0/1 value through the sum
@@ -326,7 +455,7 @@ __device__ inline void corr_pair(int t, int& cat, int& a, int& b) {
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_p4(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
+k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
      const T* __restrict__ u, const int* __restrict__ nbd,
      uint8_t* __restrict__ iwhere, const T* __restrict__ W, T* __restrict__ z,
      T* __restrict__ r, DevScalars* ds, double* part) {
@@ -346,8 +475,7 @@ k_p4(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
   const bool newcol = ds->updatd && col > 0;
   const double theta = ds->theta;
   const int upcl = ds->updatd ? col - 1 : col;
-  int newest = head + col - 1;
-  if (newest >= m) newest -= m;
+  const RingReads rr = ring_reads(head, m, col, -1);
   double mcy[kMaxM], mcs[kMaxM];
 #pragma unroll
   for (int j = 0; j < kMaxM; j++) {
@@ -379,133 +507,141 @@ k_p4(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
     wb[q] = cat == 1 ? 8 + sb : sb;
   }
 
-  for (long long i0 = blockIdx.x * (long long)kBlock; i0 < n;
-       i0 += (long long)gridDim.x * kBlock) {
-    const long long i = i0 + threadIdx.x;
+  // one element: Cauchy point, free/entering/leaving flags, the new-column
+  // Gram sums, r and W'r.  Returns +1 / -1 for an entering / leaving variable.
+  auto element = [&](const long long i, const BulkIn<T>& e) __attribute__((always_inline)) -> int {
     int flag = 0;
-    if (i < n) {
-      const T xi = x[i], gi = g[i];
-      const uint8_t iwb = iwhere[i];
-      const bool wasfree = iw_wasfree(iwb);
-      int iw = iw_get(iwb);
-      T zi = xi;
-      if (!shortcut) {
-        const T li = l[i], ui = u[i];
-        const int nb = nbd[i];
-        T neggi, tb;
-        bool isfree, isbrk;
-        classify(xi, gi, li, ui, nb, iw, iw, neggi, isfree, isbrk, tb);
-        const T dcp = isfree ? neggi : (T)0;
-        if (mode == 3) {
-          if (isbrk) {
-            iw = dcp > 0 ? 2 : 1;
-            zi = dcp > 0 ? ui : li;
-          }
-        } else if (mode != 2) {
-          bool passed = false;
-          if (mode == 1 && isbrk) {
-            const double tt = (double)tb;
-            passed = tt < t_last || (tt == t_last && i <= i_last);
-          }
-          if (passed) {
-            iw = dcp > 0 ? 2 : 1;
-            zi = dcp > 0 ? ui : li;
-          } else {
-            zi = xi + tsum * dcp;
-          }
+    const T xi = e.x, gi = e.g, li = e.l, ui = e.u;
+    const int nb = e.nb;
+    const uint8_t iwb = e.iwb;
+    const bool wasfree = iw_wasfree(iwb);
+    int iw = iw_get(iwb);
+    T zi = xi;
+    if (!shortcut) {
+      T neggi, tb;
+      bool isfree, isbrk;
+      classify(xi, gi, li, ui, nb, iw, iw, neggi, isfree, isbrk, tb);
+      const T dcp = isfree ? neggi : (T)0;
+      if (mode == 3) {
+        if (isbrk) {
+          iw = dcp > 0 ? 2 : 1;
+          zi = dcp > 0 ? ui : li;
         }
-      }
-      const bool freenow = iw <= 0;
-      if (!shortcut) {
-        acc[P4_NFREE] += freenow ? 1.0 : 0.0;
-        if (track) {
-          if (!wasfree && freenow) {
-            acc[P4_NENTER] += 1.0;
-            flag = 1;
-          }
-          if (wasfree && !freenow) {
-            acc[P4_NLEAVE] += 1.0;
-            flag = -1;
-          }
+      } else if (mode != 2) {
+        bool passed = false;
+        if (mode == 1 && isbrk) {
+          const double tt = (double)tb;
+          passed = tt < t_last || (tt == t_last && i <= i_last);
         }
-        iwhere[i] = iw_make(iw, freenow, wasfree);
-        z[i] = zi;
-      }
-      if (col > 0) {
-        T yv[kMaxM], sv[kMaxM];
-#pragma unroll
-        for (int j = 0; j < kMaxM; j++) {
-          yv[j] = 0;
-          sv[j] = 0;
-          if (j < col) {
-            int slot = head + j;
-            if (slot >= m) slot -= m;
-            yv[j] = W[widx(i, slot)];
-            sv[j] = W[widx(i, 8 + slot)];
-          }
-        }
-        if (newcol) {
-          T yn = 0, sn = 0;
-#pragma unroll
-          for (int j = 0; j < kMaxM; j++)
-            if (j == col - 1) {
-              yn = yv[j];
-              sn = sv[j];
-            }
-#pragma unroll
-          for (int j = 0; j < kMaxM; j++) {
-            if (j < col) {
-              if (freenow) {
-                acc[P4_NC + j] += (double)yn * (double)yv[j];
-                acc[P4_NC + 24 + j] += (double)sv[j] * (double)yn;
-              } else {
-                acc[P4_NC + 8 + j] += (double)sn * (double)sv[j];
-                acc[P4_NC + 16 + j] += (double)sn * (double)yv[j];
-              }
-            }
-          }
-        }
-        if (freenow) {
-          // r = -theta (z - x) - g + W (M c)  (lbfgsbcpu.cpp:847-867)
-          double ri;
-          if (shortcut) {
-            ri = -(double)gi;
-          } else {
-            ri = -theta * (double)(zi - xi) - (double)gi;
-#pragma unroll
-            for (int j = 0; j < kMaxM; j++)
-              if (j < col) ri += (double)yv[j] * mcy[j] + (double)sv[j] * mcs[j];
-          }
-          const T rt = (T)ri;
-          r[i] = rt;
-#pragma unroll
-          for (int j = 0; j < kMaxM; j++) {
-            if (j < col) {
-              acc[P4_WV - 0 + j] += (double)yv[j] * (double)rt;
-              acc[P4_WV + 8 + j] += (double)sv[j] * (double)rt;
-            }
-          }
+        if (passed) {
+          iw = dcp > 0 ? 2 : 1;
+          zi = dcp > 0 ? ui : li;
+        } else {
+          zi = xi + tsum * dcp;
         }
       }
     }
-    // signed Gram over this wave's entering / leaving variables, one
-    // flagged variable at a time in lane order (no block barrier)
-    if (track && col > 0) {
-      unsigned long long fm = __ballot(flag != 0);
-      if (fm && lane == 0) ds->p4_flagged = 1;
-      while (fm) {
-        const int src = __ffsll((long long)fm) - 1;
-        fm &= fm - 1ull;
-        const long long e = __shfl(i, src, kWave);
-        const double sg = (double)__shfl(flag, src, kWave);
-        const double wrow = lane < 16 ? (double)W[widx(e, lane)] : 0.0;
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-          const double va = __shfl(wrow, wa[q], kWave);
-          const double vb = __shfl(wrow, wb[q], kWave);
-          if (pact[q]) corr[q] += sg * va * vb;
+    const bool freenow = iw <= 0;
+    if (!shortcut) {
+      acc[P4_NFREE] += freenow ? 1.0 : 0.0;
+      if (track) {
+        if (!wasfree && freenow) {
+          acc[P4_NENTER] += 1.0;
+          flag = 1;
+        }
+        if (wasfree && !freenow) {
+          acc[P4_NLEAVE] += 1.0;
+          flag = -1;
         }
       }
+      iwhere[i] = iw_make(iw, freenow, wasfree);
+      z[i] = zi;
+    }
+    // Positions j >= col hold zeros (load_ring), so the sums below run over
+    // all kMaxM positions without guards: unused accumulators collect exact
+    // zeros and are never read.
+    if (col > 0) {
+      if (newcol) {
+        // free variables feed the Y'Y / S'Y sums, active ones the S'S / S'Y
+        // sums; the other side adds an exact zero (no divergent stores,
+        // which the compiler would merge into a dynamically indexed one)
+        const T yf = freenow ? e.yn : (T)0, sa = freenow ? (T)0 : e.sn;
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++) {
+          acc_prod(acc[P4_NC + j], yf, e.yv[j]);
+          acc_prod(acc[P4_NC + 24 + j], e.sv[j], yf);
+          acc_prod(acc[P4_NC + 8 + j], sa, e.sv[j]);
+          acc_prod(acc[P4_NC + 16 + j], sa, e.yv[j]);
+        }
+      }
+      // r = -theta (z - x) - g + W (M c) over the free set
+      // (lbfgsbcpu.cpp:847-867); 0 elsewhere, so W'r needs no branch
+      double ri = -(double)gi;
+      if (!shortcut) {
+        ri = -theta * (double)(zi - xi) - (double)gi;
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++) {
+          ri = __builtin_fma((double)e.yv[j], mcy[j], ri);
+          ri = __builtin_fma((double)e.sv[j], mcs[j], ri);
+        }
+      }
+      const T rt = freenow ? (T)ri : (T)0;
+      if (freenow) r[i] = rt;
+#pragma unroll
+      for (int j = 0; j < kMaxM; j++) {
+        acc_prod(acc[P4_WV + j], e.yv[j], rt);
+        acc_prod(acc[P4_WV + 8 + j], e.sv[j], rt);
+      }
+    }
+    return flag;
+  };
+
+  // signed Gram over this wave's entering / leaving variables, one flagged
+  // variable at a time in lane order (no block barrier)
+  auto corr_step = [&](const long long i, const int flag) __attribute__((always_inline)) {
+    if (!(track && col > 0)) return;
+    unsigned long long fm = __ballot(flag != 0);
+    if (fm && lane == 0) ds->p4_flagged = 1;
+    while (fm) {
+      const int src = __ffsll((long long)fm) - 1;
+      fm &= fm - 1ull;
+      const long long ei = __shfl(i, src, kWave);
+      const double sg = (double)__shfl(flag, src, kWave);
+      const double wrow = lane < 16 ? (double)W[widx(ei, lane)] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const double va = __shfl(wrow, wa[q], kWave);
+        const double vb = __shfl(wrow, wb[q], kWave);
+        if (pact[q]) corr[q] += sg * va * vb;
+      }
+    }
+  };
+
+  if constexpr (sizeof(T) == 4) {
+    // FP32: each block takes two consecutive runs of 256 elements per step;
+    // the loads of both are issued before either is processed, so every wave
+    // has two elements' worth of requests in flight (the register footprint
+    // leaves two waves per SIMD)
+    for (long long i0 = blockIdx.x * (long long)(2 * kBlock); i0 < n;
+         i0 += (long long)gridDim.x * (2 * kBlock)) {
+      const long long ia = i0 + threadIdx.x, ib = ia + kBlock;
+      BulkIn<T> ea, eb;
+      load_bulk_in(ea, ia < n ? ia : n - 1, x, g, l, u, nbd, iwhere, W, rr);
+      load_bulk_in(eb, ib < n ? ib : n - 1, x, g, l, u, nbd, iwhere, W, rr);
+      keep_bulk_in(ea);
+      keep_bulk_in(eb);
+      corr_step(ia, ia < n ? element(ia, ea) : 0);
+      corr_step(ib, ib < n ? element(ib, eb) : 0);
+    }
+  } else {
+    // FP64: one element per step (two would not fit two waves per SIMD)
+    for (long long i0 = blockIdx.x * (long long)kBlock; i0 < n;
+         i0 += (long long)gridDim.x * kBlock) {
+      const long long ia = i0 + threadIdx.x;
+      BulkIn<T> ea;
+      load_bulk_in(ea, ia < n ? ia : n - 1, x, g, l, u, nbd, iwhere, W, rr);
+      keep_bulk_in(ea);
+      corr_step(ia, ia < n ? element(ia, ea) : 0);
     }
   }
   block_sum_store<51>(acc, sh, part + blockIdx.x, gridDim.x);
@@ -542,35 +678,37 @@ k_p5a(long long n, const T* __restrict__ l, const T* __restrict__ u,
     wy[j] = j < col ? ds->wv[j] / theta : 0.0;
     wsv[j] = j < col ? ds->wv[col + j] : 0.0;
   }
+  const RingReads rr = ring_reads(head, m, col, -1);
   double amin = 1.0;
   long long ai = 0x7fffffffffffffffLL;
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
-    const bool freenow = shortcut || iw_get(iwhere[i]) <= 0;
+    // every load of the element is issued before the first use, so a wave
+    // keeps ~22 requests in flight instead of waiting on each W pair
+    const uint8_t iwb = iwhere[i];
+    const T ri = r[i], zi = z[i], li = l[i], ui = u[i];
+    const int nb = nbd[i];
+    T yv[kMaxM], sv[kMaxM];
+    load_ring(W, i, rr, yv, sv);
+    keep_loads(ri, zi, li, ui, nb, iwb, yv, sv);
+    const bool freenow = shortcut || iw_get(iwb) <= 0;
     if (!freenow) continue;
-    double di = (double)r[i];
+    double di = (double)ri;
 #pragma unroll
-    for (int j = 0; j < kMaxM; j++) {
-      if (j < col) {
-        int slot = head + j;
-        if (slot >= m) slot -= m;
-        di += (double)W[widx(i, slot)] * wy[j] + (double)W[widx(i, 8 + slot)] * wsv[j];
-      }
-    }
+    for (int j = 0; j < kMaxM; j++)
+      if (j < col) di += (double)yv[j] * wy[j] + (double)sv[j] * wsv[j];
     const T dk = (T)(di / theta);
     dsub[i] = dk;
-    const int nb = nbd[i];
     if (nb != 0) {
       double cand = 2.0;
-      const T zi = z[i];
       if (dk < 0 && nb <= 2) {
-        const T t2 = l[i] - zi;
+        const T t2 = li - zi;
         if (t2 >= 0)
           cand = 0.0;
         else if (dk < t2)
           cand = (double)(t2 / dk);
       } else if (dk > 0 && nb >= 2) {
-        const T t2 = u[i] - zi;
+        const T t2 = ui - zi;
         if (t2 <= 0)
           cand = 0.0;
         else if (dk > t2)
@@ -607,7 +745,7 @@ k_p5a(long long n, const T* __restrict__ l, const T* __restrict__ u,
 // and the line-search set-up sums d'd, g'd and the nbd-aware max step.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_p5b(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
+k_p5b(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
       const T* __restrict__ u, const int* __restrict__ nbd,
       const uint8_t* __restrict__ iwhere, T* __restrict__ z,
       const T* __restrict__ dsub, T* __restrict__ d, DevScalars* ds,
@@ -628,16 +766,22 @@ k_p5b(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
   double smin = 1.0e10;
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
-    const T xi = x[i];
-    T zi = shortcut ? xi : z[i];
-    if (sub && (shortcut || iw_get(iwhere[i]) <= 0)) {
-      T dk = dsub[i];
+    // all loads first (see load_ring)
+    const T xi = x[i], gi = g[i], li = l[i], ui = u[i];
+    const T zr = shortcut ? (T)0 : z[i];
+    const uint8_t iwb = iwhere[i];
+    const T dl = sub ? dsub[i] : (T)0;
+    const int nb = nbd[i];
+    keep_loads(xi, gi, li, ui, zr, iwb, dl, nb);
+    T zi = shortcut ? xi : zr;
+    if (sub && (shortcut || iw_get(iwb) <= 0)) {
+      T dk = dl;
       if (pin && i == ibd) {
         if (dk > 0) {
-          zi = u[i];
+          zi = ui;
           dk = 0;
         } else if (dk < 0) {
-          zi = l[i];
+          zi = li;
           dk = 0;
         }
       }
@@ -649,18 +793,17 @@ k_p5b(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
     const T di = zi - xi;
     d[i] = di;
     acc[0] += (double)di * (double)di;
-    acc[1] += (double)g[i] * (double)di;
+    acc[1] += (double)gi * (double)di;
     if (want_stpmx) {
-      const int nb = nbd[i];
       if (nb != 0) {
         if (di < 0 && nb <= 2) {
-          const T a2 = l[i] - xi;
+          const T a2 = li - xi;
           if (a2 >= 0)
             smin = 0.0;
           else if (di * big < a2)
             smin = fmin(smin, (double)(a2 / di));
         } else if (di > 0 && nb >= 2) {
-          const T a2 = u[i] - xi;
+          const T a2 = ui - xi;
           if (a2 <= 0)
             smin = 0.0;
           else if (di * big > a2)
@@ -685,7 +828,7 @@ k_p5b(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
 // (lbfgsbcpu.cpp:1329-1335).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_trial(long long n, T* const* X, const T* __restrict__ z, const T* __restrict__ d,
+k_trial(long long n, Pair<T> X, const T* __restrict__ z, const T* __restrict__ d,
         const DevScalars* ds) {
   if (!ds->write_trial) return;
   const T* __restrict__ xa = X[ds->trial_src];
@@ -704,7 +847,7 @@ k_trial(long long n, T* const* X, const T* __restrict__ z, const T* __restrict__
 // g_b . d after an evaluation (lbfgsbcpu.cpp:1312).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_gd(long long n, T* const* Gv, const T* __restrict__ d, const DevScalars* ds,
+k_gd(long long n, Pair<T> Gv, const T* __restrict__ d, const DevScalars* ds,
      double* part) {
   __shared__ double sh[kWavesPerBlock];
   const int cur = ds->phase & 1;
@@ -736,8 +879,8 @@ void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g,
                      pmax);
 }
 template <typename T>
-void launch_pass_a(hipStream_t s, int G, bool fused, long long n, T* const* X,
-                   T* const* Gv, const T* l, const T* u, const int* nbd,
+void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X,
+                   Pair<T> Gv, const T* l, const T* u, const int* nbd,
                    uint8_t* iwhere, T* W, const T* d, DevScalars* ds, double* part,
                    double* pmax, double* pmin, long long* pmini) {
   if (fused)
@@ -748,7 +891,7 @@ void launch_pass_a(hipStream_t s, int G, bool fused, long long n, T* const* X,
                        Gv, l, u, nbd, iwhere, W, d, ds, part, pmax, pmin, pmini);
 }
 template <typename T>
-void launch_p4(hipStream_t s, int G, long long n, T* const* X, T* const* Gv,
+void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
                const T* l, const T* u, const int* nbd, uint8_t* iwhere,
                const T* W, T* z, T* r, DevScalars* ds, double* part) {
   hipLaunchKernelGGL(k_p4<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd,
@@ -763,7 +906,7 @@ void launch_p5a(hipStream_t s, int G, long long n, const T* l, const T* u,
                      W, z, r, dsub, ds, pmin, pmini);
 }
 template <typename T>
-void launch_p5b(hipStream_t s, int G, long long n, T* const* X, T* const* Gv,
+void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
                 const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
                 T* z, const T* dsub, T* d, DevScalars* ds, double* part,
                 double* pmin) {
@@ -771,12 +914,12 @@ void launch_p5b(hipStream_t s, int G, long long n, T* const* X, T* const* Gv,
                      iwhere, z, dsub, d, ds, part, pmin);
 }
 template <typename T>
-void launch_trial(hipStream_t s, int G, long long n, T* const* X, const T* z,
+void launch_trial(hipStream_t s, int G, long long n, Pair<T> X, const T* z,
                   const T* d, const DevScalars* ds) {
   hipLaunchKernelGGL(k_trial<T>, dim3(G), dim3(kBlock), 0, s, n, X, z, d, ds);
 }
 template <typename T>
-void launch_gd(hipStream_t s, int G, long long n, T* const* Gv, const T* d,
+void launch_gd(hipStream_t s, int G, long long n, Pair<T> Gv, const T* d,
                const DevScalars* ds, double* part) {
   hipLaunchKernelGGL(k_gd<T>, dim3(G), dim3(kBlock), 0, s, n, Gv, d, ds, part);
 }
@@ -788,22 +931,22 @@ void launch_gd(hipStream_t s, int G, long long n, T* const* Gv, const T* d,
                                  const T*, const int*, uint8_t*, double*);          \
   template void launch_projgr<T>(hipStream_t, int, long long, const T*, const T*,   \
                                  const T*, const T*, const int*, double*);          \
-  template void launch_pass_a<T>(hipStream_t, int, bool, long long, T* const*,      \
-                                 T* const*, const T*, const T*, const int*,         \
+  template void launch_pass_a<T>(hipStream_t, int, bool, long long, Pair<T>,      \
+                                 Pair<T>, const T*, const T*, const int*,         \
                                  uint8_t*, T*, const T*, DevScalars*, double*,      \
                                  double*, double*, long long*);                     \
-  template void launch_p4<T>(hipStream_t, int, long long, T* const*, T* const*,     \
+  template void launch_p4<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,     \
                              const T*, const T*, const int*, uint8_t*, const T*,    \
                              T*, T*, DevScalars*, double*);                         \
   template void launch_p5a<T>(hipStream_t, int, long long, const T*, const T*,      \
                               const int*, const uint8_t*, const T*, const T*,       \
                               const T*, T*, DevScalars*, double*, long long*);      \
-  template void launch_p5b<T>(hipStream_t, int, long long, T* const*, T* const*,    \
+  template void launch_p5b<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
                               const T*, const T*, const int*, const uint8_t*, T*,   \
                               const T*, T*, DevScalars*, double*, double*);         \
-  template void launch_trial<T>(hipStream_t, int, long long, T* const*, const T*,   \
+  template void launch_trial<T>(hipStream_t, int, long long, Pair<T>, const T*,   \
                                 const T*, const DevScalars*);                       \
-  template void launch_gd<T>(hipStream_t, int, long long, T* const*, const T*,      \
+  template void launch_gd<T>(hipStream_t, int, long long, Pair<T>, const T*,      \
                              const DevScalars*, double*);
 
 LBFGSB_INST_BULK(float)

@@ -24,6 +24,17 @@ __host__ __device__ inline size_t widx(size_t i, int j) {
 }
 __host__ __device__ inline size_t wsize(size_t n) { return ((n + 63) / 64) * 16 * 64; }
 
+// The two buffers of a double-buffered n-vector (accepted point and trial);
+// kernels pick one with the device-side phase bit.  Passed by value, so the
+// compiler sees kernel-argument pointers (global memory, global_load) rather
+// than pointers read from memory (flat_load).
+template <typename T>
+struct Pair {
+  T* b0;
+  T* b1;
+  __host__ __device__ T* operator[](int s) const { return s ? b1 : b0; }
+};
+
 // ---------------------------------------------------------------------------
 // iwhere is kept as one byte per variable:
 //   bits 0..3: iwhere + 4   (values -3, -1, 0, 1, 2, 3 -> 1, 3, 4, 5, 6, 7)
@@ -72,6 +83,14 @@ __device__ inline void wave_argmin(double& v, long long& idx) {
   }
 }
 
+// acc += a * b with the product formed in FP64.  For FP32 operands the
+// product is exact in FP64, so a fused multiply-add rounds exactly like the
+// separate multiply and add; FP64 operands keep the separate operations.
+__device__ inline void acc_prod(double& acc, float a, float b) {
+  acc = __builtin_fma((double)a, (double)b, acc);
+}
+__device__ inline void acc_prod(double& acc, double a, double b) { acc += a * b; }
+
 // Error-free transformation a + b = s + e (Knuth two-sum).
 __device__ __host__ inline void two_sum(double a, double b, double& s, double& e) {
   s = a + b;
@@ -110,6 +129,25 @@ __device__ inline void block_sum_store(double (&vals)[NV], double* sh, double* o
     out[(size_t)v * out_stride] = s;
   }
   __syncthreads();
+}
+
+// Marks loaded values as needed at this point.  Placed after the last load
+// of an element, it keeps the compiler from sinking any of those loads into
+// later conditional code, where each would be issued only after the waits
+// before it: the element's loads go out together and the wave waits once.
+// Emits no instruction.
+__device__ inline void keep_one(float v) { asm volatile("" ::"v"(v)); }
+__device__ inline void keep_one(double v) { asm volatile("" ::"v"(v)); }
+__device__ inline void keep_one(int v) { asm volatile("" ::"v"(v)); }
+__device__ inline void keep_one(uint8_t v) { asm volatile("" ::"v"((int)v)); }
+template <typename T, int N>
+__device__ inline void keep_one(const T (&v)[N]) {
+#pragma unroll
+  for (int k = 0; k < N; k++) keep_one(v[k]);
+}
+template <typename... A>
+__device__ inline void keep_loads(const A&... a) {
+  (keep_one(a), ...);
 }
 
 template <typename T>

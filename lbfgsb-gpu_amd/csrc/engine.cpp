@@ -54,8 +54,8 @@ struct Engine final : EngineBase {
   hipStream_t st;
   T* X[2];
   T* Gr[2];
-  T** dX = nullptr;
-  T** dG = nullptr;
+  Pair<T> dX{nullptr, nullptr};  // X[0], X[1] as kernel arguments
+  Pair<T> dG{nullptr, nullptr};  // Gr[0], Gr[1]
   const T* l;
   const T* u;
   const int* nbd;
@@ -109,7 +109,7 @@ struct Engine final : EngineBase {
         dmalloc(dsub, nn) || dmalloc(iwhere, nn))
       return -1;
     HIPCHK(hipMemsetAsync(W, 0, wsize(nn) * sizeof(T), st));
-    if (dmalloc(dX, 2) || dmalloc(dG, 2) || dmalloc(ds, 1)) return -1;
+    if (dmalloc(ds, 1)) return -1;
     HIPCHK(hipHostMalloc((void**)&hs, sizeof(DevScalars), hipHostMallocDefault));
     std::memset(hs, 0, sizeof(DevScalars));
     const size_t nv = std::max<int>(P4_COUNT, PA_COUNT);
@@ -121,10 +121,8 @@ struct Engine final : EngineBase {
         dmalloc(tpre, 33 * ntiles) || dmalloc(total, 40) || dmalloc(tile_state, 33 * ntiles) ||
         dmalloc(last_state, 40) || dmalloc(tile_stop, ntiles))
       return -1;
-    T* hp[2] = {X[0], X[1]};
-    HIPCHK(hipMemcpyAsync(dX, hp, sizeof(hp), hipMemcpyHostToDevice, st));
-    T* hg[2] = {Gr[0], Gr[1]};
-    HIPCHK(hipMemcpyAsync(dG, hg, sizeof(hg), hipMemcpyHostToDevice, st));
+    dX = Pair<T>{X[0], X[1]};
+    dG = Pair<T>{Gr[0], Gr[1]};
     HIPCHK(hipStreamSynchronize(st));
     return 0;
   }
@@ -180,8 +178,6 @@ struct Engine final : EngineBase {
     T* bufs[] = {X[1], Gr[0], Gr[1], W, d, z, r, dsub};
     for (T* p : bufs) hipFree(p);
     hipFree(iwhere);
-    hipFree(dX);
-    hipFree(dG);
     hipFree(ds);
     hipHostFree(hs);
     hipFree(part);

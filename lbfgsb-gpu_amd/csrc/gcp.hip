@@ -56,7 +56,7 @@ __device__ inline bool breakpoint_of(T x, T g, T l, T u, int nb, int iw, T& t) {
 // entries beyond `cap` are not written).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_gcp_select(long long n, T* const* X, T* const* Gv, const T* __restrict__ l,
+k_gcp_select(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
              const T* __restrict__ u, const int* __restrict__ nbd,
              const uint8_t* __restrict__ iwhere, DevScalars* ds, Cand* cand, long long cap) {
   if (!ds->gcp_pending) return;
@@ -196,7 +196,7 @@ __device__ inline void contribution(const Cand& c, const T* __restrict__ g,
 // Layout of a state vector in this file: [P_y(8) P_s(8) Q_y(8) Q_s(8) D].
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_gcp_tile_reduce(const Cand* __restrict__ cand, T* const* Gv, const T* __restrict__ W,
+k_gcp_tile_reduce(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__ W,
                   const DevScalars* ds, long long cap, double* tsum, int tstride) {
   __shared__ double sh[kWavesPerBlock * kNV];
   if (!ds->gcp_pending) return;
@@ -291,7 +291,7 @@ __device__ inline bool stop_test(const DevScalars* ds, const double* st, double 
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_gcp_tile_eval(const Cand* __restrict__ cand, T* const* Gv, const T* __restrict__ W,
+k_gcp_tile_eval(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__ W,
                 const DevScalars* ds, long long cap, const double* tpre, int tstride,
                 long long* tile_stop, double* tile_state, double* last_state) {
   __shared__ double sh[kWavesPerBlock * kNV];
@@ -478,7 +478,7 @@ __global__ void s_gcp_round_end(DevScalars* ds, const Cand* __restrict__ cand, l
 static constexpr int kSmallRound = 2048;
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-s_gcp_round_small(const Cand* __restrict__ cand, T* const* Gv, const T* __restrict__ W,
+s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__ W,
                   DevScalars* ds) {
   __shared__ Cand sc[kSmallRound];
   __shared__ double swave[kWavesPerBlock * kNV];
@@ -584,7 +584,7 @@ s_gcp_round_small(const Cand* __restrict__ cand, T* const* Gv, const T* __restri
 
 // ---------------------------------------------------------------------------
 template <typename T>
-void launch_gcp_select(hipStream_t s, int G, long long n, T* const* X, T* const* Gv,
+void launch_gcp_select(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
                        const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
                        DevScalars* ds, Cand* ca, long long buf_cap) {
   hipLaunchKernelGGL(k_gcp_select<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd, iwhere,
@@ -594,7 +594,7 @@ void launch_gcp_select(hipStream_t s, int G, long long n, T* const* X, T* const*
 // Sort + scan + round end for at most `cap` candidates (the device count
 // decides how much of the launch does work).
 template <typename T>
-void launch_gcp_sortscan(hipStream_t s, T* const* Gv, const T* W, DevScalars* ds, Cand* ca,
+void launch_gcp_sortscan(hipStream_t s, Pair<T> Gv, const T* W, DevScalars* ds, Cand* ca,
                          Cand* cb, long long cap, const GcpScratch& w) {
   if (cap < 1) cap = 1;
   const long long tiles = (cap + kSortTile - 1) / kSortTile;
@@ -622,18 +622,18 @@ void launch_gcp_sortscan(hipStream_t s, T* const* Gv, const T* W, DevScalars* ds
 }
 
 template <typename T>
-void launch_gcp_round_small(hipStream_t s, const Cand* ca, T* const* Gv, const T* W,
+void launch_gcp_round_small(hipStream_t s, const Cand* ca, Pair<T> Gv, const T* W,
                             DevScalars* ds) {
   hipLaunchKernelGGL(s_gcp_round_small<T>, dim3(1), dim3(kBlock), 0, s, ca, Gv, W, ds);
 }
 
 #define LBFGSB_INST_GCP(T)                                                                  \
-  template void launch_gcp_round_small<T>(hipStream_t, const Cand*, T* const*, const T*,   \
+  template void launch_gcp_round_small<T>(hipStream_t, const Cand*, Pair<T>, const T*,   \
                                           DevScalars*);                                    \
-  template void launch_gcp_select<T>(hipStream_t, int, long long, T* const*, T* const*,    \
+  template void launch_gcp_select<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
                                      const T*, const T*, const int*, const uint8_t*,       \
                                      DevScalars*, Cand*, long long);                       \
-  template void launch_gcp_sortscan<T>(hipStream_t, T* const*, const T*, DevScalars*,      \
+  template void launch_gcp_sortscan<T>(hipStream_t, Pair<T>, const T*, DevScalars*,      \
                                        Cand*, Cand*, long long, const GcpScratch&);
 
 LBFGSB_INST_GCP(float)

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.hpp"
 #include "state.hpp"
 
 namespace lbfgsb {
@@ -27,12 +28,12 @@ template <typename T>
 void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g, const T* l,
                    const T* u, const int* nbd, double* pmax);
 template <typename T>
-void launch_pass_a(hipStream_t s, int G, bool fused, long long n, T* const* X, T* const* Gv,
+void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X, Pair<T> Gv,
                    const T* l, const T* u, const int* nbd, uint8_t* iwhere, T* W, const T* d,
                    DevScalars* ds, double* part, double* pmax, double* pmin,
                    long long* pmini);
 template <typename T>
-void launch_p4(hipStream_t s, int G, long long n, T* const* X, T* const* Gv, const T* l,
+void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                const T* u, const int* nbd, uint8_t* iwhere, const T* W, T* z, T* r,
                DevScalars* ds, double* part);
 template <typename T>
@@ -40,14 +41,14 @@ void launch_p5a(hipStream_t s, int G, long long n, const T* l, const T* u, const
                 const uint8_t* iwhere, const T* W, const T* z, const T* r, T* dsub,
                 DevScalars* ds, double* pmin, long long* pmini);
 template <typename T>
-void launch_p5b(hipStream_t s, int G, long long n, T* const* X, T* const* Gv, const T* l,
+void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                 const T* u, const int* nbd, const uint8_t* iwhere, T* z, const T* dsub, T* d,
                 DevScalars* ds, double* part, double* pmin);
 template <typename T>
-void launch_trial(hipStream_t s, int G, long long n, T* const* X, const T* z, const T* d,
+void launch_trial(hipStream_t s, int G, long long n, Pair<T> X, const T* z, const T* d,
                   const DevScalars* ds);
 template <typename T>
-void launch_gd(hipStream_t s, int G, long long n, T* const* Gv, const T* d,
+void launch_gd(hipStream_t s, int G, long long n, Pair<T> Gv, const T* d,
                const DevScalars* ds, double* part);
 
 // small.hip
@@ -79,14 +80,14 @@ struct GcpScratch {
   double* last_state;
 };
 template <typename T>
-void launch_gcp_select(hipStream_t s, int G, long long n, T* const* X, T* const* Gv,
+void launch_gcp_select(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
                        const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
                        DevScalars* ds, Cand* ca, long long buf_cap);
 template <typename T>
-void launch_gcp_round_small(hipStream_t s, const Cand* ca, T* const* Gv, const T* W,
+void launch_gcp_round_small(hipStream_t s, const Cand* ca, Pair<T> Gv, const T* W,
                             DevScalars* ds);
 template <typename T>
-void launch_gcp_sortscan(hipStream_t s, T* const* Gv, const T* W, DevScalars* ds, Cand* ca,
+void launch_gcp_sortscan(hipStream_t s, Pair<T> Gv, const T* W, DevScalars* ds, Cand* ca,
                          Cand* cb, long long cap, const GcpScratch& w);
 
 }  // namespace lbfgsb
