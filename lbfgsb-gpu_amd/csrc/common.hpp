@@ -150,6 +150,65 @@ __device__ inline void keep_loads(const A&... a) {
   (keep_one(a), ...);
 }
 
+// ---------------------------------------------------------------------------
+// Reading the S/Y memory of one element.
+// Ring slot of each memory position j = 0..kMaxM-1 (j = 0 oldest).
+__device__ inline void ring_slots(int head, int m, int (&slot)[kMaxM]) {
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    slot[j] = head + j;
+    if (slot[j] >= m) slot[j] -= m;
+  }
+}
+
+// Which W slots the kernel reads for each memory position j (uniform over
+// the grid).  Positions that are not used (j >= col, or the slot that is
+// being overwritten, skip_slot) read the slot of the first used position
+// instead: an address the same wave is reading anyway, so the load hits in
+// cache and the loop needs no branch around it.
+struct RingReads {
+  int rs[kMaxM];
+  bool use[kMaxM];
+  int newest;  // slot of position col - 1 (0 when the memory is empty)
+};
+__device__ inline RingReads ring_reads(int head, int m, int col, int skip_slot) {
+  RingReads r;
+  int slot[kMaxM];
+  ring_slots(head, m, slot);
+  r.newest = col > 0 ? (head + col - 1) % m : 0;
+  int fb = -1;
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    r.use[j] = j < col && slot[j] != skip_slot;
+    if (r.use[j] && fb < 0) fb = slot[j];
+  }
+  if (fb < 0) fb = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) r.rs[j] = r.use[j] ? slot[j] : fb;
+  return r;
+}
+
+// The retained Y and S entries of element i in ring order (0 for unused
+// positions).  The 16 loads are unconditional and issued ahead of any use,
+// so a wave has the whole W row in flight at once (one 256 B segment per
+// slot in FP32) instead of waiting on each pair.
+template <typename T>
+__device__ inline void load_ring(const T* __restrict__ W, long long i, const RingReads& rr,
+                                 T (&yv)[kMaxM], T (&sv)[kMaxM]) {
+  const T* __restrict__ row = W + widx((size_t)i, 0);
+  T ly[kMaxM], ls[kMaxM];
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    ly[j] = row[rr.rs[j] * kTile];
+    ls[j] = row[(8 + rr.rs[j]) * kTile];
+  }
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    yv[j] = rr.use[j] ? ly[j] : (T)0;
+    sv[j] = rr.use[j] ? ls[j] : (T)0;
+  }
+}
+
 template <typename T>
 __device__ inline T ld_nt(const T* p) {
   return __builtin_nontemporal_load(p);

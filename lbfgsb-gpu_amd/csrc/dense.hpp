@@ -75,7 +75,6 @@ __device__ inline int d_dtrsl(const double* t, int ldt, int n, double* b, int jo
 __device__ inline int d_bmv(const double* sy, const double* wt, int col, const double* v,
                             double* p) {
   if (col == 0) return 0;
-  double work[8];
   p[col] = v[col];
   for (int i = 2; i <= col; i++) {
     int i2 = col + i;
@@ -83,14 +82,12 @@ __device__ inline int d_bmv(const double* sy, const double* wt, int col, const d
     for (int k = 1; k <= i - 1; k++) s = s + A1(sy, 8, i, k) * v[k - 1] / A1(sy, 8, k, k);
     p[i2 - 1] = v[i2 - 1] + s;
   }
-  for (int i = 0; i < col; i++) work[i] = p[col + i];
-  int info = d_dtrsl(wt, 8, col, work, 11);
-  for (int i = 0; i < col; i++) p[col + i] = work[i];
+  // the two triangular solves work in place on p[col .. 2col-1] (no
+  // per-thread temporaries: those would live in scratch memory)
+  int info = d_dtrsl(wt, 8, col, p + col, 11);
   if (info != 0) return info;
   for (int i = 1; i <= col; i++) p[i - 1] = v[i - 1] / sqrt(A1(sy, 8, i, i));
-  for (int i = 0; i < col; i++) work[i] = p[col + i];
-  info = d_dtrsl(wt, 8, col, work, 1);
-  for (int i = 0; i < col; i++) p[col + i] = work[i];
+  info = d_dtrsl(wt, 8, col, p + col, 1);
   if (info != 0) return info;
   for (int i = 1; i <= col; i++) p[i - 1] = -p[i - 1] / sqrt(A1(sy, 8, i, i));
   for (int i = 1; i <= col; i++) {

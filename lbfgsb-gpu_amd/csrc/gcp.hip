@@ -172,25 +172,43 @@ k_merge(const Cand* __restrict__ src, Cand* __restrict__ dst, const DevScalars* 
 // Contribution of candidate c: [d w (2col) ; t d w (2col) ; d^2].
 template <typename T>
 __device__ inline void contribution(const Cand& c, const T* __restrict__ g,
-                                    const T* __restrict__ W, int col, int head, int m,
+                                    const T* __restrict__ W, const RingReads& rr,
                                     double theta, double (&v)[kNV]) {
   const long long i = c.idx;
-  const double d = -(double)g[i];
+  // the whole W row and g_i are loaded before any use (positions j >= col
+  // read as zero)
+  T yv[kMaxM], sv[kMaxM];
+  const T gi = g[i];
+  load_ring(W, i, rr, yv, sv);
+  const double d = -(double)gi;
 #pragma unroll
   for (int j = 0; j < kMaxM; j++) {
-    double wy = 0.0, ws = 0.0;
-    if (j < col) {
-      int slot = head + j;
-      if (slot >= m) slot -= m;
-      wy = (double)W[widx(i, slot)];
-      ws = theta * (double)W[widx(i, 8 + slot)];
-    }
+    const double wy = (double)yv[j];
+    const double ws = theta * (double)sv[j];
     v[j] = d * wy;
     v[8 + j] = d * ws;
     v[16 + j] = c.t * d * wy;
     v[24 + j] = c.t * d * ws;
   }
   v[32] = d * d;
+}
+
+// p at the start of the search and the middle matrix M in the padded
+// layout of the breakpoint sums (positions 0..7 Y, 8..15 S, zero beyond
+// col), staged in LDS by every thread of the block.
+struct GcpConst {
+  double pp[16];
+  double Mp[256];
+};
+__device__ inline void stage_gcp_const(const DevScalars* ds, GcpConst& gc) {
+  const int col = ds->col;
+  for (int q = threadIdx.x; q < 256; q += blockDim.x) gc.Mp[q] = ds->Mx[q];
+  if (threadIdx.x < 16) {
+    const int q = threadIdx.x;
+    const int j = q & 7;
+    gc.pp[q] = j < col ? ds->p[(q < 8 ? 0 : col) + j] : 0.0;
+  }
+  __syncthreads();
 }
 
 // Layout of a state vector in this file: [P_y(8) P_s(8) Q_y(8) Q_s(8) D].
@@ -202,13 +220,13 @@ k_gcp_tile_reduce(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
   if (!ds->gcp_pending) return;
   const long long K = round_size(ds, cap);
   if ((long long)blockIdx.x * kBlock >= K) return;
-  const int col = ds->col;
+  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1);
   const T* g = Gv[ds->phase & 1];
   double v[kNV];
 #pragma unroll
   for (int q = 0; q < kNV; q++) v[q] = 0.0;
   const long long pos = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (pos < K) contribution(cand[pos], g, W, col, ds->head, ds->m, ds->theta, v);
+  if (pos < K) contribution(cand[pos], g, W, rr, ds->theta, v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < kNV; q++) {
@@ -254,25 +272,28 @@ __global__ void s_gcp_tile_scan(DevScalars* ds, long long cap, const double* tsu
 }
 
 // Stop test at the state `st` (passed breakpoints before t_next).
-__device__ inline bool stop_test(const DevScalars* ds, const double* st, double t_prev,
-                                 double t_next, bool floor_on, double* out) {
-  const int col = ds->col;
+// All loops run over the padded 16 positions with compile-time indices (the
+// entries beyond col are exact zeros, which leaves every sum unchanged), so
+// the temporaries stay in registers.
+__device__ inline bool stop_test(const DevScalars* ds, const GcpConst& gc,
+                                 const double (&st)[kNV], double t_prev, double t_next,
+                                 bool floor_on, double (&out)[3]) {
   const double theta = ds->theta;
   double p[16], mp[16];
-  for (int j = 0; j < col; j++) {
-    p[j] = ds->p[j] - st[j];
-    p[col + j] = ds->p[col + j] - st[8 + j];
-  }
-  const int c2 = 2 * col;
-  for (int r = 0; r < c2; r++) {
+#pragma unroll
+  for (int q = 0; q < 16; q++) p[q] = gc.pp[q] - st[q];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
     double s = 0.0;
-    for (int k = 0; k < c2; k++) s += ds->Mx[r * 16 + k] * p[k];
+#pragma unroll
+    for (int k = 0; k < 16; k++) s += gc.Mp[r * 16 + k] * p[k];
     mp[r] = s;
   }
   double pmp = 0.0, qmp = 0.0;
-  for (int j = 0; j < col; j++) {
-    pmp += p[j] * mp[j] + p[col + j] * mp[col + j];
-    qmp += st[16 + j] * mp[j] + st[24 + j] * mp[col + j];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    pmp += p[j] * mp[j] + p[8 + j] * mp[8 + j];
+    qmp += st[16 + j] * mp[j] + st[24 + j] * mp[8 + j];
   }
   const double D = ds->gcp_D0 - st[32];
   const double f2u = theta * D - pmp;
@@ -296,10 +317,12 @@ k_gcp_tile_eval(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__
                 long long* tile_stop, double* tile_state, double* last_state) {
   __shared__ double sh[kWavesPerBlock * kNV];
   __shared__ long long sstop;
+  __shared__ GcpConst gc;
   if (!ds->gcp_pending) return;
   const long long K = round_size(ds, cap);
   if ((long long)blockIdx.x * kBlock >= K) return;
-  const int col = ds->col;
+  stage_gcp_const(ds, gc);
+  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1);
   const T* g = Gv[ds->phase & 1];
   const long long pos = (long long)blockIdx.x * kBlock + threadIdx.x;
   double v[kNV], ex[kNV];
@@ -311,7 +334,7 @@ k_gcp_tile_eval(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__
   c.pad = 0;
   if (pos < K) {
     c = cand[pos];
-    contribution(c, g, W, col, ds->head, ds->m, ds->theta, v);
+    contribution(c, g, W, rr, ds->theta, v);
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (threadIdx.x == 0) sstop = 0x7fffffffffffffffLL;
@@ -338,7 +361,7 @@ k_gcp_tile_eval(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__
     if (k > 0) {
       const double t_prev = pos > 0 ? cand[pos - 1].t : ds->gcp_t_prev;
       double out[3];
-      if (stop_test(ds, ex, t_prev, c.t, true, out)) atomicMin(&sstop, pos);
+      if (stop_test(ds, gc, ex, t_prev, c.t, true, out)) atomicMin(&sstop, pos);
     }
     if (pos == K - 1)
       for (int q = 0; q < kNV; q++) last_state[q] = ex[q];
@@ -362,11 +385,11 @@ __device__ void gcp_done(DevScalars* ds) {
 // search stops (its exclusive state in `st`), or -1 when every candidate
 // passed (then `st` is the inclusive total).  last_state = exclusive state
 // before the last candidate.
-__device__ void round_conclude(DevScalars* ds, const Cand* cand, long long K, long long stop_pos,
-                               const double* stv, const double* last_state) {
+__device__ void round_conclude(DevScalars* ds, const GcpConst& gc, const Cand* cand, long long K,
+                               long long stop_pos, const double* stv, const double* last_state) {
   const int col = ds->col;
-  double st[kNV];
-  for (int q = 0; q < kNV; q++) st[q] = stv[q];
+  // the state stays where the caller keeps it (LDS or global), read in place
+  const double(&st)[kNV] = *reinterpret_cast<const double(*)[kNV]>(stv);
   double tk, dtm = 0.0;
   long long ik, kpass;
   if (stop_pos >= 0) {
@@ -375,7 +398,7 @@ __device__ void round_conclude(DevScalars* ds, const Cand* cand, long long K, lo
     ik = pos > 0 ? cand[pos - 1].idx : ds->gcp_idx_prev;
     kpass = ds->gcp_kpassed + pos;
     double out[3];
-    stop_test(ds, st, tk, cand[pos].t, kpass > 0, out);
+    stop_test(ds, gc, st, tk, cand[pos].t, kpass > 0, out);
     dtm = out[0];
   } else {
     for (int q = 0; q < kNV; q++) ds->gcp_carry[q] = st[q];
@@ -419,7 +442,7 @@ __device__ void round_conclude(DevScalars* ds, const Cand* cand, long long K, lo
       return;
     }
     double out[3];
-    stop_test(ds, st, tk, tk, true, out);
+    stop_test(ds, gc, st, tk, tk, true, out);
     dtm = ds->gcp_unbnd ? out[0] : 0.0;
   }
   if (dtm <= 0) dtm = 0;
@@ -458,6 +481,8 @@ __global__ void s_gcp_round_end(DevScalars* ds, const Cand* __restrict__ cand, l
     }
     return;
   }
+  __shared__ GcpConst gc;
+  stage_gcp_const(ds, gc);
   const long long K = (long long)ds->cand_count;
   const int ntiles = (int)((K + kBlock - 1) / kBlock);
   if (threadIdx.x == 0) sbest = 0x7fffffffffffffffLL;
@@ -468,7 +493,7 @@ __global__ void s_gcp_round_end(DevScalars* ds, const Cand* __restrict__ cand, l
   if (threadIdx.x != 0) return;
   const bool stopped = sbest != 0x7fffffffffffffffLL;
   const double* st = stopped ? tile_state + (size_t)(sbest / kBlock) * kNV : total;
-  round_conclude(ds, cand, K, stopped ? sbest : -1, st, last_state);
+  round_conclude(ds, gc, cand, K, stopped ? sbest : -1, st, last_state);
 }
 
 // A whole round in one workgroup when it has at most kSmallRound candidates
@@ -486,6 +511,7 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
   __shared__ double sstop_state[kNV];
   __shared__ double slast[kNV];
   __shared__ long long sstop;
+  __shared__ GcpConst gc;
   if (!ds->gcp_pending || ds->halt) return;
   const long long K = (long long)ds->cand_count;
   if (K > kSmallRound) {
@@ -495,6 +521,7 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
     }
     return;
   }
+  stage_gcp_const(ds, gc);
   int P = 64;
   while (P < K) P <<= 1;
   for (int j = threadIdx.x; j < P; j += blockDim.x) {
@@ -523,7 +550,7 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
       __syncthreads();
     }
   }
-  const int col = ds->col, head = ds->head, m = ds->m;
+  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1);
   const double theta = ds->theta;
   const long long kpassed = ds->gcp_kpassed;
   const double t_prev0 = ds->gcp_t_prev;
@@ -534,7 +561,7 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
     double v[kNV], ex[kNV];
 #pragma unroll
     for (int q = 0; q < kNV; q++) v[q] = 0.0;
-    if (pos < K) contribution(sc[pos], g, W, col, head, m, theta, v);
+    if (pos < K) contribution(sc[pos], g, W, rr, theta, v);
 #pragma unroll
     for (int q = 0; q < kNV; q++) {
       double incl = v[q];
@@ -557,7 +584,7 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
     if (pos < K && kpassed + pos > 0) {
       const double tp = pos > 0 ? sc[pos - 1].t : t_prev0;
       double out[3];
-      stop = stop_test(ds, ex, tp, sc[pos].t, true, out);
+      stop = stop_test(ds, gc, ex, tp, sc[pos].t, true, out);
     }
     if (stop) atomicMin(&sstop, pos);
     if (pos == K - 1)
@@ -579,7 +606,7 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
   __syncthreads();
   if (threadIdx.x != 0) return;
   const bool stopped = sstop != 0x7fffffffffffffffLL;
-  round_conclude(ds, sc, K, stopped ? sstop : -1, stopped ? sstop_state : scarry, slast);
+  round_conclude(ds, gc, sc, K, stopped ? sstop : -1, stopped ? sstop_state : scarry, slast);
 }
 
 // ---------------------------------------------------------------------------

@@ -314,12 +314,23 @@ __device__ void gcp_controller(DevScalars* ds) {
 // 0 .. 2col-1 (sy, wt in LDS).
 __device__ void build_middle(DevScalars* ds) {
   const int col = ds->col;
-  if (ds->gcp_pending && (int)threadIdx.x < 2 * col) {
-    double e[16], out[16];
+  // per-thread unit vector and result rows in LDS (not scratch memory)
+  __shared__ double s_e[16][17], s_out[16][17];
+  if (!ds->gcp_pending) return;
+  // Mx is stored in the padded layout of the breakpoint sums: index a of
+  // the packed 2col vector [y-part ; s-part] goes to a (a < col) or
+  // 8 + a - col; the other entries are zero
+  for (int q = threadIdx.x; q < 256; q += blockDim.x) ds->Mx[q] = 0.0;
+  __syncthreads();
+  if ((int)threadIdx.x < 2 * col) {
+    double* e = s_e[threadIdx.x];
+    double* out = s_out[threadIdx.x];
     for (int i = 0; i < 16; i++) e[i] = 0.0;
     e[threadIdx.x] = 1.0;
     d_bmv(ds->sy, ds->wt, col, e, out);
-    for (int r = 0; r < 2 * col; r++) ds->Mx[r * 16 + threadIdx.x] = out[r];
+    const int k = (int)threadIdx.x;
+    const int kp = k < col ? k : 8 + k - col;
+    for (int r = 0; r < 2 * col; r++) ds->Mx[(r < col ? r : 8 + r - col) * 16 + kp] = out[r];
   }
 }
 
@@ -499,7 +510,7 @@ __device__ bool formk_device(DevScalars* ds, const double* nc, const double* cor
     return false;
   }
   const int col2 = 2 * col;
-  double work[8];
+  double* work = ds->tmp;  // 8 doubles
   for (int js = col + 1; js <= col2; js++) {
     for (int i = 1; i <= col; i++) work[i - 1] = A1(wn, 16, i, js);
     d_dtrsl(wn, 16, col, work, 11);
@@ -509,7 +520,7 @@ __device__ bool formk_device(DevScalars* ds, const double* nc, const double* cor
     for (int js = iis; js <= col2; js++)
       A1(wn, 16, iis, js) = A1(wn, 16, iis, js) +
                             ddot(&A1(wn, 16, 1, iis), 16, &A1(wn, 16, 1, js), 16, col);
-  double wm[64];
+  double* wm = ds->tmp + 16;  // 8 x 8
   for (int i = 1; i <= col; i++)
     for (int k = 1; k <= col; k++) A1(wm, 8, i, k) = A1(wn, 16, col + i, col + k);
   if (!d_dpofa(wm, 8, col)) {

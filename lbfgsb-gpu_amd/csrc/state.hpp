@@ -107,6 +107,9 @@ struct DevScalars {
   double p[16], c[16], v[16], wv[16], mc[16];
   double newcol[32];
   double pending_s_ss_diag;
+  // workspace of the small dense routines; the control kernels stage this
+  // struct in LDS, so temporaries placed here avoid per-thread scratch memory
+  double tmp[80];
 
   // ---- reduced pass-A values (see PartA)
   double pa[PA_COUNT];
