@@ -159,6 +159,41 @@ int lbfgsb_problem_funcgrad_f32(void* ctx, float* x, float* f, float* g, void* s
 int lbfgsb_problem_eval_device(lbfgsb_problem* p, const void* x, void* g, double* f_dev,
                                void* stream);
 
+/* ---- Batched form (no counterpart in the reference, which solves one
+ * problem per call).  Many independent problems with built-in objectives in
+ * one arena of HBM slots, solved by a persistent kernel in which each
+ * workgroup runs whole problems with the iteration chain of the plain form
+ * (the same decisions, block-level reductions).  Used for batches of small
+ * problems and by the serving loop.
+ *   capacity      elements of all admitted problems together (the arena
+ *                 holds about 160 B (fp32) / 290 B (fp64) per element)
+ *   max_problems  slots
+ * _add copies the problem's device arrays (x0, nbd, l, u and the objective
+ * data its kind uses: kind 1 b; 2 b, c; 3 none; 4 a0, a1, a2, b) into a free
+ * slot and returns the slot, -1 on bad input (the checks of lbfgsbminimize),
+ * -3 when no slot or range is free.  _run advances every unfinished slot,
+ * largest first, by at most iter_budget iterations (0 = to completion) and
+ * returns how many remain unfinished (negative on a device error); threads
+ * is 256 or 512 per problem, blocks 0 = the device's concurrent capacity.
+ * Results follow the summary conventions (info as in lbfgsb_summary). */
+typedef struct lbfgsb_batch lbfgsb_batch;
+lbfgsb_batch* lbfgsb_batch_create(int dtype, long long capacity, int max_problems, int m,
+                                  void* stream);
+void lbfgsb_batch_destroy(lbfgsb_batch* b);
+int lbfgsb_batch_add(lbfgsb_batch* b, long long n, const void* x0, const int* nbd, const void* l,
+                     const void* u, int kind, long long nx, long long ny, double h, const void* bv,
+                     const void* c, const void* a0, const void* a1, const void* a2, double eps_g,
+                     double eps_f, double eps_x, int max_iteration);
+int lbfgsb_batch_run(lbfgsb_batch* b, int threads, int blocks, int iter_budget);
+int lbfgsb_batch_result(lbfgsb_batch* b, int slot, double* f, int* iter, int* info,
+                        double* residual_g, int* done);
+/* f, iter, info, done for all max_problems slots (host arrays). */
+int lbfgsb_batch_results(lbfgsb_batch* b, double* f, int* iter, int* info, int* done);
+/* The slot's current iterate into a device array of n elements. */
+int lbfgsb_batch_get_x(lbfgsb_batch* b, int slot, void* out);
+int lbfgsb_batch_remove(lbfgsb_batch* b, int slot);
+long long lbfgsb_batch_free_elems(lbfgsb_batch* b);
+
 /* Library version string. */
 const char* lbfgsb_amd_version(void);
 

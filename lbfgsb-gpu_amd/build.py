@@ -24,8 +24,8 @@ LIB = os.path.join(LIBDIR, "liblbfgsb_amd.so")
 EXAMPLE_SRC = os.path.join(HERE, "examples", "dsscfg.hip")
 EXAMPLE = os.path.join(LIBDIR, "dsscfg_example")
 
-SOURCES = ["bulk.hip", "small.hip", "gcp.hip", "freev.hip", "objectives.hip",
-           "engine.cpp", "capi.cpp"]
+SOURCES = ["batch.hip", "bulk.hip", "small.hip", "gcp.hip", "freev.hip", "objectives.hip",
+           "engine.cpp", "capi.cpp", "batch_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LBFGSB_AMD_ARCH", "gfx950")

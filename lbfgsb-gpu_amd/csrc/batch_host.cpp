@@ -1,0 +1,352 @@
+// Host side of the batched form: an arena of problem slots in HBM and the
+// launches of the persistent per-problem kernel (batch.hip).
+//
+// All per-problem arrays live in slices of arena-wide arrays, allocated once
+// (first fit over 64-element-aligned ranges, so a problem's W block starts at
+// 16 x its offset).  A slot holds one problem from admission to removal;
+// `run` launches the kernel over every unfinished slot, largest first, either
+// to completion or for an iteration budget (the serving loop resumes the
+// suspended problems in its next launch).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "../../include/lbfgsb_amd.h"
+#include "batch.hpp"
+#include "common.hpp"
+
+using namespace lbfgsb;
+
+namespace {
+
+struct Slot {
+  bool used = false;
+  bool done = false;
+  long long off = 0, n = 0, npad = 0;
+};
+
+struct BatchBase {
+  virtual ~BatchBase() {}
+  virtual int add(long long n, const void* x0, const int* nbd, const void* l, const void* u,
+                  int kind, long long nx, long long ny, double h, const void* b, const void* c,
+                  const void* a0, const void* a1, const void* a2, double eps_g, double eps_f,
+                  double eps_x, int max_iteration) = 0;
+  virtual int run(int threads, int blocks, int iter_budget) = 0;
+  virtual int result(int slot, double* f, int* iter, int* info, double* residual_g,
+                     int* done) = 0;
+  virtual int get_x(int slot, void* out) = 0;
+  virtual int remove(int slot) = 0;
+  virtual long long free_elems() const = 0;
+  virtual int results_all(double* f, int* iter, int* info, int* done) = 0;
+  bool ok = false;
+};
+
+#define BCHK(expr)                    \
+  do {                                \
+    if ((expr) != hipSuccess) return -2; \
+  } while (0)
+
+template <typename T>
+struct Arena final : BatchBase {
+  hipStream_t st;
+  int m;
+  long long cap;  // elements, multiple of 64
+  int maxp;
+  T *x = nullptr, *xb = nullptr, *g = nullptr, *gb = nullptr, *l = nullptr, *u = nullptr;
+  T *W = nullptr, *d = nullptr, *z = nullptr, *r = nullptr, *dsub = nullptr;
+  T *ob = nullptr, *oc = nullptr, *oa0 = nullptr, *oa1 = nullptr, *oa2 = nullptr;
+  int* nbd = nullptr;
+  uint8_t* iwhere = nullptr;
+  Cand *ca = nullptr, *cb = nullptr;
+  DevScalars* dsa = nullptr;
+  BatchResult* res = nullptr;
+  BatchProb* probs = nullptr;
+  int* order = nullptr;
+  unsigned* next = nullptr;
+  std::vector<Slot> slots;
+  std::vector<BatchProb> hprobs;
+  std::vector<BatchResult> hres;
+  std::map<long long, long long> freel;  // free ranges: offset -> length
+
+  Arena(long long capacity, int max_problems, int m_, hipStream_t s) : st(s), m(m_) {
+    cap = ((capacity + 63) / 64) * 64;
+    maxp = max_problems;
+    ok = alloc() == 0;
+    slots.resize(maxp);
+    hprobs.resize(maxp);
+    hres.resize(maxp);
+    std::memset(hprobs.data(), 0, sizeof(BatchProb) * hprobs.size());
+    std::memset(hres.data(), 0, sizeof(BatchResult) * hres.size());
+    if (cap > 0) freel[0] = cap;
+  }
+  template <typename P>
+  int dm(P*& p, size_t count) {
+    BCHK(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(P)));
+    return 0;
+  }
+  int alloc() {
+    const size_t c = (size_t)cap;
+    T** vec[] = {&x, &xb, &g, &gb, &l, &u, &d, &z, &r, &dsub, &ob, &oc, &oa0, &oa1, &oa2};
+    for (T** p : vec)
+      if (dm(*p, c)) return -1;
+    if (dm(W, c * 16) || dm(nbd, c) || dm(iwhere, c) || dm(ca, c) || dm(cb, c)) return -1;
+    if (dm(dsa, maxp) || dm(res, maxp) || dm(probs, maxp) || dm(order, maxp) || dm(next, 1))
+      return -1;
+    BCHK(hipMemsetAsync(W, 0, c * 16 * sizeof(T), st));
+    BCHK(hipStreamSynchronize(st));
+    return 0;
+  }
+  ~Arena() override {
+    hipStreamSynchronize(st);
+    void* ptrs[] = {x,   xb,  g,   gb,     l,  u,  W,   d,   z,     r,   dsub, ob,    oc,
+                    oa0, oa1, oa2, nbd,    iwhere, ca, cb, dsa, res, probs, order, next};
+    for (void* p : ptrs)
+      if (p) hipFree(p);
+  }
+
+  long long free_elems() const override {
+    long long s = 0;
+    for (const auto& kv : freel) s += kv.second;
+    return s;
+  }
+  long long take_range(long long len) {
+    for (auto it = freel.begin(); it != freel.end(); ++it) {
+      if (it->second >= len) {
+        const long long off = it->first, rest = it->second - len;
+        freel.erase(it);
+        if (rest > 0) freel[off + len] = rest;
+        return off;
+      }
+    }
+    return -1;
+  }
+  void give_range(long long off, long long len) {
+    auto it = freel.emplace(off, len).first;
+    auto nx = std::next(it);
+    if (nx != freel.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      freel.erase(nx);
+    }
+    if (it != freel.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        freel.erase(it);
+      }
+    }
+  }
+
+  int add(long long n, const void* x0, const int* nbd_in, const void* l_in, const void* u_in,
+          int kind, long long nx, long long ny, double h, const void* b, const void* c,
+          const void* a0, const void* a1, const void* a2, double eps_g, double eps_f,
+          double eps_x, int max_iteration) override {
+    // errclb's scalar checks (lbfgsbcpu.cpp:879-890) and the option checks
+    if (n <= 0 || m <= 0 || m > n || eps_f < 0 || eps_g < 0 || eps_x < 0 || max_iteration < 0)
+      return -1;
+    if (kind < OBJK_GRID_QUAD || kind > OBJK_PENTA) return -1;
+    if ((kind == OBJK_GRID_QUAD || kind == OBJK_GRID_COS) && (nx <= 0 || ny <= 0 || nx * ny != n))
+      return -1;
+    if (n >= (1LL << 31)) return -1;  // candidate indices are 32-bit
+    int slot = -1;
+    for (int k = 0; k < maxp; k++)
+      if (!slots[k].used) {
+        slot = k;
+        break;
+      }
+    if (slot < 0) return -3;
+    const long long npad = ((n + 63) / 64) * 64;
+    const long long off = take_range(npad);
+    if (off < 0) return -3;
+    Slot& s = slots[slot];
+    s.used = true;
+    s.done = false;
+    s.off = off;
+    s.n = n;
+    s.npad = npad;
+    const size_t bytes = (size_t)n * sizeof(T);
+    auto cp = [&](T* dst, const void* src) -> int {
+      if (src) BCHK(hipMemcpyAsync(dst + off, src, bytes, hipMemcpyDeviceToDevice, st));
+      return 0;
+    };
+    if (cp(x, x0) || cp(l, l_in) || cp(u, u_in) || cp(ob, b) || cp(oc, c) || cp(oa0, a0) ||
+        cp(oa1, a1) || cp(oa2, a2))
+      return -2;
+    BCHK(hipMemcpyAsync(nbd + off, nbd_in, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, st));
+    // the memory slots of a reused range may hold another problem's pairs:
+    // they are multiplied by zero coefficients only, but keep them finite
+    BCHK(hipMemsetAsync(W + off * 16, 0, (size_t)npad * 16 * sizeof(T), st));
+    BatchProb& p = hprobs[slot];
+    p.off = off;
+    p.n = n;
+    p.woff = off * 16;
+    p.obj.kind = kind;
+    p.obj.nx = nx;
+    p.obj.ny = ny;
+    p.obj.h = h;
+    p.obj.b = ob + off;
+    p.obj.c = oc + off;
+    p.obj.a0 = oa0 + off;
+    p.obj.a1 = oa1 + off;
+    p.obj.a2 = oa2 + off;
+    // the state the host driver starts from (capi.cpp: drive)
+    DevScalars ds;
+    std::memset(&ds, 0, sizeof(ds));
+    ds.n = n;
+    ds.m = m;
+    ds.theta = 1.0;
+    ds.nfree = (int)n;
+    ds.eps_g = eps_g;
+    ds.eps_f = eps_f;
+    ds.eps_x = eps_x;
+    ds.max_iteration = max_iteration;
+    ds.ibd = -1;
+    BCHK(hipMemcpyAsync(dsa + slot, &ds, sizeof(ds), hipMemcpyHostToDevice, st));
+    std::memset(&hres[slot], 0, sizeof(BatchResult));
+    BCHK(hipStreamSynchronize(st));
+    return slot;
+  }
+
+  int run(int threads, int blocks, int iter_budget) override {
+    std::vector<int> ord;
+    for (int k = 0; k < maxp; k++)
+      if (slots[k].used && !slots[k].done) ord.push_back(k);
+    if (ord.empty()) return 0;
+    // largest problems first: the tail of the persistent queue is short work
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int a, int b) { return slots[a].n > slots[b].n; });
+    if (threads != 256 && threads != 512) threads = 256;
+    if (blocks <= 0) {
+      int dev = 0, cus = 0;
+      hipGetDevice(&dev);
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      blocks = std::max(1, cus) * (threads == 256 ? 2 : 1);
+    }
+    blocks = std::min<int>(blocks, (int)ord.size());
+    BCHK(hipMemcpyAsync(probs, hprobs.data(), sizeof(BatchProb) * maxp, hipMemcpyHostToDevice,
+                        st));
+    BCHK(hipMemcpyAsync(order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, st));
+    BCHK(hipMemsetAsync(next, 0, sizeof(unsigned), st));
+    BatchBufs<T> B{x, xb, g, gb, l, u, nbd, iwhere, W, d, z, r, dsub, ca, cb};
+    if (launch_batch<T>(st, threads, blocks, probs, order, (int)ord.size(), next, B, dsa, res,
+                        iter_budget) != 0)
+      return -1;
+    BCHK(hipGetLastError());
+    BCHK(hipMemcpyAsync(hres.data(), res, sizeof(BatchResult) * maxp, hipMemcpyDeviceToHost,
+                        st));
+    BCHK(hipStreamSynchronize(st));
+    int left = 0;
+    for (int k : ord) {
+      slots[k].done = hres[k].done != 0;
+      if (!slots[k].done) left++;
+    }
+    return left;
+  }
+
+  int result(int slot, double* f, int* iter, int* info, double* residual_g, int* done) override {
+    if (slot < 0 || slot >= maxp || !slots[slot].used) return -1;
+    const BatchResult& r = hres[slot];
+    if (f) *f = r.f;
+    if (iter) *iter = r.iter;
+    if (info) *info = r.info;
+    if (residual_g) *residual_g = r.residual_g;
+    if (done) *done = r.done;
+    return 0;
+  }
+  int results_all(double* f, int* iter, int* info, int* done) override {
+    for (int k = 0; k < maxp; k++) {
+      const BatchResult& r = hres[k];
+      const bool u = slots[k].used;
+      if (f) f[k] = u ? r.f : 0.0;
+      if (iter) iter[k] = u ? r.iter : 0;
+      if (info) info[k] = u ? r.info : 0;
+      if (done) done[k] = u ? r.done : 0;
+    }
+    return 0;
+  }
+  int get_x(int slot, void* out) override {
+    if (slot < 0 || slot >= maxp || !slots[slot].used) return -1;
+    BCHK(hipMemcpyAsync(out, x + slots[slot].off, (size_t)slots[slot].n * sizeof(T),
+                        hipMemcpyDeviceToDevice, st));
+    BCHK(hipStreamSynchronize(st));
+    return 0;
+  }
+  int remove(int slot) override {
+    if (slot < 0 || slot >= maxp || !slots[slot].used) return -1;
+    give_range(slots[slot].off, slots[slot].npad);
+    slots[slot] = Slot();
+    return 0;
+  }
+};
+
+}  // namespace
+
+struct lbfgsb_batch {
+  BatchBase* b;
+};
+
+extern "C" {
+
+lbfgsb_batch* lbfgsb_batch_create(int dtype, long long capacity, int max_problems, int m,
+                                  void* stream) {
+  if (capacity <= 0 || max_problems <= 0 || m <= 0) return nullptr;
+  const int mm = m < 8 ? m : 8;
+  BatchBase* b = nullptr;
+  if (dtype == 0)
+    b = new Arena<float>(capacity, max_problems, mm, (hipStream_t)stream);
+  else
+    b = new Arena<double>(capacity, max_problems, mm, (hipStream_t)stream);
+  if (!b->ok) {
+    delete b;
+    return nullptr;
+  }
+  return new lbfgsb_batch{b};
+}
+
+void lbfgsb_batch_destroy(lbfgsb_batch* b) {
+  if (!b) return;
+  delete b->b;
+  delete b;
+}
+
+int lbfgsb_batch_add(lbfgsb_batch* b, long long n, const void* x0, const int* nbd, const void* l,
+                     const void* u, int kind, long long nx, long long ny, double h, const void* bv,
+                     const void* c, const void* a0, const void* a1, const void* a2, double eps_g,
+                     double eps_f, double eps_x, int max_iteration) {
+  if (!b) return -1;
+  return b->b->add(n, x0, nbd, l, u, kind, nx, ny, h, bv, c, a0, a1, a2, eps_g, eps_f, eps_x,
+                   max_iteration);
+}
+
+int lbfgsb_batch_run(lbfgsb_batch* b, int threads, int blocks, int iter_budget) {
+  if (!b) return -1;
+  return b->b->run(threads, blocks, iter_budget);
+}
+
+int lbfgsb_batch_result(lbfgsb_batch* b, int slot, double* f, int* iter, int* info,
+                        double* residual_g, int* done) {
+  if (!b) return -1;
+  return b->b->result(slot, f, iter, info, residual_g, done);
+}
+
+int lbfgsb_batch_results(lbfgsb_batch* b, double* f, int* iter, int* info, int* done) {
+  if (!b) return -1;
+  return b->b->results_all(f, iter, info, done);
+}
+
+int lbfgsb_batch_get_x(lbfgsb_batch* b, int slot, void* out) {
+  if (!b) return -1;
+  return b->b->get_x(slot, out);
+}
+
+int lbfgsb_batch_remove(lbfgsb_batch* b, int slot) {
+  if (!b) return -1;
+  return b->b->remove(slot);
+}
+
+long long lbfgsb_batch_free_elems(lbfgsb_batch* b) { return b ? b->b->free_elems() : -1; }
+
+}  // extern "C"

@@ -57,7 +57,7 @@ const ScalarField kFields[] = {
     SF_I(gcp_floor_hits), SF_D(alpha), SF_L(ibd), SF_I(write_trial), SF_I(trial_src),
     SF_I(trial_dst), SF_I(has_after_iteration), SF_I(has_custom_stop), SF_I(gcp_pending),
     SF_I(upd_done), SF_I(p4_flagged), SF_I(gcp_rounds_run), SF_L(gcp_passed_total),
-    SF_L(cand_count)};
+    SF_L(cand_count), SF_I(gcp_need_select), SF_I(gcp_ranges)};
 
 const ScalarField* find_field(const char* name) {
   for (const auto& f : kFields)

@@ -1,0 +1,532 @@
+// Scalar control logic of the reference driver as device functions over a
+// DevScalars block (normally staged in LDS).  The single-workgroup control
+// kernels of the plain engine and the persistent per-problem kernel of the
+// batched solver both run these, so the two forms take the same decisions.
+// Thread 0 runs everything except build_middle (threads 0 .. 2col-1).
+#pragma once
+
+#include "common.hpp"
+#include "dense.hpp"
+#include "state.hpp"
+
+namespace lbfgsb {
+
+// a(i,j) of a row-major matrix with leading dimension ld, 1-based
+#define A1(a, ld, i, j) (a)[((i)-1) * (ld) + ((j)-1)]
+
+static constexpr double kMachEps = 5e-16;  // culbfgsb/ap.cpp:38, both precisions
+static constexpr double kBig = 1.0e10;     // lbfgsbcpu.cpp:1259
+static constexpr double kFtol = 1.0e-3, kGtol = 0.9, kXtol = 0.1;
+
+// BFGS memory reset used by every restart of the reference driver
+// (lbfgsbcpu.cpp:297-304, 317-324, 333-340, 380-385, 452-459).
+__device__ inline void reset_memory(DevScalars* ds) {
+  ds->internalinfo = 0;
+  ds->col = 0;
+  ds->head = 0;
+  ds->theta = 1.0;
+  ds->iupdat = 0;
+  ds->updatd = 0;
+  ds->restarts++;
+}
+
+__device__ inline void set_done(DevScalars* ds, int info) {
+  ds->info = info;
+  ds->status = ST_DONE;
+  ds->halt = 1;
+}
+
+// Cauchy-search controller (lbfgsbcpu.cpp:620-724 and the first step of the
+// breakpoint loop :725-747).  Thread 0.  Sets gcp_pending when the sorted
+// breakpoint rounds must run.
+__device__ inline void gcp_controller(DevScalars* ds) {
+  ds->gcp_pending = 0;
+  ds->p4_flagged = 0;
+  for (;;) {
+    const int col = ds->col;
+    ds->iword = -1;
+    ds->shortcut = (!ds->cnstnd && col > 0);
+    if (ds->shortcut) {
+      ds->wrk = ds->updatd;
+      ds->nint = 0;
+      ds->gcp_mode = 2;
+      return;
+    }
+    const double theta = ds->theta;
+    const int nbreak = (int)ds->pa[PA_NBREAK];
+    const int nfreegcp = (int)ds->pa[PA_NFREEGCP];
+    ds->gcp_nbreak = nbreak;
+    ds->gcp_nfreegcp = nfreegcp;
+    ds->gcp_unbnd = ds->pa[PA_UNBND] > 0;
+    const double f1 = -ds->pa[PA_F1];
+    ds->gcp_D0 = ds->pa[PA_F1];
+    for (int j = 0; j < col; j++) {
+      ds->p[j] = ds->pa[PA_P + j];
+      ds->p[col + j] = theta != 1.0 ? ds->pa[PA_P + 8 + j] * theta : ds->pa[PA_P + 8 + j];
+    }
+    if (nbreak == 0 && nfreegcp == 0) {
+      // nothing moves: x_cp = x, c and nint keep their previous values
+      ds->gcp_mode = 2;
+      if (col > 0 && d_bmv(ds->sy, ds->wt, col, ds->c, ds->mc) != 0) {
+        reset_memory(ds);
+        continue;
+      }
+      ds->nintol += ds->nint;
+      return;
+    }
+    for (int j = 0; j < 2 * col; j++) ds->c[j] = 0.0;
+    double f2 = -theta * f1;
+    const double f2org = f2;
+    if (col > 0) {
+      if (d_bmv(ds->sy, ds->wt, col, ds->p, ds->v) != 0) {
+        reset_memory(ds);
+        continue;
+      }
+      f2 = f2 - ddot(ds->v, 1, ds->p, 1, 2 * col);
+    }
+    double dtm = -f1 / f2;
+    ds->gcp_f1 = f1;
+    ds->gcp_f2 = f2;
+    ds->gcp_f2org = f2org;
+    ds->nint = 1;
+    ds->gcp_kpassed = 0;
+    ds->gcp_t_prev = 0.0;
+    ds->gcp_idx_prev = -1;
+    if (nbreak == 0 || dtm < ds->gcp_bkmin) {
+      if (dtm <= 0) dtm = 0;
+      ds->gcp_dtm = dtm;
+      ds->gcp_tsum = dtm;
+      for (int j = 0; j < 2 * col; j++) ds->c[j] += dtm * ds->p[j];
+      ds->gcp_mode = 0;
+      if (col > 0 && d_bmv(ds->sy, ds->wt, col, ds->c, ds->mc) != 0) {
+        reset_memory(ds);
+        continue;
+      }
+      ds->nintol += ds->nint;
+      return;
+    }
+    // at least the smallest breakpoint is passed: sorted rounds, the first
+    // over t in [0, 2 dtm) (dtm >= bkmin > 0 here)
+    ds->gcp_pending = 1;
+    ds->gcp_need_select = 1;
+    ds->gcp_rounds_run++;
+    ds->gcp_round = 0;
+    ds->gcp_floor_hits = 0;
+    for (int q = 0; q < 40; q++) ds->gcp_carry[q] = 0.0;
+    ds->gcp_tau_lo = -__builtin_inf();
+    const double t0 = 2.0 * dtm;
+    ds->gcp_tau_hi = t0 < 1e300 ? t0 : __builtin_inf();
+    ds->cand_count = 0;
+    return;
+  }
+}
+
+// Columns M e_k of the middle matrix for the breakpoint rounds; threads
+// 0 .. 2col-1 (sy, wt in LDS).
+__device__ inline void build_middle(DevScalars* ds) {
+  const int col = ds->col;
+  // per-thread unit vector and result rows in LDS (not scratch memory)
+  __shared__ double s_e[16][17], s_out[16][17];
+  if (!ds->gcp_pending) return;
+  // Mx is stored in the padded layout of the breakpoint sums: index a of
+  // the packed 2col vector [y-part ; s-part] goes to a (a < col) or
+  // 8 + a - col; the other entries are zero
+  for (int q = threadIdx.x; q < 256; q += blockDim.x) ds->Mx[q] = 0.0;
+  __syncthreads();
+  if ((int)threadIdx.x < 2 * col) {
+    double* e = s_e[threadIdx.x];
+    double* out = s_out[threadIdx.x];
+    for (int i = 0; i < 16; i++) e[i] = 0.0;
+    e[threadIdx.x] = 1.0;
+    d_bmv(ds->sy, ds->wt, col, e, out);
+    const int k = (int)threadIdx.x;
+    const int kp = k < col ? k : 8 + k - col;
+    for (int r = 0; r < 2 * col; r++) ds->Mx[(r < col ? r : 8 + r - col) * 16 + kp] = out[r];
+  }
+}
+
+// matupd's small part + formt (lbfgsbcpu.cpp:1365-1386, 1105-1133).
+__device__ inline void memory_update(DevScalars* ds) {
+  if (!ds->updatd) return;
+  const int m = ds->m, col = ds->col;
+  ds->rr = ds->pa[PA_RR];
+  ds->theta = ds->rr / ds->dr;
+  if (ds->iupdat > m) {
+    for (int j = 1; j <= col - 1; j++) {
+      for (int i = 1; i <= j; i++) A1(ds->ss, 8, i, j) = A1(ds->ss, 8, i + 1, j + 1);
+      for (int i = j; i <= col - 1; i++) A1(ds->sy, 8, i, j) = A1(ds->sy, 8, i + 1, j + 1);
+    }
+  }
+  for (int j = 1; j <= col - 1; j++) {
+    A1(ds->sy, 8, col, j) = ds->pa[PA_SY + j - 1];
+    A1(ds->ss, 8, j, col) = ds->pa[PA_SS + j - 1];
+  }
+  A1(ds->ss, 8, col, col) = ds->stp == 1.0 ? ds->dtd : ds->stp * ds->stp * ds->dtd;
+  A1(ds->sy, 8, col, col) = ds->dr;
+  if (!d_formt(ds->wt, ds->sy, ds->ss, col, ds->theta)) reset_memory(ds);
+}
+
+// formk (lbfgsbcpu.cpp:907-1103) from the inner products of k_p4.
+__device__ inline bool formk_device(DevScalars* ds, const double* nc, const double* corr) {
+  const int m = ds->m, col = ds->col;
+  double* wn1 = ds->wn1;
+  double* wn = ds->wn;
+  const double theta = ds->theta;
+  int upcl;
+  if (ds->updatd) {
+    if (ds->iupdat > m) {
+      for (int jy = 1; jy <= m - 1; jy++) {
+        const int js = m + jy;
+        for (int i = 0; i < m - jy; i++) A1(wn1, 16, jy + i, jy) = A1(wn1, 16, jy + 1 + i, jy + 1);
+        for (int i = 0; i < m - jy; i++) A1(wn1, 16, js + i, js) = A1(wn1, 16, js + 1 + i, js + 1);
+        for (int i = 0; i < m - 1; i++) A1(wn1, 16, m + 1 + i, jy) = A1(wn1, 16, m + 2 + i, jy + 1);
+      }
+    }
+    for (int jy = 1; jy <= col; jy++) {
+      A1(wn1, 16, col, jy) = nc[jy - 1];
+      A1(wn1, 16, m + col, m + jy) = nc[8 + jy - 1];
+      A1(wn1, 16, m + col, jy) = nc[16 + jy - 1];
+    }
+    for (int i = 1; i <= col; i++) A1(wn1, 16, m + i, col) = nc[24 + i - 1];
+    upcl = col - 1;
+  } else {
+    upcl = col;
+  }
+  if (corr) {
+    for (int iy = 1; iy <= upcl; iy++) {
+      for (int jy = 1; jy <= iy; jy++) {
+        const int q = (iy - 1) * iy / 2 + (jy - 1);
+        A1(wn1, 16, iy, jy) = A1(wn1, 16, iy, jy) + corr[q];
+        A1(wn1, 16, m + iy, m + jy) = A1(wn1, 16, m + iy, m + jy) - corr[36 + q];
+      }
+    }
+    for (int i = 1; i <= upcl; i++) {
+      for (int jy = 1; jy <= upcl; jy++) {
+        const double v = corr[72 + (i - 1) * 8 + (jy - 1)];
+        if (i <= jy)
+          A1(wn1, 16, m + i, jy) = A1(wn1, 16, m + i, jy) + v;
+        else
+          A1(wn1, 16, m + i, jy) = A1(wn1, 16, m + i, jy) - v;
+      }
+    }
+  }
+  for (int iy = 1; iy <= col; iy++) {
+    const int iis = col + iy, is1 = m + iy;
+    for (int jy = 1; jy <= iy; jy++) {
+      const int js = col + jy, js1 = m + jy;
+      A1(wn, 16, jy, iy) = A1(wn1, 16, iy, jy) / theta;
+      A1(wn, 16, js, iis) = A1(wn1, 16, is1, js1) * theta;
+    }
+    for (int jy = 1; jy <= iy - 1; jy++) A1(wn, 16, jy, iis) = -A1(wn1, 16, is1, jy);
+    for (int jy = iy; jy <= col; jy++) A1(wn, 16, jy, iis) = A1(wn1, 16, is1, jy);
+    A1(wn, 16, iy, iy) = A1(wn, 16, iy, iy) + A1(ds->sy, 8, iy, iy);
+  }
+  if (!d_dpofa(wn, 16, col)) {
+    ds->internalinfo = -1;
+    return false;
+  }
+  const int col2 = 2 * col;
+  double* work = ds->tmp;  // 8 doubles
+  for (int js = col + 1; js <= col2; js++) {
+    for (int i = 1; i <= col; i++) work[i - 1] = A1(wn, 16, i, js);
+    d_dtrsl(wn, 16, col, work, 11);
+    for (int i = 1; i <= col; i++) A1(wn, 16, i, js) = work[i - 1];
+  }
+  for (int iis = col + 1; iis <= col2; iis++)
+    for (int js = iis; js <= col2; js++)
+      A1(wn, 16, iis, js) = A1(wn, 16, iis, js) +
+                            ddot(&A1(wn, 16, 1, iis), 16, &A1(wn, 16, 1, js), 16, col);
+  double* wm = ds->tmp + 16;  // 8 x 8
+  for (int i = 1; i <= col; i++)
+    for (int k = 1; k <= col; k++) A1(wm, 8, i, k) = A1(wn, 16, col + i, col + k);
+  if (!d_dpofa(wm, 8, col)) {
+    ds->internalinfo = -2;
+    return false;
+  }
+  for (int i = 1; i <= col; i++)
+    for (int k = 1; k <= col; k++) A1(wn, 16, col + i, col + k) = A1(wm, 8, i, k);
+  return true;
+}
+
+__device__ inline void request_trial(DevScalars* ds) {
+  ds->task = 1;
+  ds->ifun = ds->ifun + 1;
+  ds->nfgv = ds->nfgv + 1;
+  ds->iback = ds->ifun - 1;
+  ds->write_trial = 1;
+  ds->trial_src = ds->phase & 1;
+  ds->trial_dst = (ds->phase & 1) ^ 1;
+}
+
+// Step accepted: the reference increments iter and, after its stopping
+// tests, decides the update (lbfgsbcpu.cpp:388-449).  The decision only
+// depends on line-search scalars, so it is taken here and the ring slot is
+// chosen before the fused tail pass writes it.
+__device__ inline void accept_step(DevScalars* ds) {
+  ds->phase ^= 1;
+  ds->iter = ds->iter + 1;
+  ds->bp_count = 0;
+  const double stp = ds->stp;
+  if (stp == 1.0) {
+    ds->dr = ds->gd - ds->gdold;
+    ds->ddum = -ds->gdold;
+  } else {
+    ds->dr = (ds->gd - ds->gdold) * stp;
+    ds->ddum = -ds->gdold * stp;
+  }
+  if (ds->dr <= kMachEps * ds->ddum) {
+    ds->nskip = ds->nskip + 1;
+    ds->updatd = 0;
+  } else {
+    const int m = ds->m;
+    ds->updatd = 1;
+    ds->iupdat = ds->iupdat + 1;
+    if (ds->iupdat <= m) {
+      ds->col = ds->iupdat;
+      ds->itail = (ds->head + ds->iupdat - 1) % m;
+    } else {
+      ds->itail = (ds->itail + 1) % m;
+      ds->head = (ds->head + 1) % m;
+    }
+  }
+  if (ds->has_after_iteration) {
+    ds->status = ST_ACCEPTED;
+    ds->halt = 1;
+  } else {
+    ds->status = ST_RUNNING;
+  }
+}
+
+// Line-search failure (lbfgsbcpu.cpp:359-387): the iterate and gradient are
+// still the accepted ones (buffers were never swapped), f is restored.
+__device__ inline void ls_failure(DevScalars* ds) {
+  ds->f = ds->fold;
+  ds->write_trial = 0;
+  if (ds->col == 0) {
+    ds->task = 2;
+    ds->iter = ds->iter + 1;
+    set_done(ds, -2);
+    return;
+  }
+  reset_memory(ds);
+  ds->status = ST_RESTART;
+  ds->halt = 1;
+}
+
+
+// ---------------------------------------------------------------------------
+// Per-step logic, thread 0, on reduced values.  Each control kernel and the
+// batched kernel call these after finishing their reductions.
+
+// errclb verdict (lbfgsbcpu.cpp:871-905): counts of bad nbd and of l > u.
+__device__ inline void logic_setup_errclb(DevScalars* ds, double bad_nbd, double bad_box) {
+  if (bad_nbd > 0 || bad_box > 0) set_done(ds, -1);
+}
+
+// active's flags (lbfgsbcpu.cpp:467-515): projected, constrained, not-boxed
+// counts.
+__device__ inline void logic_setup_active(DevScalars* ds, const double (&r)[3]) {
+  ds->prjctd = r[0] > 0;
+  ds->cnstnd = r[1] > 0;
+  ds->boxed = !(r[2] > 0);
+}
+
+// First projected-gradient test after the initial evaluation
+// (lbfgsbcpu.cpp:271-278).
+__device__ inline void logic_setup_projgr(DevScalars* ds, double sbgnrm, double f) {
+  ds->f = f;
+  ds->nfgv = 1;
+  ds->sbgnrm = sbgnrm;
+  ds->residual_g = sbgnrm;
+  if (sbgnrm <= ds->eps_g) set_done(ds, 4);
+}
+
+__device__ inline void logic_resume(DevScalars* ds, int reset_bp) {
+  ds->halt = 0;
+  ds->status = ST_RUNNING;
+  ds->write_trial = 0;
+  if (reset_bp) ds->bp_count = 0;
+}
+
+// Stopping tests after an accepted step (lbfgsbcpu.cpp:394-429) on the
+// pass-A sums `pa`, the projected-gradient norm and the smallest breakpoint;
+// without a customised stopping test it continues with the memory update
+// and the next Cauchy controller.
+__device__ inline void logic_stop_tests(DevScalars* s, const double* pa, double sbg, double bk,
+                                        long long ibk) {
+  s->write_trial = 0;
+  s->upd_done = 0;
+  for (int v = 0; v < PA_COUNT; v++) s->pa[v] = pa[v];
+  s->gcp_bkmin = bk;
+  s->gcp_ibkmin = ibk;
+  s->sbgnrm = sbg;
+  s->residual_g = sbg;
+  const double tf = sqrt(pa[PA_TF]);
+  const double ddum = dmax_(fabs(s->fold), dmax_(fabs(s->f), 1.0));
+  if (sbg <= s->eps_g) {
+    set_done(s, 4);
+    return;
+  }
+  s->tf = tf;
+  s->residual_x = tf;
+  if (tf <= s->eps_x) {
+    set_done(s, 2);
+    return;
+  }
+  s->ddum = ddum;
+  s->residual_f = (s->fold - s->f) / ddum;
+  if (s->fold - s->f <= s->eps_f * ddum) {
+    set_done(s, 1);
+  } else if (s->iter > s->max_iteration && s->max_iteration > 0) {
+    set_done(s, 5);
+  } else if (s->has_custom_stop) {
+    s->status = ST_CHECKED;
+    s->halt = 1;
+  } else {
+    memory_update(s);
+    gcp_controller(s);
+    s->upd_done = 1;
+  }
+}
+
+// Memory update + Cauchy controller.  reduce: iteration start / restart,
+// where the classification sums `pa` and the smallest breakpoint come from
+// a standalone pass A (no update then).
+__device__ inline void logic_update_gcp(DevScalars* s, int reduce, const double* pa, double bk,
+                                        long long ibk) {
+  s->write_trial = 0;
+  if (reduce) {
+    for (int v = 0; v < PA_COUNT; v++) s->pa[v] = pa[v];
+    s->gcp_bkmin = bk;
+    s->gcp_ibkmin = ibk;
+  } else {
+    memory_update(s);
+  }
+  gcp_controller(s);
+  s->upd_done = 1;
+}
+
+// After the Cauchy-point pass: free/active bookkeeping
+// (lbfgsbcpu.cpp:1143-1161), formk, and the two triangular solves of subsm
+// (lbfgsbcpu.cpp:1459-1471).  r = the pass's sums (P4_* layout).
+__device__ inline void logic_formk_subsm(DevScalars* s, const double* r, bool flagged) {
+  const long long n = s->n;
+  const int col = s->col;
+  if (!s->shortcut) {
+    s->nfree = (int)r[P4_NFREE];
+    if (s->iter > 0 && s->cnstnd) {
+      s->nenter = (int)r[P4_NENTER];
+      s->nleave = (int)r[P4_NLEAVE];
+    } else {
+      s->nenter = 0;
+      s->nleave = 0;
+    }
+    s->ileave = (int)(n + 1 - s->nleave);
+    s->wrk = s->nleave > 0 || s->nenter > 0 || s->updatd;
+  }
+  s->skip_subsm = (s->nfree == 0 || col == 0);
+  if (s->skip_subsm) return;
+  bool fail = false;
+  if (s->wrk) fail = !formk_device(s, r + P4_NC, flagged ? r + P4_CORR : nullptr);
+  if (!fail) {
+    const double theta = s->theta;
+    double* wv = s->wv;
+    for (int i = 0; i < col; i++) {
+      wv[i] = r[P4_WV + i];
+      wv[col + i] = theta * r[P4_WV + 8 + i];
+    }
+    int info = d_dtrsl(s->wn, 16, 2 * col, wv, 11);
+    if (info == 0) {
+      for (int i = 0; i < col; i++) wv[i] = -wv[i];
+      info = d_dtrsl(s->wn, 16, 2 * col, wv, 1);
+    }
+    fail = info != 0;
+  }
+  if (fail) {
+    reset_memory(s);
+    s->status = ST_RESTART;
+    s->halt = 1;
+  }
+}
+
+// Backtrack length of subsm (lbfgsbcpu.cpp:1484-1538): the smallest
+// step-to-box ratio a and its variable.
+__device__ inline void logic_alpha(DevScalars* ds, double a, long long i) {
+  if (a < 1.0) {
+    ds->alpha = a;
+    ds->ibd = i;
+    ds->iword = 1;
+  } else {
+    ds->alpha = 1.0;
+    ds->ibd = -1;
+    ds->iword = 0;
+  }
+}
+
+// First call of lnsrlb for an iteration (lbfgsbcpu.cpp:1263-1339) on d'd,
+// g'd and the smallest step to the box.
+__device__ inline void logic_ls_init(DevScalars* s, double dtd, double gd, double smin) {
+  s->dtd = dtd;
+  s->dnrm = sqrt(s->dtd);
+  s->stpmx = kBig;
+  if (s->cnstnd) {
+    if (s->iter == 0)
+      s->stpmx = 1.0;
+    else
+      s->stpmx = dmin_(kBig, smin);
+  }
+  if (s->iter == 0 && !s->boxed)
+    s->stp = dmin_(1.0 / s->dnrm, s->stpmx);
+  else
+    s->stp = 1.0;
+  s->fold = s->f;
+  s->ifun = 0;
+  s->iback = 0;
+  s->csave = 0;
+  s->gd = gd;
+  s->gdold = s->gd;
+  if (s->gd >= 0) {
+    s->internalinfo = -4;
+    ls_failure(s);
+    return;
+  }
+  d_dcsrch(s->f, s->gd, s->stp, kFtol, kGtol, kXtol, 0.0, s->stpmx, s->csave, s->isave,
+           s->dsave);
+  s->xstep = s->stp * s->dnrm;
+  if (s->csave != 4 && s->csave != 3) {
+    request_trial(s);
+    s->status = ST_NEED_EVAL;
+    s->halt = 1;
+  } else {
+    s->task = 5;
+    accept_step(s);
+  }
+}
+
+// Later calls of lnsrlb, after an evaluation with value f and g'd
+// (lbfgsbcpu.cpp:1312-1339 and the loop exit at :352).
+__device__ inline void logic_ls_step(DevScalars* s, double f, double gd) {
+  s->halt = 0;
+  s->status = ST_RUNNING;
+  s->f = f;
+  s->gd = gd;
+  s->write_trial = 0;
+  d_dcsrch(s->f, s->gd, s->stp, kFtol, kGtol, kXtol, 0.0, s->stpmx, s->csave, s->isave,
+           s->dsave);
+  s->xstep = s->stp * s->dnrm;
+  if (s->csave != 4 && s->csave != 3) {
+    request_trial(s);
+    if (s->iback >= 20) {
+      // the reference leaves the loop before evaluating this trial point
+      accept_step(s);
+    } else {
+      s->status = ST_NEED_EVAL;
+      s->halt = 1;
+    }
+  } else {
+    s->task = 5;
+    accept_step(s);
+  }
+}
+
+}  // namespace lbfgsb

@@ -321,25 +321,28 @@ struct Engine final : EngineBase {
 
   // Host-sized breakpoint rounds of the Cauchy search (gcp.hip): used when a
   // round holds more candidates than one workgroup sorts, and for the rounds
-  // after the first.  The candidates of the current round are already
-  // selected when this is entered.
+  // after the first.  Entered either with the current t-range selected (the
+  // one-workgroup round found too many candidates) or with a new t-range
+  // still to select (gcp_need_select, set when a round passed all of its
+  // candidates).
   int run_rounds() {
     GcpScratch w{tsum, tpre, tstride, total, tile_stop, tile_state, last_state};
-    for (int guard = 0; guard < 64; guard++) {
+    for (int guard = 0; guard < 128; guard++) {
       if (read_scalars()) return -1;
       if (!hs->gcp_pending) return 0;
-      const long long K = (long long)hs->cand_count;
       // the previous kernel halted the chain: clear it for the round kernels
       launch_resume(st, ds, 0);
+      if (hs->gcp_need_select) {
+        launch_gcp_select<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, ca, n);
+        HIPCHK(hipGetLastError());
+        continue;  // size the round with the count just selected
+      }
+      const long long K = (long long)hs->cand_count;
       launch_gcp_sortscan<T>(st, dG, W, ds, ca, cb, K, w);
       HIPCHK(hipGetLastError());
       if (read_scalars()) return -1;
       if (!hs->gcp_pending) return 0;
       if (hs->status != ST_GCP_ROUNDS) return -1;
-      // next round: select the next t-range (sorted at its size above)
-      launch_resume(st, ds, 0);
-      launch_gcp_select<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, ca, n);
-      HIPCHK(hipGetLastError());
     }
     return -1;
   }

@@ -15,38 +15,6 @@
 
 namespace lbfgsb {
 
-// Kernel steps of the chain.
-enum Step : int {
-  K_RESUME_RESET = 0,
-  K_RESUME,
-  K_PASSA_SA,   // standalone Cauchy classification pass
-  K_UPD_REDUCE, // controller after the standalone pass
-  K_PASSA_FUSED,
-  K_STOP,
-  K_UPD,
-  K_P4,
-  K_FORMK,
-  K_P5A,
-  K_ALPHA,
-  K_P5B,
-  K_LSINIT,
-  K_TRIAL,
-  K_GD,
-  K_LSSTEP,
-  K_END,        // also the timing slot of host-sized breakpoint rounds
-  K_GCP = 17    // device-sized breakpoint round (select + one-workgroup round)
-};
-constexpr int kNumSteps = 18;
-
-// Programs (sequences of steps).
-enum Program : int {
-  PROG_ITER_START = 0,  // iteration 0 and after every memory reset
-  PROG_AFTER_EVAL = 1,  // a function value for the pending trial is known
-  PROG_AFTER_ACCEPT = 2,
-  PROG_AFTER_CHECK = 3,
-  PROG_TAIL = 4
-};
-
 struct EngineBase {
   virtual ~EngineBase() {}
   virtual int dtype() const = 0;  // 0 = fp32 bulk, 1 = fp64 bulk

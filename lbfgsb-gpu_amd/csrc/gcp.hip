@@ -21,36 +21,9 @@
 // size read on the device when one workgroup can sort it (2048 candidates);
 // larger or later rounds are
 // sized by the host.
-#include "common.hpp"
-#include "dense.hpp"
-#include "launch.hpp"
-#include "state.hpp"
+#include "gcp_device.hpp"
 
 namespace lbfgsb {
-
-static constexpr int kSortTile = 2048;
-static constexpr int kNV = 33;
-static constexpr double kMachEpsG = 5e-16;
-
-__device__ inline bool cand_less(const Cand& a, const Cand& b) {
-  return a.t < b.t || (a.t == b.t && a.idx < b.idx);
-}
-
-// Same classification as the bulk passes (lbfgsbcpu.cpp:634-700).
-template <typename T>
-__device__ inline bool breakpoint_of(T x, T g, T l, T u, int nb, int iw, T& t) {
-  const T neggi = -g;
-  if (iw != 0 && iw != -1) return false;
-  if (nb <= 2 && nb != 0 && neggi < 0) {
-    t = (x - l) / (-neggi);
-    return true;
-  }
-  if (nb >= 2 && neggi > 0) {
-    t = (u - x) / neggi;
-    return true;
-  }
-  return false;
-}
 
 // Candidates t in [tau_lo, tau_hi) appended to cand (count in ds->cand_count;
 // entries beyond `cap` are not written).
@@ -65,6 +38,8 @@ k_gcp_select(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
   const T* __restrict__ g = Gv[cur];
   const double lo = ds->gcp_tau_lo, hi = ds->gcp_tau_hi;
   const int lane = threadIdx.x & 63;
+  // no block reads the flag; the host reads it after the launch
+  if (blockIdx.x == 0 && threadIdx.x == 0) ds->gcp_need_select = 0;
   for (long long i0 = blockIdx.x * (long long)kBlock; i0 < n;
        i0 += (long long)gridDim.x * kBlock) {
     const long long i = i0 + threadIdx.x;
@@ -169,49 +144,7 @@ k_merge(const Cand* __restrict__ src, Cand* __restrict__ dst, const DevScalars* 
   }
 }
 
-// Contribution of candidate c: [d w (2col) ; t d w (2col) ; d^2].
-template <typename T>
-__device__ inline void contribution(const Cand& c, const T* __restrict__ g,
-                                    const T* __restrict__ W, const RingReads& rr,
-                                    double theta, double (&v)[kNV]) {
-  const long long i = c.idx;
-  // the whole W row and g_i are loaded before any use (positions j >= col
-  // read as zero)
-  T yv[kMaxM], sv[kMaxM];
-  const T gi = g[i];
-  load_ring(W, i, rr, yv, sv);
-  const double d = -(double)gi;
-#pragma unroll
-  for (int j = 0; j < kMaxM; j++) {
-    const double wy = (double)yv[j];
-    const double ws = theta * (double)sv[j];
-    v[j] = d * wy;
-    v[8 + j] = d * ws;
-    v[16 + j] = c.t * d * wy;
-    v[24 + j] = c.t * d * ws;
-  }
-  v[32] = d * d;
-}
-
-// p at the start of the search and the middle matrix M in the padded
-// layout of the breakpoint sums (positions 0..7 Y, 8..15 S, zero beyond
-// col), staged in LDS by every thread of the block.
-struct GcpConst {
-  double pp[16];
-  double Mp[256];
-};
-__device__ inline void stage_gcp_const(const DevScalars* ds, GcpConst& gc) {
-  const int col = ds->col;
-  for (int q = threadIdx.x; q < 256; q += blockDim.x) gc.Mp[q] = ds->Mx[q];
-  if (threadIdx.x < 16) {
-    const int q = threadIdx.x;
-    const int j = q & 7;
-    gc.pp[q] = j < col ? ds->p[(q < 8 ? 0 : col) + j] : 0.0;
-  }
-  __syncthreads();
-}
-
-// Layout of a state vector in this file: [P_y(8) P_s(8) Q_y(8) Q_s(8) D].
+// Per-tile sums of the candidates' contributions.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_gcp_tile_reduce(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__ W,
@@ -269,45 +202,6 @@ __global__ void s_gcp_tile_scan(DevScalars* ds, long long cap, const double* tsu
     if (lane == 0) total[q] = run;
   }
   (void)sh;
-}
-
-// Stop test at the state `st` (passed breakpoints before t_next).
-// All loops run over the padded 16 positions with compile-time indices (the
-// entries beyond col are exact zeros, which leaves every sum unchanged), so
-// the temporaries stay in registers.
-__device__ inline bool stop_test(const DevScalars* ds, const GcpConst& gc,
-                                 const double (&st)[kNV], double t_prev, double t_next,
-                                 bool floor_on, double (&out)[3]) {
-  const double theta = ds->theta;
-  double p[16], mp[16];
-#pragma unroll
-  for (int q = 0; q < 16; q++) p[q] = gc.pp[q] - st[q];
-#pragma unroll
-  for (int r = 0; r < 16; r++) {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) s += gc.Mp[r * 16 + k] * p[k];
-    mp[r] = s;
-  }
-  double pmp = 0.0, qmp = 0.0;
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    pmp += p[j] * mp[j] + p[8 + j] * mp[8 + j];
-    qmp += st[16 + j] * mp[j] + st[24 + j] * mp[8 + j];
-  }
-  const double D = ds->gcp_D0 - st[32];
-  const double f2u = theta * D - pmp;
-  const double f1 = -D + t_prev * f2u - qmp;
-  double f2 = f2u;
-  if (floor_on) {
-    const double fl = kMachEpsG * ds->gcp_f2org;
-    if (fl > f2) f2 = fl;
-  }
-  const double dtm = -f1 / f2;
-  out[0] = dtm;
-  out[1] = f1;
-  out[2] = f2;
-  return dtm < t_next - t_prev;
 }
 
 template <typename T>
@@ -372,98 +266,6 @@ k_gcp_tile_eval(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__
   if (threadIdx.x == 0) tile_stop[blockIdx.x] = sstop;
 }
 
-__device__ void gcp_done(DevScalars* ds) {
-  const int col = ds->col;
-  if (col > 0 && d_bmv(ds->sy, ds->wt, col, ds->c, ds->mc) != 0) ds->internalinfo = -8;
-  ds->nintol += ds->nint;
-  ds->gcp_pending = 0;
-  ds->status = ST_RUNNING;
-  ds->halt = 0;
-}
-
-// Conclusion of a round (thread 0).  stop_pos = first candidate at which the
-// search stops (its exclusive state in `st`), or -1 when every candidate
-// passed (then `st` is the inclusive total).  last_state = exclusive state
-// before the last candidate.
-__device__ void round_conclude(DevScalars* ds, const GcpConst& gc, const Cand* cand, long long K,
-                               long long stop_pos, const double* stv, const double* last_state) {
-  const int col = ds->col;
-  // the state stays where the caller keeps it (LDS or global), read in place
-  const double(&st)[kNV] = *reinterpret_cast<const double(*)[kNV]>(stv);
-  double tk, dtm = 0.0;
-  long long ik, kpass;
-  if (stop_pos >= 0) {
-    const long long pos = stop_pos;
-    tk = pos > 0 ? cand[pos - 1].t : ds->gcp_t_prev;
-    ik = pos > 0 ? cand[pos - 1].idx : ds->gcp_idx_prev;
-    kpass = ds->gcp_kpassed + pos;
-    double out[3];
-    stop_test(ds, gc, st, tk, cand[pos].t, kpass > 0, out);
-    dtm = out[0];
-  } else {
-    for (int q = 0; q < kNV; q++) ds->gcp_carry[q] = st[q];
-    ds->gcp_kpassed += K;
-    if (K > 0) {
-      ds->gcp_t_prev = cand[K - 1].t;
-      ds->gcp_idx_prev = cand[K - 1].idx;
-    }
-    if (ds->gcp_kpassed < ds->gcp_nbreak) {
-      // next round: a 4x larger t-range; sized by the host
-      ds->gcp_round++;
-      ds->gcp_tau_lo = ds->gcp_tau_hi;
-      const double nt = ds->gcp_tau_hi * 4.0;
-      ds->gcp_tau_hi = (ds->gcp_round >= 24 || !(nt < 1e300)) ? __builtin_inf() : nt;
-      ds->cand_count = 0;
-      ds->status = ST_GCP_ROUNDS;
-      ds->halt = 1;
-      return;
-    }
-    kpass = ds->gcp_kpassed;
-    tk = ds->gcp_t_prev;
-    ik = ds->gcp_idx_prev;
-    if (ds->gcp_nbreak == ds->n) {
-      // every variable was a breakpoint and all were passed: the reference
-      // returns before updating p with the last one (lbfgsbcpu.cpp:763-769);
-      // c = t_K p_{K-1} + Q_{K-1}
-      for (int j = 0; j < col; j++) {
-        const double py = ds->p[j] - last_state[j];
-        const double ps = ds->p[col + j] - last_state[8 + j];
-        ds->c[j] = tk * py + last_state[16 + j];
-        ds->c[col + j] = tk * ps + last_state[24 + j];
-        ds->p[j] = py;
-        ds->p[col + j] = ps;
-      }
-      ds->nint = (int)ds->gcp_nbreak;
-      ds->gcp_mode = 3;
-      ds->gcp_tsum = tk;
-      ds->gcp_last_t = tk;
-      ds->gcp_last_idx = ik;
-      gcp_done(ds);
-      return;
-    }
-    double out[3];
-    stop_test(ds, gc, st, tk, tk, true, out);
-    dtm = ds->gcp_unbnd ? out[0] : 0.0;
-  }
-  if (dtm <= 0) dtm = 0;
-  for (int j = 0; j < col; j++) {
-    const double py = ds->p[j] - st[j];
-    const double ps = ds->p[col + j] - st[8 + j];
-    ds->p[j] = py;
-    ds->p[col + j] = ps;
-    ds->c[j] = (tk + dtm) * py + st[16 + j];
-    ds->c[col + j] = (tk + dtm) * ps + st[24 + j];
-  }
-  ds->gcp_dtm = dtm;
-  ds->gcp_tsum = tk + dtm;
-  ds->gcp_last_t = tk;
-  ds->gcp_last_idx = ik;
-  ds->gcp_mode = kpass > 0 ? 1 : 0;
-  ds->nint = (int)(kpass + 1);
-  ds->gcp_passed_total += kpass;
-  gcp_done(ds);
-}
-
 // End of a multi-block round: either the first stop is known (finish the
 // Cauchy point) or every candidate passed (carry the sums; next round or
 // terminal case).  `cap` is the candidate capacity the launches were sized
@@ -498,20 +300,13 @@ __global__ void s_gcp_round_end(DevScalars* ds, const Cand* __restrict__ cand, l
 
 // A whole round in one workgroup when it has at most kSmallRound candidates
 // (the common case once the active set settles): bitonic sort in LDS, then
-// 256-candidate chunks scanned in order, stopping at the first chunk that
-// holds the stop.  Larger rounds are left to the multi-block path.
-static constexpr int kSmallRound = 2048;
+// chunks scanned in order (gcp_device.hpp).  Larger rounds are left to the
+// multi-block path.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__ W,
                   DevScalars* ds) {
-  __shared__ Cand sc[kSmallRound];
-  __shared__ double swave[kWavesPerBlock * kNV];
-  __shared__ double scarry[kNV];
-  __shared__ double sstop_state[kNV];
-  __shared__ double slast[kNV];
-  __shared__ long long sstop;
-  __shared__ GcpConst gc;
+  __shared__ RoundLds L;
   if (!ds->gcp_pending || ds->halt) return;
   const long long K = (long long)ds->cand_count;
   if (K > kSmallRound) {
@@ -521,92 +316,9 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
     }
     return;
   }
-  stage_gcp_const(ds, gc);
-  int P = 64;
-  while (P < K) P <<= 1;
-  for (int j = threadIdx.x; j < P; j += blockDim.x) {
-    if (j < K) {
-      sc[j] = cand[j];
-    } else {
-      sc[j].t = __builtin_inf();
-      sc[j].idx = 0x7fffffff;
-      sc[j].pad = 0;
-    }
-  }
-  if (threadIdx.x < kNV) scarry[threadIdx.x] = ds->gcp_carry[threadIdx.x];
-  if (threadIdx.x == 0) sstop = 0x7fffffffffffffffLL;
-  __syncthreads();
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int p = threadIdx.x; p < P / 2; p += blockDim.x) {
-        const int i = (p / j) * 2 * j + (p % j), l = i + j;
-        const bool up = (i & k) == 0;
-        if (cand_less(sc[l], sc[i]) == up) {
-          Cand tmp = sc[i];
-          sc[i] = sc[l];
-          sc[l] = tmp;
-        }
-      }
-      __syncthreads();
-    }
-  }
-  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1);
-  const double theta = ds->theta;
-  const long long kpassed = ds->gcp_kpassed;
-  const double t_prev0 = ds->gcp_t_prev;
-  const T* g = Gv[ds->phase & 1];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (long long c0 = 0; c0 < K; c0 += kBlock) {
-    const long long pos = c0 + threadIdx.x;
-    double v[kNV], ex[kNV];
-#pragma unroll
-    for (int q = 0; q < kNV; q++) v[q] = 0.0;
-    if (pos < K) contribution(sc[pos], g, W, rr, theta, v);
-#pragma unroll
-    for (int q = 0; q < kNV; q++) {
-      double incl = v[q];
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const double up = __shfl_up(incl, o, kWave);
-        if (lane >= o) incl += up;
-      }
-      ex[q] = incl - v[q];
-      if (lane == 63) swave[wid * kNV + q] = incl;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kNV; q++) {
-      double off = scarry[q];
-      for (int w = 0; w < wid; w++) off += swave[w * kNV + q];
-      ex[q] += off;
-    }
-    bool stop = false;
-    if (pos < K && kpassed + pos > 0) {
-      const double tp = pos > 0 ? sc[pos - 1].t : t_prev0;
-      double out[3];
-      stop = stop_test(ds, gc, ex, tp, sc[pos].t, true, out);
-    }
-    if (stop) atomicMin(&sstop, pos);
-    if (pos == K - 1)
-      for (int q = 0; q < kNV; q++) slast[q] = ex[q];
-    __syncthreads();
-    if (sstop != 0x7fffffffffffffffLL) {
-      if (pos == sstop)
-        for (int q = 0; q < kNV; q++) sstop_state[q] = ex[q];
-      break;
-    }
-    // carry the chunk total
-    if (threadIdx.x < kNV) {
-      double t = scarry[threadIdx.x];
-      for (int w = 0; w < kWavesPerBlock; w++) t += swave[w * kNV + threadIdx.x];
-      scarry[threadIdx.x] = t;
-    }
-    __syncthreads();
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const bool stopped = sstop != 0x7fffffffffffffffLL;
-  round_conclude(ds, gc, sc, K, stopped ? sstop : -1, stopped ? sstop_state : scarry, slast);
+  stage_gcp_const(ds, L.gc);
+  block_sort_small(cand, K, L.sc);
+  block_round_scan(ds, L, L.sc, K, (const T*)Gv[ds->phase & 1], W);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,400 @@
+// Device side of the breakpoint search of the generalized Cauchy point:
+// candidate order, per-candidate contributions, the closed-form stop test and
+// the conclusion of a round.  Used by the grid kernels of gcp.hip and by the
+// per-problem batched kernel.  The method is described at the top of
+// gcp.hip.
+#pragma once
+
+#include "common.hpp"
+#include "control.hpp"
+#include "dense.hpp"
+#include "launch.hpp"
+#include "state.hpp"
+
+namespace lbfgsb {
+
+static constexpr int kSortTile = 2048;
+static constexpr int kNV = 33;
+static constexpr double kMachEpsG = 5e-16;
+
+__device__ inline bool cand_less(const Cand& a, const Cand& b) {
+  return a.t < b.t || (a.t == b.t && a.idx < b.idx);
+}
+
+// Same classification as the bulk passes (lbfgsbcpu.cpp:634-700).
+template <typename T>
+__device__ inline bool breakpoint_of(T x, T g, T l, T u, int nb, int iw, T& t) {
+  const T neggi = -g;
+  if (iw != 0 && iw != -1) return false;
+  if (nb <= 2 && nb != 0 && neggi < 0) {
+    t = (x - l) / (-neggi);
+    return true;
+  }
+  if (nb >= 2 && neggi > 0) {
+    t = (u - x) / neggi;
+    return true;
+  }
+  return false;
+}
+
+// Contribution of candidate c: [d w (2col) ; t d w (2col) ; d^2].
+template <typename T>
+__device__ inline void contribution(const Cand& c, const T* __restrict__ g,
+                                    const T* __restrict__ W, const RingReads& rr,
+                                    double theta, double (&v)[kNV]) {
+  const long long i = c.idx;
+  // the whole W row and g_i are loaded before any use (positions j >= col
+  // read as zero)
+  T yv[kMaxM], sv[kMaxM];
+  const T gi = g[i];
+  load_ring(W, i, rr, yv, sv);
+  const double d = -(double)gi;
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    const double wy = (double)yv[j];
+    const double ws = theta * (double)sv[j];
+    v[j] = d * wy;
+    v[8 + j] = d * ws;
+    v[16 + j] = c.t * d * wy;
+    v[24 + j] = c.t * d * ws;
+  }
+  v[32] = d * d;
+}
+
+// p at the start of the search and the middle matrix M in the padded
+// layout of the breakpoint sums (positions 0..7 Y, 8..15 S, zero beyond
+// col), staged in LDS by every thread of the block.
+struct GcpConst {
+  double pp[16];
+  double Mp[256];
+};
+__device__ inline void stage_gcp_const(const DevScalars* ds, GcpConst& gc) {
+  const int col = ds->col;
+  for (int q = threadIdx.x; q < 256; q += blockDim.x) gc.Mp[q] = ds->Mx[q];
+  if (threadIdx.x < 16) {
+    const int q = threadIdx.x;
+    const int j = q & 7;
+    gc.pp[q] = j < col ? ds->p[(q < 8 ? 0 : col) + j] : 0.0;
+  }
+  __syncthreads();
+}
+
+// Layout of a state vector: [P_y(8) P_s(8) Q_y(8) Q_s(8) D].
+
+// Stop test at the state `st` (passed breakpoints before t_next).
+// All loops run over the padded 16 positions with compile-time indices (the
+// entries beyond col are exact zeros, which leaves every sum unchanged), so
+// the temporaries stay in registers.
+__device__ inline bool stop_test(const DevScalars* ds, const GcpConst& gc,
+                                 const double (&st)[kNV], double t_prev, double t_next,
+                                 bool floor_on, double (&out)[3]) {
+  const double theta = ds->theta;
+  double p[16], mp[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) p[q] = gc.pp[q] - st[q];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) s += gc.Mp[r * 16 + k] * p[k];
+    mp[r] = s;
+  }
+  double pmp = 0.0, qmp = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    pmp += p[j] * mp[j] + p[8 + j] * mp[8 + j];
+    qmp += st[16 + j] * mp[j] + st[24 + j] * mp[8 + j];
+  }
+  const double D = ds->gcp_D0 - st[32];
+  const double f2u = theta * D - pmp;
+  const double f1 = -D + t_prev * f2u - qmp;
+  double f2 = f2u;
+  if (floor_on) {
+    const double fl = kMachEpsG * ds->gcp_f2org;
+    if (fl > f2) f2 = fl;
+  }
+  const double dtm = -f1 / f2;
+  out[0] = dtm;
+  out[1] = f1;
+  out[2] = f2;
+  return dtm < t_next - t_prev;
+}
+
+__device__ inline void gcp_done(DevScalars* ds) {
+  const int col = ds->col;
+  if (col > 0 && d_bmv(ds->sy, ds->wt, col, ds->c, ds->mc) != 0) ds->internalinfo = -8;
+  ds->nintol += ds->nint;
+  ds->gcp_pending = 0;
+  ds->status = ST_RUNNING;
+  ds->halt = 0;
+}
+
+// Conclusion of a round (thread 0).  stop_pos = first candidate at which the
+// search stops (its exclusive state in `st`), or -1 when every candidate
+// passed (then `st` is the inclusive total).  last_state = exclusive state
+// before the last candidate.
+__device__ inline void round_conclude(DevScalars* ds, const GcpConst& gc, const Cand* cand, long long K,
+                               long long stop_pos, const double* stv, const double* last_state) {
+  const int col = ds->col;
+  // the state stays where the caller keeps it (LDS or global), read in place
+  const double(&st)[kNV] = *reinterpret_cast<const double(*)[kNV]>(stv);
+  double tk, dtm = 0.0;
+  long long ik, kpass;
+  if (stop_pos >= 0) {
+    const long long pos = stop_pos;
+    tk = pos > 0 ? cand[pos - 1].t : ds->gcp_t_prev;
+    ik = pos > 0 ? cand[pos - 1].idx : ds->gcp_idx_prev;
+    kpass = ds->gcp_kpassed + pos;
+    double out[3];
+    stop_test(ds, gc, st, tk, cand[pos].t, kpass > 0, out);
+    dtm = out[0];
+  } else {
+    for (int q = 0; q < kNV; q++) ds->gcp_carry[q] = st[q];
+    ds->gcp_kpassed += K;
+    if (K > 0) {
+      ds->gcp_t_prev = cand[K - 1].t;
+      ds->gcp_idx_prev = cand[K - 1].idx;
+    }
+    if (ds->gcp_kpassed < ds->gcp_nbreak) {
+      // next round: a 4x larger t-range; sized by the host
+      ds->gcp_round++;
+      ds->gcp_ranges++;
+      ds->gcp_need_select = 1;
+      ds->gcp_tau_lo = ds->gcp_tau_hi;
+      const double nt = ds->gcp_tau_hi * 4.0;
+      ds->gcp_tau_hi = (ds->gcp_round >= 24 || !(nt < 1e300)) ? __builtin_inf() : nt;
+      ds->cand_count = 0;
+      ds->status = ST_GCP_ROUNDS;
+      ds->halt = 1;
+      return;
+    }
+    kpass = ds->gcp_kpassed;
+    tk = ds->gcp_t_prev;
+    ik = ds->gcp_idx_prev;
+    if (ds->gcp_nbreak == ds->n) {
+      // every variable was a breakpoint and all were passed: the reference
+      // returns before updating p with the last one (lbfgsbcpu.cpp:763-769);
+      // c = t_K p_{K-1} + Q_{K-1}
+      for (int j = 0; j < col; j++) {
+        const double py = ds->p[j] - last_state[j];
+        const double ps = ds->p[col + j] - last_state[8 + j];
+        ds->c[j] = tk * py + last_state[16 + j];
+        ds->c[col + j] = tk * ps + last_state[24 + j];
+        ds->p[j] = py;
+        ds->p[col + j] = ps;
+      }
+      ds->nint = (int)ds->gcp_nbreak;
+      ds->gcp_mode = 3;
+      ds->gcp_tsum = tk;
+      ds->gcp_last_t = tk;
+      ds->gcp_last_idx = ik;
+      gcp_done(ds);
+      return;
+    }
+    double out[3];
+    stop_test(ds, gc, st, tk, tk, true, out);
+    dtm = ds->gcp_unbnd ? out[0] : 0.0;
+  }
+  if (dtm <= 0) dtm = 0;
+  for (int j = 0; j < col; j++) {
+    const double py = ds->p[j] - st[j];
+    const double ps = ds->p[col + j] - st[8 + j];
+    ds->p[j] = py;
+    ds->p[col + j] = ps;
+    ds->c[j] = (tk + dtm) * py + st[16 + j];
+    ds->c[col + j] = (tk + dtm) * ps + st[24 + j];
+  }
+  ds->gcp_dtm = dtm;
+  ds->gcp_tsum = tk + dtm;
+  ds->gcp_last_t = tk;
+  ds->gcp_last_idx = ik;
+  ds->gcp_mode = kpass > 0 ? 1 : 0;
+  ds->nint = (int)(kpass + 1);
+  ds->gcp_passed_total += kpass;
+  gcp_done(ds);
+}
+
+// ---------------------------------------------------------------------------
+// A whole round in one workgroup (any blockDim that is a multiple of 64, up
+// to 1024): sort, then chunks of blockDim candidates scanned in order,
+// stopping at the first chunk that holds the stop.
+static constexpr int kSmallRound = 2048;  // candidates one workgroup sorts in LDS
+static constexpr int kMaxRoundWaves = 16;
+
+struct RoundLds {
+  Cand sc[kSmallRound];
+  double swave[kMaxRoundWaves * kNV];
+  double scarry[kNV];
+  double sstop_state[kNV];
+  double slast[kNV];
+  long long sstop;
+  GcpConst gc;
+};
+
+// Bitonic sort of sc[0 .. P) in LDS, P a power of two.
+__device__ inline void block_bitonic(Cand* sc, int P) {
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int p = threadIdx.x; p < P / 2; p += blockDim.x) {
+        const int i = (p / j) * 2 * j + (p % j), l = i + j;
+        const bool up = (i & k) == 0;
+        if (cand_less(sc[l], sc[i]) == up) {
+          Cand tmp = sc[i];
+          sc[i] = sc[l];
+          sc[l] = tmp;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Load K <= kSmallRound candidates into LDS, padded to a power of two, and
+// sort them.
+__device__ inline void block_sort_small(const Cand* cand, long long K, Cand* sc) {
+  int P = 64;
+  while (P < K) P <<= 1;
+  for (int j = threadIdx.x; j < P; j += blockDim.x) {
+    if (j < K) {
+      sc[j] = cand[j];
+    } else {
+      sc[j].t = __builtin_inf();
+      sc[j].idx = 0x7fffffff;
+      sc[j].pad = 0;
+    }
+  }
+  __syncthreads();
+  block_bitonic(sc, P);
+}
+
+// Sort K candidates of a (global) with one workgroup: kSmallRound-long runs
+// in LDS, then merge-path passes between a and b.  Returns the buffer that
+// holds the sorted sequence.
+__device__ inline Cand* block_sort_large(Cand* a, Cand* b, long long K, Cand* sc) {
+  for (long long base = 0; base < K; base += kSmallRound) {
+    const long long len = K - base < kSmallRound ? K - base : kSmallRound;
+    block_sort_small(a + base, len, sc);
+    for (int j = threadIdx.x; j < len; j += blockDim.x) a[base + j] = sc[j];
+    __syncthreads();
+  }
+  Cand* src = a;
+  Cand* dst = b;
+  for (long long width = kSmallRound; width < K; width <<= 1) {
+    const long long per = (K + blockDim.x - 1) / blockDim.x;
+    const long long o0 = (long long)threadIdx.x * per;
+    long long o = o0;
+    while (o < K && o < o0 + per) {
+      // output position o lies in the pair of runs starting at base
+      const long long base = (o / (2 * width)) * 2 * width;
+      const long long a0 = base, a1 = base + width < K ? base + width : K;
+      const long long b0 = a1, b1 = base + 2 * width < K ? base + 2 * width : K;
+      const long long la = a1 - a0, lb = b1 - b0;
+      const long long diag = o - base;
+      long long lo = diag - lb > 0 ? diag - lb : 0, hi = diag < la ? diag : la;
+      while (lo < hi) {
+        const long long mid = (lo + hi) >> 1;
+        if (cand_less(src[a0 + mid], src[b0 + diag - 1 - mid]))
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      long long i = lo, j = diag - lo;
+      const long long oend = (o0 + per < b1) ? o0 + per : b1;
+      for (; o < oend; o++) {
+        bool takeA;
+        if (i >= la)
+          takeA = false;
+        else if (j >= lb)
+          takeA = true;
+        else
+          takeA = !cand_less(src[b0 + j], src[a0 + i]);
+        dst[o] = takeA ? src[a0 + i++] : src[b0 + j++];
+      }
+    }
+    __syncthreads();
+    Cand* t = src;
+    src = dst;
+    dst = t;
+  }
+  return src;
+}
+
+// Scan sorted candidates (LDS or global) in chunks of blockDim: prefix sums
+// of the contributions, the stop test at every position, and the round's
+// conclusion by thread 0.  L.gc must be staged.
+template <typename T>
+__device__ inline void block_round_scan(DevScalars* ds, RoundLds& L, const Cand* cands,
+                                        long long K, const T* g, const T* W) {
+  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1);
+  const double theta = ds->theta;
+  const long long kpassed = ds->gcp_kpassed;
+  const double t_prev0 = ds->gcp_t_prev;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  if (threadIdx.x < kNV) L.scarry[threadIdx.x] = ds->gcp_carry[threadIdx.x];
+  if (threadIdx.x == 0) L.sstop = 0x7fffffffffffffffLL;
+  __syncthreads();
+  for (long long c0 = 0; c0 < K; c0 += blockDim.x) {
+    const long long pos = c0 + threadIdx.x;
+    double v[kNV], ex[kNV];
+#pragma unroll
+    for (int q = 0; q < kNV; q++) v[q] = 0.0;
+    Cand c;
+    c.t = 0;
+    c.idx = 0;
+    c.pad = 0;
+    if (pos < K) {
+      c = cands[pos];
+      contribution(c, g, W, rr, theta, v);
+    }
+#pragma unroll
+    for (int q = 0; q < kNV; q++) {
+      double incl = v[q];
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const double up = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += up;
+      }
+      ex[q] = incl - v[q];
+      if (lane == 63) L.swave[wid * kNV + q] = incl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kNV; q++) {
+      double off = L.scarry[q];
+      for (int w = 0; w < wid; w++) off += L.swave[w * kNV + q];
+      ex[q] += off;
+    }
+    bool stop = false;
+    if (pos < K && kpassed + pos > 0) {
+      const double tp = pos > 0 ? cands[pos - 1].t : t_prev0;
+      double out[3];
+      stop = stop_test(ds, L.gc, ex, tp, c.t, true, out);
+    }
+    if (stop) atomicMin(&L.sstop, pos);
+    if (pos == K - 1)
+      for (int q = 0; q < kNV; q++) L.slast[q] = ex[q];
+    __syncthreads();
+    if (L.sstop != 0x7fffffffffffffffLL) {
+      if (pos == L.sstop)
+        for (int q = 0; q < kNV; q++) L.sstop_state[q] = ex[q];
+      break;
+    }
+    // carry the chunk total
+    if (threadIdx.x < kNV) {
+      double t = L.scarry[threadIdx.x];
+      for (int w = 0; w < nw; w++) t += L.swave[w * kNV + threadIdx.x];
+      L.scarry[threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const bool stopped = L.sstop != 0x7fffffffffffffffLL;
+    round_conclude(ds, L.gc, cands, K, stopped ? L.sstop : -1,
+                   stopped ? L.sstop_state : L.scarry, L.slast);
+  }
+  __syncthreads();
+}
+
+}  // namespace lbfgsb

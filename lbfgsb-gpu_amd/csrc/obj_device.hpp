@@ -1,0 +1,70 @@
+// Per-element terms of the built-in objectives (device side), shared by the
+// grid-wide objective kernel and the per-problem batched kernel.
+//   GRID_QUAD  f = 0.5 sum_edges (v_a - v_b)^2 - h^2 sum b v  (zero ring)
+//   GRID_COS   f = 0.5 sum_edges (dv)^2 + h^2 sum (c cos v - b v)
+//   ROSEN      f = sum_i 100 (x_{2i} - x_{2i-1}^2)^2 + (1 - x_{2i-1})^2
+//   PENTA      f = 0.5 x'Ax - b'x, A symmetric pentadiagonal
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "objectives.hpp"
+
+namespace lbfgsb {
+
+// Number of terms k of an objective over n variables (ROSEN: pairs).
+__device__ inline long long obj_terms(const ObjDesc& o, long long n) {
+  return o.kind == OBJK_ROSEN ? n / 2 : n;
+}
+
+// Term k: writes the gradient entries it owns and returns its share of f.
+template <typename T>
+__device__ inline double obj_term(const ObjDesc& o, long long n, const T* __restrict__ x,
+                                  T* __restrict__ g, long long k) {
+  if (o.kind == OBJK_GRID_QUAD || o.kind == OBJK_GRID_COS) {
+    const T* b = (const T*)o.b;
+    const double h2 = o.h * o.h;
+    const long long nx = o.nx, ny = o.ny;
+    const long long ix = k % nx, iy = k / nx;
+    const double v = (double)x[k];
+    const double vl = ix > 0 ? (double)x[k - 1] : 0.0;
+    const double vr = ix < nx - 1 ? (double)x[k + 1] : 0.0;
+    const double vd = iy > 0 ? (double)x[k - nx] : 0.0;
+    const double vu = iy < ny - 1 ? (double)x[k + nx] : 0.0;
+    const double bk = (double)b[k];
+    double gk = (4.0 * v - vl - vr - vd - vu) - h2 * bk;
+    const double er = vr - v, eu = vu - v;
+    double e = er * er + eu * eu;
+    if (ix == 0) e += v * v;
+    if (iy == 0) e += v * v;
+    double fk = 0.5 * e - h2 * bk * v;
+    if (o.kind == OBJK_GRID_COS) {
+      const double ck = (double)((const T*)o.c)[k];
+      fk += h2 * ck * cos(v);
+      gk -= h2 * ck * sin(v);
+    }
+    g[k] = (T)gk;
+    return fk;
+  }
+  if (o.kind == OBJK_ROSEN) {
+    const double a = (double)x[2 * k], bb = (double)x[2 * k + 1];
+    const double t1 = bb - a * a, t2 = 1.0 - a;
+    g[2 * k] = (T)(-400.0 * t1 * a - 2.0 * t2);
+    g[2 * k + 1] = (T)(200.0 * t1);
+    return 100.0 * t1 * t1 + t2 * t2;
+  }
+  const T* a0 = (const T*)o.a0;
+  const T* a1 = (const T*)o.a1;
+  const T* a2 = (const T*)o.a2;
+  const T* b = (const T*)o.b;
+  const double xi = (double)x[k];
+  double ax = (double)a0[k] * xi;
+  if (k + 1 < n) ax += (double)a1[k] * (double)x[k + 1];
+  if (k + 2 < n) ax += (double)a2[k] * (double)x[k + 2];
+  if (k >= 1) ax += (double)a1[k - 1] * (double)x[k - 1];
+  if (k >= 2) ax += (double)a2[k - 2] * (double)x[k - 2];
+  g[k] = (T)(ax - (double)b[k]);
+  return 0.5 * ax * xi - (double)b[k] * xi;
+}
+
+}  // namespace lbfgsb

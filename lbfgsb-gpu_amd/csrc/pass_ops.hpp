@@ -1,0 +1,580 @@
+// Element operators of the bulk passes.  Each operator holds the uniform
+// parameters of one pass (read from the device state), the per-thread
+// accumulators, and three steps per element: load (every global load of the
+// element, issued together), keep (see keep_loads) and element (the
+// arithmetic and the stores).  The grid-wide kernels of bulk.hip and the
+// per-problem batched kernel drive the same operators with different loops
+// and reductions, so both forms compute the same per-element values.
+#pragma once
+
+#include "common.hpp"
+#include "pass_device.hpp"
+#include "state.hpp"
+
+namespace lbfgsb {
+
+// ---------------------------------------------------------------------------
+// Pass A: the iteration tail (projected gradient norm, ||x - x_old||^2,
+// y = g - g_old, y'y, the new S/Y pair written at ds->itail and its Gram row
+// s'y_j, s_j's) when FUSED, and the Cauchy classification sums (f1, p = W'd
+// over the GCP-free set, breakpoint counts, smallest breakpoint)
+// (lbfgsbcpu.cpp:394-449, 567-701, 1341-1387).
+template <typename T, bool FUSED>
+struct PassAOp {
+  const T* __restrict__ x;
+  const T* __restrict__ g;
+  const T* __restrict__ xold;
+  const T* __restrict__ gold;
+  const T* __restrict__ l;
+  const T* __restrict__ u;
+  const int* __restrict__ nbd;
+  uint8_t* __restrict__ iwhere;
+  T* __restrict__ W;
+  const T* __restrict__ d;
+  int col, itail;
+  bool upd, stp1;
+  T stp;
+  RingReads rr;
+  double acc[PA_COUNT];
+  double sbg;
+  double pny, pns;  // new pair's share of p (position col - 1)
+  double bkmin;
+  long long ibk;
+
+  struct In {
+    T x, g, l, u, xo, go, dd;
+    int nb;
+    uint8_t iwb;
+    T yv[kMaxM], sv[kMaxM];
+  };
+
+  __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
+                              const T* u_, const int* nbd_, uint8_t* iwhere_, T* W_,
+                              const T* d_) {
+    const int cur = ds->phase & 1;
+    x = X[cur];
+    g = Gv[cur];
+    xold = X[cur ^ 1];
+    gold = Gv[cur ^ 1];
+    l = l_;
+    u = u_;
+    nbd = nbd_;
+    iwhere = iwhere_;
+    W = W_;
+    d = d_;
+    col = ds->col;
+    itail = ds->itail;
+    upd = FUSED && ds->updatd;
+    stp = (T)ds->stp;
+    stp1 = ds->stp == 1.0;
+    // the slot being replaced by the new pair is not read
+    rr = ring_reads(ds->head, ds->m, col, upd ? itail : -1);
+#pragma unroll
+    for (int v = 0; v < PA_COUNT; v++) acc[v] = 0.0;
+    sbg = 0.0;
+    pny = pns = 0.0;
+    bkmin = __builtin_inf();
+    ibk = 0x7fffffffffffffffLL;
+  }
+  __device__ __forceinline__ void load(In& e, long long i) const {
+    e.x = x[i];
+    e.g = g[i];
+    e.l = l[i];
+    e.u = u[i];
+    e.nb = nbd[i];
+    e.iwb = iwhere[i];
+    e.xo = FUSED ? xold[i] : (T)0;
+    e.go = upd ? gold[i] : (T)0;
+    e.dd = upd ? d[i] : (T)0;
+    load_ring(W, i, rr, e.yv, e.sv);
+  }
+  __device__ __forceinline__ void keep(const In& e) const {
+    keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.xo, e.go, e.dd, e.yv, e.sv);
+  }
+  __device__ __forceinline__ void element(long long i, In& e) {
+    const T xi = e.x, gi = e.g, li = e.l, ui = e.u;
+    const int nb = e.nb;
+    const uint8_t iwb = e.iwb;
+    T ys = 0, ss = 0;
+    if (FUSED) {
+      const T gp = projected_gi(xi, gi, li, ui, nb);
+      sbg = fmax(sbg, (double)fabs(gp));
+      const T xd = e.xo - xi;
+      acc_prod(acc[PA_TF], xd, xd);
+      if (upd) {
+        ys = gi - e.go;
+        ss = stp1 ? e.dd : e.dd * stp;
+        acc_prod(acc[PA_RR], ys, ys);
+        W[widx(i, itail)] = ys;
+        W[widx(i, 8 + itail)] = ss;
+      }
+    }
+    // Unused positions hold zeros (load_ring), and with an update the
+    // newest position col - 1 (slot itail, not read) is zero too: the sums
+    // run over all positions without guards, and the new pair's share of p
+    // goes to pny / pns (added at position col - 1 by finish()).
+    if (upd) {
+#pragma unroll
+      for (int j = 0; j < kMaxM - 1; j++) {
+        acc_prod(acc[PA_SY + j], ss, e.yv[j]);
+        acc_prod(acc[PA_SS + j], e.sv[j], ss);
+      }
+    }
+    int iw;
+    T neggi, tbrk;
+    bool isfree, isbrk;
+    classify(xi, gi, li, ui, nb, iw_get(iwb), iw, neggi, isfree, isbrk, tbrk);
+    // GCP-free variables feed f1 and p; the others add an exact zero
+    const T nf = isfree ? neggi : (T)0;
+    acc_prod(acc[PA_F1], nf, nf);
+#pragma unroll
+    for (int j = 0; j < kMaxM; j++) {
+      acc_prod(acc[PA_P + j], e.yv[j], nf);
+      acc_prod(acc[PA_P + 8 + j], e.sv[j], nf);
+    }
+    if (upd) {
+      acc_prod(pny, ys, nf);
+      acc_prod(pns, ss, nf);
+    }
+    if (isfree) {
+      if (isbrk) {
+        acc[PA_NBREAK] += 1.0;
+        const double tt = (double)tbrk;
+        if (tt < bkmin || (tt == bkmin && i < ibk)) {
+          bkmin = tt;
+          ibk = i;
+        }
+      } else {
+        acc[PA_NFREEGCP] += 1.0;
+        if (fabs(neggi) > 0) acc[PA_UNBND] = 1.0;  // an OR, kept 0/1 through the sum
+      }
+    }
+    iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb));
+  }
+  __device__ __forceinline__ void finish() {
+    if (upd) {
+#pragma unroll
+      for (int j = 0; j < kMaxM; j++) {
+        acc[PA_P + j] += j == col - 1 ? pny : 0.0;
+        acc[PA_P + 8 + j] += j == col - 1 ? pns : 0.0;
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Cauchy-point pass: x_cp from the search's outcome, free / entering /
+// leaving sets, the new formk column, the signed entering/leaving Gram
+// pairs, r = -theta (z - x) - g + W M c and W'r over the free set
+// (lbfgsbcpu.cpp:705-817, 819-869, 907-1054, 1135-1173, 1446-1458).
+template <typename T>
+struct P4Op {
+  static constexpr int kAcc = P4_CORR;  // 3 counts, 32 new-column sums, 16 W'r
+  const T* __restrict__ x;
+  const T* __restrict__ g;
+  const T* __restrict__ l;
+  const T* __restrict__ u;
+  const int* __restrict__ nbd;
+  uint8_t* __restrict__ iwhere;
+  const T* __restrict__ W;
+  T* __restrict__ z;
+  T* __restrict__ r;
+  DevScalars* ds;
+  int col, mode;
+  bool shortcut, track, newcol;
+  T tsum;
+  double t_last, theta;
+  long long i_last;
+  RingReads rr;
+  double mcy[kMaxM], mcs[kMaxM];
+  double acc[kAcc];
+  // signed Gram pairs over entering (+1) / leaving (-1) variables: lane L of
+  // every wave owns pairs L, L+64, L+128 (< 136); wa/wb are the W columns
+  // (0..7 = Y slots, 8..15 = S slots) of the pair's two factors
+  double corr[3];
+  int wa[3], wb[3];
+  bool pact[3];
+
+  struct In {
+    T x, g, l, u;
+    int nb;
+    uint8_t iwb;
+    T yv[kMaxM], sv[kMaxM];
+    T yn, sn;  // the newest pair again (a cache hit): no select over positions
+  };
+
+  __device__ __forceinline__ void init(DevScalars* ds_, Pair<T> X, Pair<T> Gv, const T* l_, const T* u_,
+                              const int* nbd_, uint8_t* iwhere_, const T* W_, T* z_, T* r_) {
+    ds = ds_;
+    const int cur = ds->phase & 1;
+    x = X[cur];
+    g = Gv[cur];
+    l = l_;
+    u = u_;
+    nbd = nbd_;
+    iwhere = iwhere_;
+    W = W_;
+    z = z_;
+    r = r_;
+    const int m = ds->m, head = ds->head;
+    col = ds->col;
+    shortcut = ds->shortcut;
+    mode = ds->gcp_mode;
+    tsum = (T)ds->gcp_tsum;
+    t_last = ds->gcp_last_t;
+    i_last = ds->gcp_last_idx;
+    track = ds->iter > 0 && ds->cnstnd && !shortcut;
+    newcol = ds->updatd && col > 0;
+    theta = ds->theta;
+    const int upcl = ds->updatd ? col - 1 : col;
+    rr = ring_reads(head, m, col, -1);
+#pragma unroll
+    for (int j = 0; j < kMaxM; j++) {
+      mcy[j] = j < col ? ds->mc[j] : 0.0;
+      mcs[j] = j < col ? theta * ds->mc[col + j] : 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < kAcc; v++) acc[v] = 0.0;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      corr[q] = 0.0;
+      const int t = lane + 64 * q;
+      int cat = 0, a = 0, b = 0;
+      if (t < 136) corr_pair(t, cat, a, b);
+      pact[q] = t < 136 && a < upcl && b < upcl;
+      int sa = head + a, sb = head + b;
+      if (sa >= m) sa -= m;
+      if (sb >= m) sb -= m;
+      wa[q] = cat == 0 ? sa : 8 + sa;
+      wb[q] = cat == 1 ? 8 + sb : sb;
+    }
+  }
+  __device__ __forceinline__ void load(In& e, long long i) const {
+    e.x = x[i];
+    e.g = g[i];
+    e.l = l[i];
+    e.u = u[i];
+    e.nb = nbd[i];
+    e.iwb = iwhere[i];
+    load_ring(W, i, rr, e.yv, e.sv);
+    const T* __restrict__ row = W + widx((size_t)i, 0);
+    e.yn = row[rr.newest * kTile];
+    e.sn = row[(8 + rr.newest) * kTile];
+  }
+  __device__ __forceinline__ void keep(const In& e) const {
+    keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.yv, e.sv, e.yn, e.sn);
+  }
+  // Returns +1 / -1 for an entering / leaving variable, 0 otherwise.
+  __device__ __forceinline__ int element(long long i, const In& e) {
+    int flag = 0;
+    const T xi = e.x, gi = e.g, li = e.l, ui = e.u;
+    const int nb = e.nb;
+    const uint8_t iwb = e.iwb;
+    const bool wasfree = iw_wasfree(iwb);
+    int iw = iw_get(iwb);
+    T zi = xi;
+    if (!shortcut) {
+      T neggi, tb;
+      bool isfree, isbrk;
+      classify(xi, gi, li, ui, nb, iw, iw, neggi, isfree, isbrk, tb);
+      const T dcp = isfree ? neggi : (T)0;
+      if (mode == 3) {
+        if (isbrk) {
+          iw = dcp > 0 ? 2 : 1;
+          zi = dcp > 0 ? ui : li;
+        }
+      } else if (mode != 2) {
+        bool passed = false;
+        if (mode == 1 && isbrk) {
+          const double tt = (double)tb;
+          passed = tt < t_last || (tt == t_last && i <= i_last);
+        }
+        if (passed) {
+          iw = dcp > 0 ? 2 : 1;
+          zi = dcp > 0 ? ui : li;
+        } else {
+          zi = xi + tsum * dcp;
+        }
+      }
+    }
+    const bool freenow = iw <= 0;
+    if (!shortcut) {
+      acc[P4_NFREE] += freenow ? 1.0 : 0.0;
+      if (track) {
+        if (!wasfree && freenow) {
+          acc[P4_NENTER] += 1.0;
+          flag = 1;
+        }
+        if (wasfree && !freenow) {
+          acc[P4_NLEAVE] += 1.0;
+          flag = -1;
+        }
+      }
+      iwhere[i] = iw_make(iw, freenow, wasfree);
+      z[i] = zi;
+    }
+    // Positions j >= col hold zeros (load_ring), so the sums below run over
+    // all kMaxM positions without guards: unused accumulators collect exact
+    // zeros and are never read.
+    if (col > 0) {
+      if (newcol) {
+        // free variables feed the Y'Y / S'Y sums, active ones the S'S / S'Y
+        // sums; the other side adds an exact zero (no divergent stores,
+        // which the compiler would merge into a dynamically indexed one)
+        const T yf = freenow ? e.yn : (T)0, sa = freenow ? (T)0 : e.sn;
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++) {
+          acc_prod(acc[P4_NC + j], yf, e.yv[j]);
+          acc_prod(acc[P4_NC + 24 + j], e.sv[j], yf);
+          acc_prod(acc[P4_NC + 8 + j], sa, e.sv[j]);
+          acc_prod(acc[P4_NC + 16 + j], sa, e.yv[j]);
+        }
+      }
+      // r over the free set; 0 elsewhere, so W'r needs no branch
+      double ri = -(double)gi;
+      if (!shortcut) {
+        ri = -theta * (double)(zi - xi) - (double)gi;
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++) {
+          ri = __builtin_fma((double)e.yv[j], mcy[j], ri);
+          ri = __builtin_fma((double)e.sv[j], mcs[j], ri);
+        }
+      }
+      const T rt = freenow ? (T)ri : (T)0;
+      if (freenow) r[i] = rt;
+#pragma unroll
+      for (int j = 0; j < kMaxM; j++) {
+        acc_prod(acc[P4_WV + j], e.yv[j], rt);
+        acc_prod(acc[P4_WV + 8 + j], e.sv[j], rt);
+      }
+    }
+    return flag;
+  }
+  // Signed Gram over this wave's entering / leaving variables, one flagged
+  // variable at a time in lane order (all lanes of the wave take part).
+  __device__ __forceinline__ void corr_step(long long i, int flag) {
+    if (!(track && col > 0)) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long fm = __ballot(flag != 0);
+    if (fm && lane == 0) ds->p4_flagged = 1;
+    while (fm) {
+      const int src = __ffsll((long long)fm) - 1;
+      fm &= fm - 1ull;
+      const long long ei = __shfl(i, src, kWave);
+      const double sg = (double)__shfl(flag, src, kWave);
+      const double wrow = lane < 16 ? (double)W[widx(ei, lane)] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const double va = __shfl(wrow, wa[q], kWave);
+        const double vb = __shfl(wrow, wb[q], kWave);
+        if (pact[q]) corr[q] += sg * va * vb;
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Subspace direction over the free variables and the step-to-box ratio
+// (lbfgsbcpu.cpp:1472-1516).
+template <typename T>
+struct P5aOp {
+  const T* __restrict__ l;
+  const T* __restrict__ u;
+  const int* __restrict__ nbd;
+  const uint8_t* __restrict__ iwhere;
+  const T* __restrict__ W;
+  const T* __restrict__ z;
+  const T* __restrict__ r;
+  T* __restrict__ dsub;
+  int col;
+  bool shortcut;
+  double theta;
+  RingReads rr;
+  double wy[kMaxM], wsv[kMaxM];
+  double amin;
+  long long ai;
+
+  struct In {
+    T r, z, l, u;
+    int nb;
+    uint8_t iwb;
+    T yv[kMaxM], sv[kMaxM];
+  };
+
+  __device__ __forceinline__ void init(const DevScalars* ds, const T* l_, const T* u_, const int* nbd_,
+                              const uint8_t* iwhere_, const T* W_, const T* z_, const T* r_,
+                              T* dsub_) {
+    l = l_;
+    u = u_;
+    nbd = nbd_;
+    iwhere = iwhere_;
+    W = W_;
+    z = z_;
+    r = r_;
+    dsub = dsub_;
+    col = ds->col;
+    shortcut = ds->shortcut;
+    theta = ds->theta;
+#pragma unroll
+    for (int j = 0; j < kMaxM; j++) {
+      wy[j] = j < col ? ds->wv[j] / theta : 0.0;
+      wsv[j] = j < col ? ds->wv[col + j] : 0.0;
+    }
+    rr = ring_reads(ds->head, ds->m, col, -1);
+    amin = 1.0;
+    ai = 0x7fffffffffffffffLL;
+  }
+  __device__ __forceinline__ void load(In& e, long long i) const {
+    e.iwb = iwhere[i];
+    e.r = r[i];
+    e.z = z[i];
+    e.l = l[i];
+    e.u = u[i];
+    e.nb = nbd[i];
+    load_ring(W, i, rr, e.yv, e.sv);
+  }
+  __device__ __forceinline__ void keep(const In& e) const {
+    keep_loads(e.r, e.z, e.l, e.u, e.nb, e.iwb, e.yv, e.sv);
+  }
+  __device__ __forceinline__ void element(long long i, const In& e) {
+    const bool freenow = shortcut || iw_get(e.iwb) <= 0;
+    if (!freenow) return;
+    double di = (double)e.r;
+#pragma unroll
+    for (int j = 0; j < kMaxM; j++)
+      if (j < col) di += (double)e.yv[j] * wy[j] + (double)e.sv[j] * wsv[j];
+    const T dk = (T)(di / theta);
+    dsub[i] = dk;
+    const int nb = e.nb;
+    if (nb != 0) {
+      double cand = 2.0;
+      if (dk < 0 && nb <= 2) {
+        const T t2 = e.l - e.z;
+        if (t2 >= 0)
+          cand = 0.0;
+        else if (dk < t2)
+          cand = (double)(t2 / dk);
+      } else if (dk > 0 && nb >= 2) {
+        const T t2 = e.u - e.z;
+        if (t2 <= 0)
+          cand = 0.0;
+        else if (dk > t2)
+          cand = (double)(t2 / dk);
+      }
+      if (cand < amin || (cand == amin && cand < 1.0 && i < ai)) {
+        amin = cand;
+        ai = i;
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Backtrack (pinning the limiting variable), d = z - x, and the line-search
+// set-up sums d'd, g'd and the nbd-aware max step (lbfgsbcpu.cpp:1517-1533,
+// 343-345, 1263-1299).
+template <typename T>
+struct P5bOp {
+  const T* __restrict__ x;
+  const T* __restrict__ g;
+  const T* __restrict__ l;
+  const T* __restrict__ u;
+  const int* __restrict__ nbd;
+  const uint8_t* __restrict__ iwhere;
+  T* __restrict__ z;
+  const T* __restrict__ dsub;
+  T* __restrict__ d;
+  bool sub, shortcut, pin, want_stpmx;
+  T alpha;
+  long long ibd;
+  double acc[P5_COUNT];
+  double smin;
+
+  struct In {
+    T x, g, l, u, z, dsub;
+    uint8_t iwb;
+    int nb;
+  };
+
+  __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
+                              const T* u_, const int* nbd_, const uint8_t* iwhere_, T* z_,
+                              const T* dsub_, T* d_) {
+    const int cur = ds->phase & 1;
+    x = X[cur];
+    g = Gv[cur];
+    l = l_;
+    u = u_;
+    nbd = nbd_;
+    iwhere = iwhere_;
+    z = z_;
+    dsub = dsub_;
+    d = d_;
+    sub = !ds->skip_subsm;
+    shortcut = ds->shortcut;
+    alpha = (T)ds->alpha;
+    pin = ds->alpha < 1.0;
+    ibd = ds->ibd;
+    want_stpmx = ds->cnstnd && ds->iter > 0;
+    acc[0] = acc[1] = 0.0;
+    smin = 1.0e10;
+  }
+  __device__ __forceinline__ void load(In& e, long long i) const {
+    e.x = x[i];
+    e.g = g[i];
+    e.l = l[i];
+    e.u = u[i];
+    e.z = shortcut ? (T)0 : z[i];
+    e.iwb = iwhere[i];
+    e.dsub = sub ? dsub[i] : (T)0;
+    e.nb = nbd[i];
+  }
+  __device__ __forceinline__ void keep(const In& e) const {
+    keep_loads(e.x, e.g, e.l, e.u, e.z, e.iwb, e.dsub, e.nb);
+  }
+  __device__ __forceinline__ void element(long long i, const In& e) {
+    const T big = (T)1.0e10;
+    const T xi = e.x;
+    T zi = shortcut ? xi : e.z;
+    if (sub && (shortcut || iw_get(e.iwb) <= 0)) {
+      T dk = e.dsub;
+      if (pin && i == ibd) {
+        if (dk > 0) {
+          zi = e.u;
+          dk = 0;
+        } else if (dk < 0) {
+          zi = e.l;
+          dk = 0;
+        }
+      }
+      zi = zi + alpha * dk;
+      z[i] = zi;
+    } else if (shortcut) {
+      z[i] = zi;
+    }
+    const T di = zi - xi;
+    d[i] = di;
+    acc[P5_DTD] += (double)di * (double)di;
+    acc[P5_GD] += (double)e.g * (double)di;
+    if (want_stpmx) {
+      const int nb = e.nb;
+      if (nb != 0) {
+        if (di < 0 && nb <= 2) {
+          const T a2 = e.l - xi;
+          if (a2 >= 0)
+            smin = 0.0;
+          else if (di * big < a2)
+            smin = fmin(smin, (double)(a2 / di));
+        } else if (di > 0 && nb >= 2) {
+          const T a2 = e.u - xi;
+          if (a2 <= 0)
+            smin = 0.0;
+          else if (di * big > a2)
+            smin = fmin(smin, (double)(a2 / di));
+        }
+      }
+    }
+  }
+};
+
+}  // namespace lbfgsb

@@ -12,6 +12,39 @@
 
 namespace lbfgsb {
 
+// Steps of the iteration chain: kernels of the plain engine, block steps of
+// the batched solver.
+enum Step : int {
+  K_RESUME_RESET = 0,
+  K_RESUME,
+  K_PASSA_SA,   // standalone Cauchy classification pass
+  K_UPD_REDUCE, // controller after the standalone pass
+  K_PASSA_FUSED,
+  K_STOP,
+  K_UPD,
+  K_P4,
+  K_FORMK,
+  K_P5A,
+  K_ALPHA,
+  K_P5B,
+  K_LSINIT,
+  K_TRIAL,
+  K_GD,
+  K_LSSTEP,
+  K_END,        // also the timing slot of host-sized breakpoint rounds
+  K_GCP = 17    // device-sized breakpoint round (select + one-workgroup round)
+};
+constexpr int kNumSteps = 18;
+
+// Programs (sequences of steps).
+enum Program : int {
+  PROG_ITER_START = 0,  // iteration 0 and after every memory reset
+  PROG_AFTER_EVAL = 1,  // a function value for the pending trial is known
+  PROG_AFTER_ACCEPT = 2,
+  PROG_AFTER_CHECK = 3,
+  PROG_TAIL = 4
+};
+
 // Host-visible status (DevScalars::status).
 enum Status : int {
   ST_RUNNING = 0,     // chain continues on the device
@@ -128,10 +161,18 @@ struct DevScalars {
   unsigned long long cand_count;
 
   int gcp_pending;   // sorted breakpoint rounds are in progress
+  int gcp_need_select;  // the current t-range's candidates are not selected yet
+  int gcp_ranges;       // statistics: t-ranges opened after the first one
   int upd_done;      // the memory update of this iteration already ran
   int p4_flagged;    // k_p4 saw entering/leaving variables
   int gcp_rounds_run;  // statistics: Cauchy searches that needed sorted rounds
   long long gcp_passed_total;  // statistics: breakpoints passed in those
+
+  // ---- resumable solve of the batched form: set up done, next program of
+  // the chain and the function value that program takes
+  int b_started;
+  int b_next_prog;
+  double b_next_f;
 
   // ---- finished row sums of the last multi-block reduction (k_rowsum)
   double red[P4_COUNT + 1];

@@ -97,6 +97,21 @@ SIGNATURES = {
                                               C.c_void_p, C.c_void_p]),
     "lbfgsb_problem_eval_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.c_void_p]),
+    "lbfgsb_batch_create": (C.c_void_p, [C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_void_p]),
+    "lbfgsb_batch_destroy": (None, [C.c_void_p]),
+    "lbfgsb_batch_add": (C.c_int, [C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_int, C.c_longlong, C.c_longlong, C.c_double,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_double, C.c_double, C.c_double, C.c_int]),
+    "lbfgsb_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "lbfgsb_batch_result": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "lbfgsb_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]),
+    "lbfgsb_batch_get_x": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "lbfgsb_batch_remove": (C.c_int, [C.c_void_p, C.c_int]),
+    "lbfgsb_batch_free_elems": (C.c_longlong, [C.c_void_p]),
     "lbfgsb_amd_version": (C.c_char_p, []),
 }
 

@@ -1,0 +1,147 @@
+"""Batched form over the C ABI (lbfgsb_batch_*): an HBM arena of problem
+slots solved by the persistent per-problem kernel.
+
+    b = Batch(np.float32, capacity=sum_n, max_problems=P)
+    slots = [b.add(p, eps=1e-6, max_iter=1000) for p in problems]
+    b.run()                     # every problem to completion
+    f, iters, info, done = b.results()
+
+`run(iter_budget=k)` advances each unfinished problem by at most k
+iterations and returns how many remain; the serving loop admits new problems
+between such calls.  `tune()` picks the workgroup size and the number of
+resident workgroups by timing short runs on the admitted problems.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib
+from .problems import Problem
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("lbfgsb_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+class Batch:
+    def __init__(self, dtype, capacity: int, max_problems: int, m: int = 8):
+        self.torch = _torch()
+        self.dtype = np.dtype(dtype)
+        self.f64 = self.dtype == np.float64
+        self.L = _lib.lib()
+        self.stream = C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+        self.h = self.L.lbfgsb_batch_create(1 if self.f64 else 0, int(capacity), int(max_problems),
+                                            int(m), self.stream)
+        if not self.h:
+            raise RuntimeError("lbfgsb_batch_create failed (device memory?)")
+        self.max_problems = int(max_problems)
+        self.n = {}
+        self.threads, self.blocks = 256, 0
+
+    def _dev(self, a, tdt):
+        return self.torch.from_numpy(np.ascontiguousarray(a)).to("cuda", tdt)
+
+    def add(self, p: Problem, eps: float = 1e-6, max_iter: int = 1000, eps_g=None, eps_f=None,
+            eps_x=None) -> int:
+        """Admit problem p (uploaded to the device first).  Returns the slot,
+        -1 for bad input, -3 when the arena is full."""
+        t = self.torch
+        tdt = t.float64 if self.f64 else t.float32
+        q = p.cast(self.dtype)
+        arrs = {"x0": self._dev(q.x0, tdt), "l": self._dev(q.l, tdt), "u": self._dev(q.u, tdt),
+                "nbd": self._dev(q.nbd, t.int32)}
+        for k in ("b", "c", "a0", "a1", "a2"):
+            a = getattr(q, k)
+            if a is not None:
+                arrs[k] = self._dev(a, tdt)
+        ptr = lambda k: C.c_void_p(arrs[k].data_ptr()) if k in arrs else None
+        slot = self.L.lbfgsb_batch_add(
+            self.h, p.n, ptr("x0"), ptr("nbd"), ptr("l"), ptr("u"), p.kind, p.nx, p.ny, p.h,
+            ptr("b"), ptr("c"), ptr("a0"), ptr("a1"), ptr("a2"),
+            eps if eps_g is None else eps_g, eps if eps_f is None else eps_f,
+            eps if eps_x is None else eps_x, int(max_iter))
+        if slot >= 0:
+            self.n[slot] = p.n
+        return slot
+
+    def add_device(self, n, x0, nbd, l, u, kind, nx=0, ny=0, h=0.0, b=None, c=None, a0=None,
+                   a1=None, a2=None, eps=1e-6, max_iter=1000) -> int:
+        """Admit a problem whose arrays are already device tensors."""
+        ptr = lambda a: C.c_void_p(a.data_ptr()) if a is not None else None
+        slot = self.L.lbfgsb_batch_add(self.h, int(n), ptr(x0), ptr(nbd), ptr(l), ptr(u),
+                                       int(kind), int(nx), int(ny), float(h), ptr(b), ptr(c),
+                                       ptr(a0), ptr(a1), ptr(a2), eps, eps, eps, int(max_iter))
+        if slot >= 0:
+            self.n[slot] = int(n)
+        return slot
+
+    def run(self, iter_budget: int = 0, threads: int | None = None, blocks: int | None = None):
+        th = self.threads if threads is None else threads
+        bl = self.blocks if blocks is None else blocks
+        left = self.L.lbfgsb_batch_run(self.h, int(th), int(bl), int(iter_budget))
+        if left < 0:
+            raise RuntimeError(f"lbfgsb_batch_run failed ({left})")
+        return left
+
+    def results(self):
+        P = self.max_problems
+        f = np.zeros(P, np.float64)
+        it = np.zeros(P, np.int32)
+        info = np.zeros(P, np.int32)
+        done = np.zeros(P, np.int32)
+        self.L.lbfgsb_batch_results(self.h, f.ctypes.data_as(C.c_void_p),
+                                    it.ctypes.data_as(C.c_void_p),
+                                    info.ctypes.data_as(C.c_void_p),
+                                    done.ctypes.data_as(C.c_void_p))
+        return f, it, info, done
+
+    def x(self, slot: int):
+        t = self.torch
+        out = t.empty(self.n[slot], dtype=t.float64 if self.f64 else t.float32, device="cuda")
+        if self.L.lbfgsb_batch_get_x(self.h, slot, C.c_void_p(out.data_ptr())) != 0:
+            raise RuntimeError("lbfgsb_batch_get_x failed")
+        return out
+
+    def remove(self, slot: int):
+        if self.L.lbfgsb_batch_remove(self.h, slot) == 0:
+            self.n.pop(slot, None)
+
+    def free_elems(self) -> int:
+        return int(self.L.lbfgsb_batch_free_elems(self.h))
+
+    def tune(self, budgets=(2,), candidates=((256, 0), (512, 0), (256, -2))):
+        """Time a short run (iter_budget) of the admitted problems for each
+        (threads, blocks) candidate -- blocks 0 = the default concurrency,
+        -k = k x the number of CUs -- and keep the fastest.  The runs advance
+        the problems, as any run does."""
+        import torch
+        props = torch.cuda.get_device_properties(0)
+        cus = props.multi_processor_count
+        best = None
+        for th, bl in candidates:
+            blocks = cus * (-bl) if bl < 0 else bl
+            t0 = time.perf_counter()
+            for b in budgets:
+                self.run(iter_budget=b, threads=th, blocks=blocks)
+            dt = time.perf_counter() - t0
+            if best is None or dt < best[0]:
+                best = (dt, th, blocks)
+        self.threads, self.blocks = best[1], best[2]
+        return {"threads": self.threads, "blocks": self.blocks, "seconds": best[0]}
+
+    def close(self):
+        if self.h:
+            self.L.lbfgsb_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
