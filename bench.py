@@ -136,6 +136,78 @@ def run_cpu_baseline(n_target: int, n_sample: int, warm: int = 9, timed: int = 1
                         f"scaled x{n_target / n_sample:g} to n={n_target:,}"))
 
 
+def run_batched(count: int, plain_sample: int = 32):
+    """Batched form on the pentadiagonal box-QP workload (count problems, n
+    log-uniform in [1e3, 2e5], tolerance 1e-6): autotuned shape, one
+    persistent launch per dtype timed with the problems resident in HBM, the
+    fp32 minima against the fp64 ones, and the plain form (one
+    lbfgsb_minimize call per problem) timed on a random sample."""
+    import numpy as np
+    import torch
+    from lbfgsb_amd import batch as bt
+    from lbfgsb_amd import problems as pb
+    from lbfgsb_amd import solver
+
+    probs = pb.batched_qps(count)
+    print(f"batched: {count} problems generated", file=sys.stderr, flush=True)
+    eps, max_iter = 1e-6, 1000
+    rng = np.random.default_rng(5)
+    sample = [probs[k] for k in rng.choice(count, size=min(256, count), replace=False)]
+    tuned = bt.autotune(sample, np.float32, eps=eps, max_iter=max_iter)
+    print(f"batched: autotuned {tuned['threads']} threads x {tuned['blocks']} workgroups",
+          file=sys.stderr, flush=True)
+    out = {"workload": f"{count} pentadiagonal box QPs, n log-uniform in [1e3, 2e5], "
+                       f"tolerance 1e-6, m=8",
+           "problems": count, "total_n": int(sum(p.n for p in probs)), "autotune": tuned}
+    fvals = {}
+    for dtype, name in ((np.float32, "f32"), (np.float64, "f64")):
+        b = bt.Batch(dtype, sum(p.n + 64 for p in probs), count)
+        for p in probs:
+            if b.add(p, eps=eps, max_iter=max_iter) < 0:
+                raise RuntimeError("batched problem rejected")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        left = b.run(threads=tuned["threads"], blocks=tuned["blocks"])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        f, it, info, done = b.results()
+        st, en = b.clocks()
+        b.close()
+        lat = (en - st)[:count]
+        s = 8 if dtype == np.float64 else 4
+        bytes_alg = sum(int(it[k]) * b_alg_per_iter(probs[k].n, s) for k in range(count))
+        fvals[name] = f[:count].copy()
+        print(f"batched: {name} {count / wall:.1f} solves/s", file=sys.stderr, flush=True)
+        out[name] = {"solves_per_s": count / wall, "seconds": wall, "unfinished": int(left),
+                     "info_counts": {str(k): int(v) for k, v in
+                                     zip(*np.unique(info[:count], return_counts=True))},
+                     "iterations_mean": float(it[:count].mean()),
+                     "solve_time_p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "solve_time_p95_ms": float(np.percentile(lat, 95) * 1e3),
+                     "solve_time_p99_ms": float(np.percentile(lat, 99) * 1e3),
+                     "alg_bytes_per_s_GB": bytes_alg / wall / 1e9,
+                     "hbm_frac": bytes_alg / wall / 1e9 / HBM_PEAK_GBS}
+    rel = np.abs(fvals["f32"] - fvals["f64"]) / np.maximum(np.abs(fvals["f64"]), 1e-300)
+    out["f32_vs_f64_rel_error"] = {"max": float(rel.max()), "median": float(np.median(rel)),
+                                   "frac_le_1e-6": float(np.mean(rel <= 1e-6))}
+    # the plain form, one call per problem, on a random sample
+    idx = rng.choice(count, size=min(plain_sample, count), replace=False)
+    t_plain = 0.0
+    for k in idx:
+        dp = solver.DeviceProblem.create(probs[k], np.float32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver.minimize(dp, eps=eps, max_iter=max_iter)
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        dp.close()
+    plain_rate = len(idx) / t_plain
+    out["plain_f32_solves_per_s"] = plain_rate
+    out["plain_sample"] = int(len(idx))
+    out["speedup_vs_plain_f32"] = out["f32"]["solves_per_s"] / plain_rate
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +217,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2 ** 22)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-count", type=int, default=4096)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -198,6 +272,8 @@ def main():
                        "iteration_roofline_frac": b_alg_per_iter(n, 8) / t64 / 1e9 / HBM_PEAK_GBS,
                        "kernels_us": {k: round(v[0] * 1e3 / max(v[1], 1), 2)
                                       for k, v in r64["kern"].items()}}
+    if not a.no_batched:
+        out["batched"] = run_batched(a.batched_count)
     if not a.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline(n, min(a.cpu_sample, n))
     print(json.dumps(out), flush=True)

@@ -189,6 +189,10 @@ int lbfgsb_batch_result(lbfgsb_batch* b, int slot, double* f, int* iter, int* in
                         double* residual_g, int* done);
 /* f, iter, info, done for all max_problems slots (host arrays). */
 int lbfgsb_batch_results(lbfgsb_batch* b, double* f, int* iter, int* info, int* done);
+/* Device wall-clock ticks at which each slot's solve started and finished
+ * (0 while unfinished), and the tick rate. */
+int lbfgsb_batch_clocks(lbfgsb_batch* b, long long* start, long long* end,
+                        double* ticks_per_second);
 /* The slot's current iterate into a device array of n elements. */
 int lbfgsb_batch_get_x(lbfgsb_batch* b, int slot, void* out);
 int lbfgsb_batch_remove(lbfgsb_batch* b, int slot);

@@ -509,6 +509,7 @@ __device__ __forceinline__ void solve_one(const BatchProb& bp, const BatchBufs<T
   __syncthreads();
   DevScalars* s = &L.s;
   if (!s->b_started) {
+    if (threadIdx.x == 0) res->clk_start = (long long)wall_clock64();
     b_errclb(P, L);
     if (s->status != ST_DONE) {
       b_active(P, L);
@@ -566,6 +567,7 @@ __device__ __forceinline__ void solve_one(const BatchProb& bp, const BatchBufs<T
     res->info = s->info;
     res->nfgv = s->nfgv;
     res->done = done ? 1 : 0;
+    if (done) res->clk_end = (long long)wall_clock64();
   }
   __syncthreads();
   {

@@ -43,6 +43,9 @@ struct BatchBufs {
 struct BatchResult {
   double f, residual_g, residual_f, residual_x;
   int iter, info, nfgv, done;
+  // device wall clock (wall_clock64 ticks) when the problem was first taken
+  // up and when it finished
+  long long clk_start, clk_end;
 };
 
 // Launch the persistent kernel: `blocks` workgroups of `threads` (256 or

@@ -42,6 +42,7 @@ struct BatchBase {
   virtual int remove(int slot) = 0;
   virtual long long free_elems() const = 0;
   virtual int results_all(double* f, int* iter, int* info, int* done) = 0;
+  virtual int clocks(long long* start, long long* end) = 0;
   bool ok = false;
 };
 
@@ -267,6 +268,14 @@ struct Arena final : BatchBase {
     }
     return 0;
   }
+  int clocks(long long* start, long long* end) override {
+    for (int k = 0; k < maxp; k++) {
+      const bool u = slots[k].used;
+      if (start) start[k] = u ? hres[k].clk_start : 0;
+      if (end) end[k] = u && hres[k].done ? hres[k].clk_end : 0;
+    }
+    return 0;
+  }
   int get_x(int slot, void* out) override {
     if (slot < 0 || slot >= maxp || !slots[slot].used) return -1;
     BCHK(hipMemcpyAsync(out, x + slots[slot].off, (size_t)slots[slot].n * sizeof(T),
@@ -335,6 +344,18 @@ int lbfgsb_batch_result(lbfgsb_batch* b, int slot, double* f, int* iter, int* in
 int lbfgsb_batch_results(lbfgsb_batch* b, double* f, int* iter, int* info, int* done) {
   if (!b) return -1;
   return b->b->results_all(f, iter, info, done);
+}
+
+int lbfgsb_batch_clocks(lbfgsb_batch* b, long long* start, long long* end,
+                        double* ticks_per_second) {
+  if (!b) return -1;
+  if (ticks_per_second) {
+    int dev = 0, khz = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+    *ticks_per_second = 1e3 * (double)khz;
+  }
+  return b->b->clocks(start, end);
 }
 
 int lbfgsb_batch_get_x(lbfgsb_batch* b, int slot, void* out) {

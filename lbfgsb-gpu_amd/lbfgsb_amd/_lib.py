@@ -109,6 +109,8 @@ SIGNATURES = {
                                       C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "lbfgsb_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p]),
+    "lbfgsb_batch_clocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.POINTER(C.c_double)]),
     "lbfgsb_batch_get_x": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "lbfgsb_batch_remove": (C.c_int, [C.c_void_p, C.c_int]),
     "lbfgsb_batch_free_elems": (C.c_longlong, [C.c_void_p]),

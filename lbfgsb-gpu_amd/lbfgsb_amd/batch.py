@@ -8,8 +8,9 @@ slots solved by the persistent per-problem kernel.
 
 `run(iter_budget=k)` advances each unfinished problem by at most k
 iterations and returns how many remain; the serving loop admits new problems
-between such calls.  `tune()` picks the workgroup size and the number of
-resident workgroups by timing short runs on the admitted problems.
+between such calls.  `autotune()` picks the workgroup size (threads per
+problem) and the number of resident workgroups by timing whole solves of a
+sample of the workload.
 """
 from __future__ import annotations
 
@@ -101,6 +102,17 @@ class Batch:
                                     done.ctypes.data_as(C.c_void_p))
         return f, it, info, done
 
+    def clocks(self):
+        """(start, end) device times in seconds of every slot's solve (end 0
+        while unfinished)."""
+        P = self.max_problems
+        st = np.zeros(P, np.int64)
+        en = np.zeros(P, np.int64)
+        rate = C.c_double(0)
+        self.L.lbfgsb_batch_clocks(self.h, st.ctypes.data_as(C.c_void_p),
+                                   en.ctypes.data_as(C.c_void_p), C.byref(rate))
+        return st / rate.value, en / rate.value
+
     def x(self, slot: int):
         t = self.torch
         out = t.empty(self.n[slot], dtype=t.float64 if self.f64 else t.float32, device="cuda")
@@ -115,26 +127,6 @@ class Batch:
     def free_elems(self) -> int:
         return int(self.L.lbfgsb_batch_free_elems(self.h))
 
-    def tune(self, budgets=(2,), candidates=((256, 0), (512, 0), (256, -2))):
-        """Time a short run (iter_budget) of the admitted problems for each
-        (threads, blocks) candidate -- blocks 0 = the default concurrency,
-        -k = k x the number of CUs -- and keep the fastest.  The runs advance
-        the problems, as any run does."""
-        import torch
-        props = torch.cuda.get_device_properties(0)
-        cus = props.multi_processor_count
-        best = None
-        for th, bl in candidates:
-            blocks = cus * (-bl) if bl < 0 else bl
-            t0 = time.perf_counter()
-            for b in budgets:
-                self.run(iter_budget=b, threads=th, blocks=blocks)
-            dt = time.perf_counter() - t0
-            if best is None or dt < best[0]:
-                best = (dt, th, blocks)
-        self.threads, self.blocks = best[1], best[2]
-        return {"threads": self.threads, "blocks": self.blocks, "seconds": best[0]}
-
     def close(self):
         if self.h:
             self.L.lbfgsb_batch_destroy(self.h)
@@ -145,3 +137,30 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# (threads per problem, resident workgroups as a multiple of the CU count)
+TUNE_CANDIDATES = ((256, 2), (256, 1), (512, 1), (256, 3))
+
+
+def autotune(sample, dtype, eps=1e-6, max_iter=1000, candidates=TUNE_CANDIDATES):
+    """Solve `sample` (a list of Problems drawn from the workload) once per
+    candidate shape and return the fastest as {"threads", "blocks", ...}."""
+    torch = _torch()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    cap = sum(p.n + 64 for p in sample)
+    timings = []
+    for th, per_cu in candidates:
+        b = Batch(dtype, cap, len(sample))
+        for p in sample:
+            if b.add(p, eps=eps, max_iter=max_iter) < 0:
+                raise RuntimeError("autotune: problem rejected")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run(threads=th, blocks=cus * per_cu)
+        dt = time.perf_counter() - t0
+        b.close()
+        timings.append((dt, th, cus * per_cu))
+    dt, th, bl = min(timings)
+    return {"threads": th, "blocks": bl, "seconds": dt,
+            "tried": [{"threads": t, "blocks": b_, "seconds": d} for d, t, b_ in timings]}

@@ -190,3 +190,11 @@ def penta_matvec(p: Problem, x: np.ndarray) -> np.ndarray:
     ax[:-2] += p.a2[:-2] * x[2:]
     ax[2:] += p.a2[:-2] * x[:-2]
     return ax
+
+
+def batched_qps(count: int = 4096, lo: float = 1e3, hi: float = 2e5,
+                seed: int = SEED_BASE + 3) -> list:
+    """The batched workload: `count` pentadiagonal box QPs, n log-uniform in
+    [lo, hi] (stream 7 of `seed`), problem k seeded with seed + (k+1)*2^20."""
+    sizes = batch_sizes(count, lo, hi, seed)
+    return [penta_qp(int(n), seed=seed + ((k + 1) << 20)) for k, n in enumerate(sizes)]
