@@ -154,7 +154,8 @@ def run_batched(count: int, plain_sample: int = 32):
     rng = np.random.default_rng(5)
     sample = [probs[k] for k in rng.choice(count, size=min(256, count), replace=False)]
     tuned = bt.autotune(sample, np.float32, eps=eps, max_iter=max_iter)
-    print(f"batched: autotuned {tuned['threads']} threads x {tuned['blocks']} workgroups",
+    print(f"batched: autotuned {tuned['threads']} threads x {tuned['blocks']} workgroups, "
+          f"segment {tuned['segment']}",
           file=sys.stderr, flush=True)
     out = {"workload": f"{count} pentadiagonal box QPs, n log-uniform in [1e3, 2e5], "
                        f"tolerance 1e-6, m=8",
@@ -162,6 +163,7 @@ def run_batched(count: int, plain_sample: int = 32):
     fvals = {}
     for dtype, name in ((np.float32, "f32"), (np.float64, "f64")):
         b = bt.Batch(dtype, sum(p.n + 64 for p in probs), count)
+        b.configure(tuned["segment"], tuned["max_gang"])
         for p in probs:
             if b.add(p, eps=eps, max_iter=max_iter) < 0:
                 raise RuntimeError("batched problem rejected")

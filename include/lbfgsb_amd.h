@@ -174,7 +174,8 @@ int lbfgsb_problem_eval_device(lbfgsb_problem* p, const void* x, void* g, double
  * -3 when no slot or range is free.  _run advances every unfinished slot,
  * largest first, by at most iter_budget iterations (0 = to completion) and
  * returns how many remain unfinished (negative on a device error); threads
- * is 256 or 512 per problem, blocks 0 = the device's concurrent capacity.
+ * is 256 or 512 per workgroup, blocks 0 = the device's concurrent capacity
+ * (a launch never exceeds what can be resident at once).
  * Results follow the summary conventions (info as in lbfgsb_summary). */
 typedef struct lbfgsb_batch lbfgsb_batch;
 lbfgsb_batch* lbfgsb_batch_create(int dtype, long long capacity, int max_problems, int m,
@@ -197,6 +198,16 @@ int lbfgsb_batch_clocks(lbfgsb_batch* b, long long* start, long long* end,
 int lbfgsb_batch_get_x(lbfgsb_batch* b, int slot, void* out);
 int lbfgsb_batch_remove(lbfgsb_batch* b, int slot);
 long long lbfgsb_batch_free_elems(lbfgsb_batch* b);
+/* Gangs: with segment > 0 (at least 4096 elements) a problem of n elements
+ * is solved by min(max_gang, ceil(n / segment)) workgroups together, each
+ * on one segment of its elements, reductions combined in member order; the
+ * result depends on n and this configuration only.  segment 0 (the
+ * default) = one workgroup per problem.  _run returns -4 when a gang is
+ * wider than the launch.  Returns 0, or -1 on a bad argument. */
+int lbfgsb_batch_configure(lbfgsb_batch* b, long long segment, int max_gang);
+/* Workgroups the slot's problem is solved by under the current
+ * configuration (-1 for a free slot). */
+int lbfgsb_batch_gang(lbfgsb_batch* b, int slot);
 
 /* Library version string. */
 const char* lbfgsb_amd_version(void);

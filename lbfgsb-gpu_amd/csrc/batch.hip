@@ -12,6 +12,15 @@
 // The objective is one of the built-in kinds, evaluated in the workgroup
 // with the value kept on the device.  No host round trip happens inside a
 // launch.
+//
+// A large problem is solved by a gang of S workgroups, each taking one
+// 64-aligned segment of its elements.  Every member runs the same control
+// logic on the same reduced values (so every member takes the same
+// decisions): a pass's block-level results are exchanged through HBM at a
+// gang barrier and summed in member order, which makes the outcome a
+// function of the problem and S only.  The breakpoint rounds of the Cauchy
+// search are sorted and scanned by the gang together (gang_round).  Only
+// member 0 writes the problem's results and control state back.
 #include "batch.hpp"
 #include "common.hpp"
 #include "control.hpp"
@@ -33,6 +42,7 @@ struct BatchLds {
   double f;                  // objective value of the last evaluation
   double rv[4];              // [sbgnrm, bkmin, amin, smin] of the last passes
   long long ri[2];           // [ibkmin, ibd]
+  int gflag;                 // gang error flag of the last barrier
   union {
     RoundLds round;
     struct {
@@ -46,6 +56,7 @@ struct BatchLds {
 template <typename T>
 struct Prob {
   long long n;
+  long long lo, hi;  // this workgroup's segment [lo, hi) of the elements
   Pair<T> X, G;
   const T* l;
   const T* u;
@@ -111,25 +122,144 @@ __device__ __forceinline__ void bargmin(double& v, long long& i, double* sh, lon
 }
 
 // ---------------------------------------------------------------------------
+// gangs
+// ON = false is the one-workgroup form: every gang step folds away at
+// compile time.
+template <bool ON>
+struct Gang {
+  int S;            // members
+  int j;            // this workgroup's member index
+  long long item0;  // queue entry of member 0
+  GangCtl* ctl;
+  double* part;     // exchange areas, 2 x kGangPart doubles per queue entry
+  unsigned epoch;   // barriers passed in this launch
+  __device__ __forceinline__ bool on() const { return ON && S > 1; }
+};
+
+// Barrier of the gang's workgroups (release of this workgroup's global
+// writes, acquire of the others').  Returns false when the gang is in error:
+// a member waited longer than kGangWaitTicks, here or before.
+//
+// The L2 of an XCD is not coherent with the others': every wave releases
+// its own stores (write-back of the L2 after its stores completed), thread 0
+// arrives and polls the counter with relaxed agent-scope loads (which read
+// the coherent copy without invalidating the L2 under every workgroup of
+// the XCD at each poll), and one wave invalidates the CU's and the XCD's
+// caches once the gang has arrived.
+__device__ __forceinline__ bool gang_sync(Gang<true>& G, int* flag) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(&G.ctl->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned target = (G.epoch + 1u) * (unsigned)G.S;
+    const long long t0 = (long long)wall_clock64();
+    int err = 0;
+    while (__hip_atomic_load(&G.ctl->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+           target) {
+      err = __hip_atomic_load(&G.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (err) break;
+      if ((long long)wall_clock64() - t0 > kGangWaitTicks) {
+        __hip_atomic_store(&G.ctl->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        err = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+    }
+    if (!err) err = __hip_atomic_load(&G.ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *flag = err;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  G.epoch++;
+  const bool ok = *flag == 0;
+  __syncthreads();
+  return ok;
+}
+
+// Combine a pass's block-level results over the gang: red[0 .. nsum) are
+// summed in member order, rv[k] is combined by mode[k] (0 none, 1 max,
+// 2 min, 3 argmin with index ri[0], 4 argmin with index ri[1]), and the
+// Part-4 flag of entering/leaving variables is or-ed.  A gang error ends
+// the problem (info -2).
+template <bool ON>
+__device__ __forceinline__ void gang_combine(Gang<ON>& G, BatchLds& L, int nsum, int m0, int m1,
+                                             int m2, int m3, int* flag) {
+  if constexpr (!ON) {
+    return;
+  } else {
+  if (!G.on()) return;
+  const unsigned par = G.epoch & 1u;
+  double* mine = G.part + ((G.item0 + G.j) * 2 + par) * kGangPart;
+  for (int q = threadIdx.x; q < nsum; q += blockDim.x) mine[q] = L.red[q];
+  if (threadIdx.x < 4) mine[kGangPart - 8 + threadIdx.x] = L.rv[threadIdx.x];
+  if (threadIdx.x < 2) mine[kGangPart - 4 + threadIdx.x] = __longlong_as_double(L.ri[threadIdx.x]);
+  if (threadIdx.x == 0) mine[kGangPart - 2] = (double)L.s.p4_flagged;
+  if (!gang_sync(G, flag)) {
+    if (threadIdx.x == 0) set_done(&L.s, -2);
+    __syncthreads();
+    return;
+  }
+  const double* base = G.part + (G.item0 * 2 + par) * kGangPart;
+  const long long stride = 2LL * kGangPart;
+  for (int q = threadIdx.x; q < nsum; q += blockDim.x) {
+    double t = 0.0;
+    for (int k = 0; k < G.S; k++) t += base[k * stride + q];
+    L.red[q] = t;
+  }
+  if (threadIdx.x == 0) {
+    const int mode[4] = {m0, m1, m2, m3};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (mode[c] == 0) continue;
+      double v = base[kGangPart - 8 + c];
+      const int ix = mode[c] - 3;
+      long long id = mode[c] >= 3 ? __double_as_longlong(base[kGangPart - 4 + ix]) : 0;
+      for (int k = 1; k < G.S; k++) {
+        const double w = base[k * stride + kGangPart - 8 + c];
+        if (mode[c] == 1) {
+          v = fmax(v, w);
+        } else if (mode[c] == 2) {
+          v = fmin(v, w);
+        } else {
+          const long long wi = __double_as_longlong(base[k * stride + kGangPart - 4 + ix]);
+          if (w < v || (w == v && wi < id)) {
+            v = w;
+            id = wi;
+          }
+        }
+      }
+      L.rv[c] = v;
+      if (mode[c] >= 3) L.ri[ix] = id;
+    }
+    int fl = 0;
+    for (int k = 0; k < G.S; k++) fl |= base[k * stride + kGangPart - 2] != 0.0;
+    L.s.p4_flagged = fl;
+  }
+  __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // set-up passes (lbfgsbcpu.cpp:258-278)
-template <typename T>
-__device__ __forceinline__ void b_errclb(const Prob<T>& P, BatchLds& L) {
+template <typename T, bool GO>
+__device__ __forceinline__ void b_errclb(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   double v[2] = {0.0, 0.0};
-  for (long long i = threadIdx.x; i < P.n; i += blockDim.x) {
+  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     const int nb = P.nbd[i];
     if (nb < 0 || nb > 3) v[0] += 1.0;
     if (nb == 2 && P.l[i] > P.u[i]) v[1] += 1.0;
   }
   bsum<2>(v, L.u.r.sh, L.red);
+  gang_combine(G, L, 2, 0, 0, 0, 0, &L.gflag);
   if (threadIdx.x == 0) logic_setup_errclb(&L.s, L.red[0], L.red[1]);
   __syncthreads();
 }
 
-template <typename T>
-__device__ __forceinline__ void b_active(const Prob<T>& P, BatchLds& L) {
+template <typename T, bool GO>
+__device__ __forceinline__ void b_active(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   T* x = P.X[0];
   double v[3] = {0.0, 0.0, 0.0};
-  for (long long i = threadIdx.x; i < P.n; i += blockDim.x) {
+  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     const int nb = P.nbd[i];
     const T xi = x[i], li = P.l[i], ui = P.u[i];
     if (nb > 0) {
@@ -156,6 +286,7 @@ __device__ __forceinline__ void b_active(const Prob<T>& P, BatchLds& L) {
     P.iwhere[i] = iw_make(iw, true, true);
   }
   bsum<3>(v, L.u.r.sh, L.red);
+  gang_combine(G, L, 3, 0, 0, 0, 0, &L.gflag);
   if (threadIdx.x == 0) {
     const double r[3] = {L.red[0], L.red[1], L.red[2]};
     logic_setup_active(&L.s, r);
@@ -165,39 +296,56 @@ __device__ __forceinline__ void b_active(const Prob<T>& P, BatchLds& L) {
 
 // Objective at X[which] into G[which]; L.f = f rounded to the bulk type, as
 // a callback returning `real& f` would hand it over.
-template <typename T>
-__device__ __forceinline__ void b_objective(const Prob<T>& P, BatchLds& L, int which) {
+// In a gang, member j evaluates the terms of its segment (ROSEN: the pairs
+// inside it, segments being even-aligned) after a barrier, since the
+// other kinds read neighbours that other members have just written.
+template <typename T, bool GO>
+__device__ __forceinline__ void b_objective(const Prob<T>& P, BatchLds& L, int which,
+                                            Gang<GO>& G) {
   const T* x = P.X[which];
   T* g = P.G[which];
+  if constexpr (GO) {
+    if (G.on() && P.obj.kind != OBJK_ROSEN && !gang_sync(G, &L.gflag)) {
+      if (threadIdx.x == 0) set_done(&L.s, -2);
+      __syncthreads();
+    }
+  }
   double acc[1] = {0.0};
-  const long long cnt = obj_terms(P.obj, P.n);
-  for (long long k = threadIdx.x; k < cnt; k += blockDim.x) acc[0] += obj_term(P.obj, P.n, x, g, k);
-  bsum<1>(acc, L.u.r.sh, &L.f);
-  if (threadIdx.x == 0) L.f = (double)(T)L.f;
+  const bool pairs = P.obj.kind == OBJK_ROSEN;
+  const long long k0 = pairs ? P.lo / 2 : P.lo;
+  const long long k1 = pairs ? P.hi / 2 : P.hi;
+  for (long long k = k0 + threadIdx.x; k < k1; k += blockDim.x)
+    acc[0] += obj_term(P.obj, P.n, x, g, k);
+  bsum<1>(acc, L.u.r.sh, L.red);
+  gang_combine(G, L, 1, 0, 0, 0, 0, &L.gflag);
+  if (threadIdx.x == 0) L.f = (double)(T)L.red[0];
   __syncthreads();
 }
 
-template <typename T>
-__device__ __forceinline__ void b_projgr(const Prob<T>& P, BatchLds& L) {
+template <typename T, bool GO>
+__device__ __forceinline__ void b_projgr(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   const T* x = P.X[0];
   const T* g = P.G[0];
   double m = 0.0;
-  for (long long i = threadIdx.x; i < P.n; i += blockDim.x)
+  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
     m = fmax(m, (double)fabs(projected_gi(x[i], g[i], P.l[i], P.u[i], P.nbd[i])));
   m = bmax(m, L.u.r.sh);
-  if (threadIdx.x == 0) logic_setup_projgr(&L.s, m, L.f);
+  if (threadIdx.x == 0) L.rv[0] = m;
+  __syncthreads();
+  gang_combine(G, L, 0, 1, 0, 0, 0, &L.gflag);
+  if (threadIdx.x == 0) logic_setup_projgr(&L.s, L.rv[0], L.f);
   __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
 // iteration passes
-template <typename T, bool FUSED>
-__device__ __forceinline__ void b_pass_a(const Prob<T>& P, BatchLds& L) {
+template <typename T, bool FUSED, bool GO>
+__device__ __forceinline__ void b_pass_a(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt) return;
   using Op = PassAOp<T, FUSED>;
   Op op;
   op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.W, P.d);
-  for (long long i = threadIdx.x; i < P.n; i += blockDim.x) {
+  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     typename Op::In e;
     op.load(e, i);
     op.keep(e);
@@ -215,20 +363,21 @@ __device__ __forceinline__ void b_pass_a(const Prob<T>& P, BatchLds& L) {
     L.ri[0] = ibk;
   }
   __syncthreads();
+  gang_combine(G, L, PA_COUNT, 1, 3, 0, 0, &L.gflag);
 }
 
-template <typename T>
-__device__ __forceinline__ void b_p4(const Prob<T>& P, BatchLds& L) {
+template <typename T, bool GO>
+__device__ __forceinline__ void b_p4(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt) return;
   P4Op<T> op;
   op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.W, P.z, P.r);
-  const long long n = P.n;
-  for (long long i0 = 0; i0 < n; i0 += blockDim.x) {
+  const long long hi = P.hi;
+  for (long long i0 = P.lo; i0 < hi; i0 += blockDim.x) {
     const long long i = i0 + threadIdx.x;
     typename P4Op<T>::In e;
-    op.load(e, i < n ? i : n - 1);
+    op.load(e, i < hi ? i : hi - 1);
     op.keep(e);
-    op.corr_step(i, i < n ? op.element(i, e) : 0);
+    op.corr_step(i, i < hi ? op.element(i, e) : 0);
   }
   bsum<P4_CORR>(op.acc, L.u.r.sh, L.red);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -246,14 +395,15 @@ __device__ __forceinline__ void b_p4(const Prob<T>& P, BatchLds& L) {
     L.red[P4_CORR + t] = s;
   }
   __syncthreads();
+  gang_combine(G, L, P4_COUNT, 0, 0, 0, 0, &L.gflag);
 }
 
-template <typename T>
-__device__ __forceinline__ void b_p5a(const Prob<T>& P, BatchLds& L) {
+template <typename T, bool GO>
+__device__ __forceinline__ void b_p5a(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt || L.s.skip_subsm) return;
   P5aOp<T> op;
   op.init(&L.s, P.l, P.u, P.nbd, P.iwhere, P.W, P.z, P.r, P.dsub);
-  for (long long i = threadIdx.x; i < P.n; i += blockDim.x) {
+  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     typename P5aOp<T>::In e;
     op.load(e, i);
     op.keep(e);
@@ -267,14 +417,15 @@ __device__ __forceinline__ void b_p5a(const Prob<T>& P, BatchLds& L) {
     L.ri[1] = ai;
   }
   __syncthreads();
+  gang_combine(G, L, 0, 0, 0, 4, 0, &L.gflag);
 }
 
-template <typename T>
-__device__ __forceinline__ void b_p5b(const Prob<T>& P, BatchLds& L) {
+template <typename T, bool GO>
+__device__ __forceinline__ void b_p5b(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt) return;
   P5bOp<T> op;
   op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.z, P.dsub, P.d);
-  for (long long i = threadIdx.x; i < P.n; i += blockDim.x) {
+  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     typename P5bOp<T>::In e;
     op.load(e, i);
     op.keep(e);
@@ -284,6 +435,7 @@ __device__ __forceinline__ void b_p5b(const Prob<T>& P, BatchLds& L) {
   const double smin = bmin(op.smin, L.u.r.sh);
   if (threadIdx.x == 0) L.rv[3] = smin;
   __syncthreads();
+  gang_combine(G, L, P5_COUNT, 0, 0, 0, 2, &L.gflag);
 }
 
 // Trial point x_b = stp d + x_a, or the subspace point when stp == 1
@@ -295,24 +447,27 @@ __device__ __forceinline__ void b_trial(const Prob<T>& P, BatchLds& L) {
   T* xb = P.X[L.s.trial_dst];
   const double stp = L.s.stp;
   const bool one = stp == 1.0;
-  for (long long i = threadIdx.x; i < P.n; i += blockDim.x)
+  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
     xb[i] = one ? P.z[i] : (T)(stp * (double)P.d[i] + (double)xa[i]);
   __syncthreads();
 }
 
 // g_b . d after an evaluation (lbfgsbcpu.cpp:1312).
-template <typename T>
-__device__ __forceinline__ void b_gd(const Prob<T>& P, BatchLds& L) {
+template <typename T, bool GO>
+__device__ __forceinline__ void b_gd(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   const T* gb = P.G[(L.s.phase & 1) ^ 1];
   double acc[1] = {0.0};
-  for (long long i = threadIdx.x; i < P.n; i += blockDim.x)
+  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
     acc[0] += (double)gb[i] * (double)P.d[i];
   bsum<1>(acc, L.u.r.sh, L.red);
+  gang_combine(G, L, 1, 0, 0, 0, 0, &L.gflag);
 }
 
-// Breakpoint candidates with t in [tau_lo, tau_hi) into P.ca.
-template <typename T>
-__device__ __forceinline__ void b_select(const Prob<T>& P, BatchLds& L) {
+// Breakpoint candidates with t in [tau_lo, tau_hi) of this workgroup's
+// segment into P.ca (slots from the LDS count, or in a gang from the gang's
+// HBM count).
+template <typename T, bool GO>
+__device__ __forceinline__ void b_select(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   DevScalars* s = &L.s;
   const int cur = s->phase & 1;
   const T* __restrict__ x = P.X[cur];
@@ -321,18 +476,24 @@ __device__ __forceinline__ void b_select(const Prob<T>& P, BatchLds& L) {
   const int lane = threadIdx.x & 63;
   __syncthreads();
   if (threadIdx.x == 0) s->gcp_need_select = 0;
-  for (long long i0 = 0; i0 < P.n; i0 += blockDim.x) {
+  for (long long i0 = P.lo; i0 < P.hi; i0 += blockDim.x) {
     const long long i = i0 + threadIdx.x;
     bool take = false;
     T tb = 0;
-    if (i < P.n && breakpoint_of(x[i], g[i], P.l[i], P.u[i], P.nbd[i], iw_get(P.iwhere[i]), tb)) {
+    if (i < P.hi && breakpoint_of(x[i], g[i], P.l[i], P.u[i], P.nbd[i], iw_get(P.iwhere[i]), tb)) {
       const double t = (double)tb;
       take = t >= lo && t < hi;
     }
     const unsigned long long mask = __ballot(take);
     if (mask) {
       unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(&s->cand_count, (unsigned long long)__popcll(mask));
+      if (lane == 0) {
+        const unsigned long long cnt = (unsigned long long)__popcll(mask);
+        if constexpr (GO)
+          base = atomicAdd(&G.ctl->cand, cnt);
+        else
+          base = atomicAdd(&s->cand_count, cnt);
+      }
       base = __shfl(base, 0, kWave);
       const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
       if (take && slot < (unsigned long long)P.n) {
@@ -347,36 +508,249 @@ __device__ __forceinline__ void b_select(const Prob<T>& P, BatchLds& L) {
   __syncthreads();
 }
 
-// The breakpoint rounds of the Cauchy search, all inside the workgroup.
+// What the breakpoint rounds read of a problem.
 template <typename T>
-__device__ __forceinline__ void b_gcp(const Prob<T>& P, BatchLds& L) {
+struct RoundArgs {
+  long long n;
+  const T* g;
+  const T* W;
+  Cand* ca;
+  Cand* cb;
+};
+
+template <typename T>
+__device__ __forceinline__ RoundArgs<T> round_args(const Prob<T>& P, const DevScalars* s) {
+  return RoundArgs<T>{P.n, P.G[s->phase & 1], P.W, P.ca, P.cb};
+}
+
+// One breakpoint round over the candidates in ca (count in the control
+// state): sort in LDS or HBM, then the closed-form scan.
+template <typename T>
+__device__ __forceinline__ void b_round(RoundArgs<T> P, BatchLds& L) {
+  DevScalars* s = &L.s;
+  const long long cnt = (long long)s->cand_count;
+  const long long K = cnt < P.n ? cnt : P.n;
+  stage_gcp_const(s, L.u.round.gc);
+  const T* g = P.g;
+  if (K <= kSmallRound) {
+    block_sort_small(P.ca, K, L.u.round.sc);
+    block_round_scan(s, L.u.round, L.u.round.sc, K, g, P.W);
+  } else {
+    const Cand* sorted = block_sort_large(P.ca, P.cb, K, L.u.round.sc);
+    block_round_scan(s, L.u.round, sorted, K, g, P.W);
+  }
+  if (s->gcp_pending) {
+    // every candidate of the t-range passed: the next range
+    if (threadIdx.x == 0) logic_resume(s, 0);
+    __syncthreads();
+  }
+}
+
+// Copy of the control state between LDS and HBM.
+__device__ __forceinline__ void ds_copy(unsigned long long* dst, const unsigned long long* src) {
+  for (int i = threadIdx.x; i < kBDsWords; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+}
+
+// Boundary of chunk c (0 <= c <= S) when K positions are split into S
+// chunks as evenly as possible.
+__device__ __forceinline__ long long chunk_at(long long K, int S, int c) {
+  return K * (long long)c / S;
+}
+
+// The chunk that holds position o (0 <= o < K).
+__device__ __forceinline__ int chunk_of(long long K, int S, long long o) {
+  int c = (int)((o * S) / K);
+  while (c + 1 < S && chunk_at(K, S, c + 1) <= o) c++;
+  while (c > 0 && chunk_at(K, S, c) > o) c--;
+  return c;
+}
+
+// Sort a[0 .. K) over the gang: member j sorts chunk j in place (LDS, or
+// LDS runs merged through b), then ceil(log2 S) merge-path passes between
+// a and b, each merging pairs of runs of 2^p chunks with the outputs split
+// evenly among the members.  Returns the buffer that holds the result.
+__device__ __forceinline__ Cand* gang_sort(Cand* a, Cand* b, long long K, Cand* sc,
+                                           Gang<true>& G, int* flag, bool& ok) {
+  const int S = G.S;
+  const long long lo = chunk_at(K, S, G.j), hi = chunk_at(K, S, G.j + 1), len = hi - lo;
+  if (len > 0) {
+    if (len <= kSmallRound) {
+      block_sort_small(a + lo, len, sc);
+      for (long long i = threadIdx.x; i < len; i += blockDim.x) a[lo + i] = sc[i];
+    } else {
+      Cand* r = block_sort_large(a + lo, b + lo, len, sc);
+      if (r != a + lo)
+        for (long long i = threadIdx.x; i < len; i += blockDim.x) a[lo + i] = r[i];
+    }
+  }
+  ok = gang_sync(G, flag);
+  Cand* src = a;
+  Cand* dst = b;
+  for (int w = 1; ok && w < S; w <<= 1) {
+    const long long per = (len + blockDim.x - 1) / blockDim.x;
+    const long long o0 = lo + (long long)threadIdx.x * per;
+    const long long oe = o0 + per < hi ? o0 + per : hi;
+    long long o = o0;
+    while (o < oe) {
+      // output position o lies in the pair of runs starting at chunk cb
+      const int cb = (chunk_of(K, S, o) / (2 * w)) * 2 * w;
+      const long long a0 = chunk_at(K, S, cb);
+      const long long a1 = chunk_at(K, S, cb + w < S ? cb + w : S);
+      const long long b0 = a1;
+      const long long b1 = chunk_at(K, S, cb + 2 * w < S ? cb + 2 * w : S);
+      const long long la = a1 - a0, lb = b1 - b0;
+      const long long diag = o - a0;
+      long long l = diag - lb > 0 ? diag - lb : 0, h = diag < la ? diag : la;
+      while (l < h) {
+        const long long mid = (l + h) >> 1;
+        if (cand_less(src[a0 + mid], src[b0 + diag - 1 - mid]))
+          l = mid + 1;
+        else
+          h = mid;
+      }
+      long long i = l, j = diag - l;
+      const long long oend = oe < b1 ? oe : b1;
+      for (; o < oend; o++) {
+        bool takeA;
+        if (i >= la)
+          takeA = false;
+        else if (j >= lb)
+          takeA = true;
+        else
+          takeA = !cand_less(src[b0 + j], src[a0 + i]);
+        dst[o] = takeA ? src[a0 + i++] : src[b0 + j++];
+      }
+    }
+    ok = gang_sync(G, flag);
+    Cand* t = src;
+    src = dst;
+    dst = t;
+  }
+  return src;
+}
+
+// One breakpoint round over the gang.  The candidates of the t-range (count
+// in the gang's HBM counter) are sorted over the gang; member j scans the
+// j-th chunk of the sorted sequence from the state before it (the round's
+// carry plus the chunk totals of members 0 .. j-1, summed in member order);
+// the first stop of the gang, with its state, concludes the round in every
+// member alike.  Returns the gang's barrier count after the round.
+template <typename T>
+__device__ __forceinline__ unsigned gang_round(RoundArgs<T> P, BatchLds& L, Gang<true> G) {
+  DevScalars* s = &L.s;
+  RoundLds& R = L.u.round;
+  const int S = G.S;
+  const long long cnt =
+      (long long)__hip_atomic_load(&G.ctl->cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const long long K = cnt < P.n ? cnt : P.n;
+  stage_gcp_const(s, R.gc);
+  const T* g = P.g;
+  bool ok = true;
+  const Cand* sorted = gang_sort(P.ca, P.cb, K, R.sc, G, &L.gflag, ok);
+  // every member has read the count: it starts from zero in the next round
+  // (no member adds to it before the barriers below)
+  if (ok && G.j == 0 && threadIdx.x == 0)
+    __hip_atomic_store(&G.ctl->cand, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const long long lo = chunk_at(K, S, G.j), hi = chunk_at(K, S, G.j + 1);
+  const long long stride = 2LL * kGangPart;
+  // chunk totals
+  if (ok) {
+    if (threadIdx.x < kNV) R.scarry[threadIdx.x] = 0.0;
+    block_scan_range(s, R, sorted, lo, hi, K, false, g, P.W);
+    const unsigned par = G.epoch & 1u;
+    double* mine = G.part + ((G.item0 + G.j) * 2 + par) * kGangPart;
+    if (threadIdx.x < kNV) mine[threadIdx.x] = R.scarry[threadIdx.x];
+    ok = gang_sync(G, &L.gflag);
+    const double* base = G.part + (G.item0 * 2 + par) * kGangPart;
+    if (ok && threadIdx.x < kNV) {
+      double c = s->gcp_carry[threadIdx.x];
+      for (int k = 0; k < G.j; k++) c += base[k * stride + threadIdx.x];
+      R.scarry[threadIdx.x] = c;
+      // the inclusive total of the round, summed alike in every member
+      double t = s->gcp_carry[threadIdx.x];
+      for (int k = 0; k < S; k++) t += base[k * stride + threadIdx.x];
+      R.swave[threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+  // the scan of this chunk and the exchange of the stops
+  if (ok) {
+    const int q = threadIdx.x;
+    double total = q < kNV ? R.swave[q] : 0.0;
+    block_scan_range(s, R, sorted, lo, hi, K, true, g, P.W);
+    const unsigned par = G.epoch & 1u;
+    double* mine = G.part + ((G.item0 + G.j) * 2 + par) * kGangPart;
+    if (q == 0) mine[0] = __longlong_as_double(R.sstop);
+    if (q < kNV) {
+      mine[1 + q] = R.sstop_state[q];
+      mine[1 + kNV + q] = R.slast[q];
+    }
+    ok = gang_sync(G, &L.gflag);
+    if (ok) {
+      const double* base = G.part + (G.item0 * 2 + par) * kGangPart;
+      long long stop = 0x7fffffffffffffffLL;
+      int owner = -1;
+      for (int k = 0; k < S; k++) {
+        const long long sk = __double_as_longlong(base[k * stride]);
+        if (sk < stop) {
+          stop = sk;
+          owner = k;
+        }
+      }
+      const int last_owner = K > 0 ? chunk_of(K, S, K - 1) : 0;
+      if (q < kNV) {
+        R.sstop_state[q] = owner >= 0 ? base[owner * stride + 1 + q] : total;
+        R.slast[q] = K > 0 ? base[last_owner * stride + 1 + kNV + q] : s->gcp_carry[q];
+      }
+      if (q == 0) R.sstop = stop;
+      __syncthreads();
+      if (q == 0) {
+        const bool stopped = R.sstop != 0x7fffffffffffffffLL;
+        round_conclude(s, R.gc, sorted, K, stopped ? R.sstop : -1, R.sstop_state, R.slast);
+      }
+      __syncthreads();
+    }
+  }
+  if (!ok) {
+    if (threadIdx.x == 0) set_done(s, -2);
+    __syncthreads();
+    return G.epoch;
+  }
+  if (s->gcp_pending) {
+    // every candidate of the t-range passed: the next range
+    if (threadIdx.x == 0) logic_resume(s, 0);
+    __syncthreads();
+  }
+  return G.epoch;
+}
+
+// The breakpoint rounds of the Cauchy search.  A gang selects over its
+// segments, then sorts and scans the round together (gang_round).
+template <typename T, bool GO>
+__device__ __forceinline__ void b_gcp(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   DevScalars* s = &L.s;
   if (!s->gcp_pending || s->halt) return;
   for (int guard = 0; guard < 64 && s->gcp_pending; guard++) {
-    b_select(P, L);
-    const long long cnt = (long long)s->cand_count;
-    const long long K = cnt < P.n ? cnt : P.n;
-    stage_gcp_const(s, L.u.round.gc);
-    const T* g = P.G[s->phase & 1];
-    if (K <= kSmallRound) {
-      block_sort_small(P.ca, K, L.u.round.sc);
-      block_round_scan(s, L.u.round, L.u.round.sc, K, g, P.W);
+    b_select(P, L, G);
+    if constexpr (!GO) {
+      b_round(round_args(P, s), L);
     } else {
-      const Cand* sorted = block_sort_large(P.ca, P.cb, K, L.u.round.sc);
-      block_round_scan(s, L.u.round, sorted, K, g, P.W);
-    }
-    if (s->gcp_pending) {
-      // every candidate of the t-range passed: the next range
-      if (threadIdx.x == 0) logic_resume(s, 0);
-      __syncthreads();
+      if (!gang_sync(G, &L.gflag)) {
+        if (threadIdx.x == 0) set_done(s, -2);
+        __syncthreads();
+        return;
+      }
+      G.epoch = gang_round(round_args(P, s), L, G);
     }
   }
 }
 
 // One step of the chain (the block form of the plain engine's kernels,
 // engine.cpp: launch_step).
-template <typename T>
-__device__ __forceinline__ void b_step(int step, double f, const Prob<T>& P, BatchLds& L) {
+template <typename T, bool GO>
+__device__ __forceinline__ void b_step(int step, double f, const Prob<T>& P, BatchLds& L,
+                                       Gang<GO>& G) {
   DevScalars* s = &L.s;
   const bool t0 = threadIdx.x == 0;
   switch (step) {
@@ -387,10 +761,10 @@ __device__ __forceinline__ void b_step(int step, double f, const Prob<T>& P, Bat
       if (t0) logic_resume(s, 0);
       break;
     case K_PASSA_SA:
-      b_pass_a<T, false>(P, L);
+      b_pass_a<T, false, GO>(P, L, G);
       break;
     case K_PASSA_FUSED:
-      b_pass_a<T, true>(P, L);
+      b_pass_a<T, true, GO>(P, L, G);
       break;
     case K_UPD_REDUCE:
       if (s->halt) {
@@ -421,24 +795,24 @@ __device__ __forceinline__ void b_step(int step, double f, const Prob<T>& P, Bat
       build_middle(s);
       break;
     case K_GCP:
-      b_gcp(P, L);
+      b_gcp(P, L, G);
       break;
     case K_P4:
-      b_p4(P, L);
+      b_p4(P, L, G);
       break;
     case K_FORMK:
       if (s->halt) break;
       if (t0) logic_formk_subsm(s, L.red, s->p4_flagged != 0);
       break;
     case K_P5A:
-      b_p5a(P, L);
+      b_p5a(P, L, G);
       break;
     case K_ALPHA:
       if (s->halt || s->skip_subsm) break;
       if (t0) logic_alpha(s, L.rv[2], L.ri[1]);
       break;
     case K_P5B:
-      b_p5b(P, L);
+      b_p5b(P, L, G);
       break;
     case K_LSINIT:
       if (s->halt) break;
@@ -448,7 +822,7 @@ __device__ __forceinline__ void b_step(int step, double f, const Prob<T>& P, Bat
       b_trial(P, L);
       break;
     case K_GD:
-      b_gd(P, L);
+      b_gd(P, L, G);
       break;
     case K_LSSTEP:
       if (t0) logic_ls_step(s, f, L.red[0]);
@@ -469,23 +843,38 @@ __constant__ int kProgSteps[3][16] = {
     {K_RESUME_RESET, K_PASSA_FUSED, K_STOP, K_UPD, K_GCP, K_P4, K_FORMK, K_P5A, K_ALPHA, K_P5B,
      K_LSINIT, K_TRIAL, -1, -1, -1, -1}};
 
-template <typename T>
-__device__ __forceinline__ void b_program(int prog, double f, const Prob<T>& P, BatchLds& L) {
+template <typename T, bool GO>
+__device__ __forceinline__ void b_program(int prog, double f, const Prob<T>& P, BatchLds& L,
+                                          Gang<GO>& G) {
   if (prog < PROG_ITER_START || prog > PROG_AFTER_ACCEPT) return;
   for (int k = 0; k < 16; k++) {
     const int step = kProgSteps[prog][k];
     if (step < 0) break;
-    b_step(step, f, P, L);
+    b_step(step, f, P, L, G);
   }
 }
 
-// One problem: set-up (first launch only) and the chain loop of the host
-// driver (capi.cpp: drive), with the objective evaluated in the workgroup.
-template <typename T>
-__device__ __forceinline__ void solve_one(const BatchProb& bp, const BatchBufs<T>& B, DevScalars* dsg,
-                          BatchResult* res, int iter_budget, BatchLds& L) {
+// One problem (or one gang member's share of it): set-up (first launch
+// only) and the chain loop of the host driver (capi.cpp: drive), with the
+// objective evaluated in the workgroup.  Two variants: the one-workgroup
+// form (GO = false) has no gang step in it.  Both are inlined into the
+// kernel, where the arrays are known to be global memory.
+template <typename T, bool GO>
+__device__ __forceinline__ void solve_one(const BatchProb& bp, const BatchBufs<T>& B,
+                                          DevScalars* dsg, BatchResult* res, int iter_budget,
+                                          BatchLds& L, Gang<GO>& G) {
   Prob<T> P;
   P.n = bp.n;
+  if constexpr (GO) {
+    // member j's segment: chunks of 64 elements split as evenly as possible
+    const long long chunks = (bp.n + 63) / 64;
+    P.lo = ((long long)G.j * chunks / G.S) * 64;
+    const long long hi = ((long long)(G.j + 1) * chunks / G.S) * 64;
+    P.hi = hi < bp.n ? hi : bp.n;
+  } else {
+    P.lo = 0;
+    P.hi = bp.n;
+  }
   P.X = Pair<T>{B.x + bp.off, B.xb + bp.off};
   P.G = Pair<T>{B.g + bp.off, B.gb + bp.off};
   P.l = B.l + bp.off;
@@ -501,20 +890,16 @@ __device__ __forceinline__ void solve_one(const BatchProb& bp, const BatchBufs<T
   P.cb = B.cb + bp.off;
   P.obj = bp.obj;
 
-  {
-    const unsigned long long* src = (const unsigned long long*)dsg;
-    unsigned long long* dst = (unsigned long long*)&L.s;
-    for (int i = threadIdx.x; i < kBDsWords; i += blockDim.x) dst[i] = src[i];
-  }
-  __syncthreads();
+  const bool lead = !GO || G.j == 0;
+  ds_copy((unsigned long long*)&L.s, (const unsigned long long*)dsg);
   DevScalars* s = &L.s;
   if (!s->b_started) {
-    if (threadIdx.x == 0) res->clk_start = (long long)wall_clock64();
-    b_errclb(P, L);
+    if (threadIdx.x == 0 && lead) res->clk_start = (long long)wall_clock64();
+    b_errclb(P, L, G);
     if (s->status != ST_DONE) {
-      b_active(P, L);
-      b_objective(P, L, 0);
-      b_projgr(P, L);
+      b_active(P, L, G);
+      b_objective(P, L, 0, G);
+      b_projgr(P, L, G);
     }
     if (threadIdx.x == 0) {
       s->b_started = 1;
@@ -537,10 +922,10 @@ __device__ __forceinline__ void solve_one(const BatchProb& bp, const BatchBufs<T
     const int prog = s->b_next_prog;
     const double fv = s->b_next_f;
     __syncthreads();
-    b_program(prog, fv, P, L);
+    b_program(prog, fv, P, L, G);
     const int st = s->status;
     if (st == ST_NEED_EVAL) {
-      b_objective(P, L, s->trial_dst);
+      b_objective(P, L, s->trial_dst, G);
       if (threadIdx.x == 0) {
         s->b_next_f = L.f;
         s->b_next_prog = PROG_AFTER_EVAL;
@@ -556,7 +941,11 @@ __device__ __forceinline__ void solve_one(const BatchProb& bp, const BatchBufs<T
   const bool done = s->status == ST_DONE;
   if (done && (s->phase & 1)) {
     // leave the accepted iterate in the caller's buffer
-    for (long long i = threadIdx.x; i < P.n; i += blockDim.x) P.X[0][i] = P.X[1][i];
+    for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) P.X[0][i] = P.X[1][i];
+  }
+  if (!lead) {
+    __syncthreads();
+    return;
   }
   if (threadIdx.x == 0) {
     res->f = s->f;
@@ -570,55 +959,107 @@ __device__ __forceinline__ void solve_one(const BatchProb& bp, const BatchBufs<T
     if (done) res->clk_end = (long long)wall_clock64();
   }
   __syncthreads();
-  {
-    const unsigned long long* src = (const unsigned long long*)&L.s;
-    unsigned long long* dst = (unsigned long long*)dsg;
-    for (int i = threadIdx.x; i < kBDsWords; i += blockDim.x) dst[i] = src[i];
-  }
-  __syncthreads();
+  ds_copy((unsigned long long*)dsg, (const unsigned long long*)&L.s);
 }
 
-template <typename T, int NT>
+// The persistent kernel over one queue: GO = true takes the gang entries
+// items[0 .. count[0]) through next[0], GO = false the one-workgroup
+// entries items[count[0] .. count[0] + count[1]) through next[1].  The two
+// are separate kernels so that each has the register allocation of its own
+// form.
+template <typename T, int NT, bool GO>
 __global__ void __launch_bounds__(NT)
-k_batch(const BatchProb* __restrict__ probs, const int* __restrict__ order, int count,
-        unsigned* next, BatchBufs<T> B, DevScalars* dsa, BatchResult* res, int iter_budget) {
+k_batch(const BatchProb* __restrict__ probs, const BatchItem* __restrict__ items,
+        const int* __restrict__ count_p, unsigned* next, GangCtl* gang, double* gpart,
+        BatchBufs<T> B, DevScalars* dsa, BatchResult* res, int iter_budget) {
   __shared__ BatchLds L;
   __shared__ int s_k;
+  const int count = count_p[GO ? 0 : 1];
+  const int first = GO ? 0 : count_p[0];
   for (;;) {
-    if (threadIdx.x == 0) s_k = (int)atomicAdd(next, 1u);
+    if (threadIdx.x == 0) s_k = (int)atomicAdd(next + (GO ? 0 : 1), 1u);
     __syncthreads();
     const int k = s_k;
     __syncthreads();
     if (k >= count) return;
-    const int p = order[k];
-    const BatchProb bp = probs[p];
-    solve_one<T>(bp, B, dsa + p, res + p, iter_budget, L);
+    const BatchItem it = items[first + k];
+    const BatchProb bp = probs[it.prob];
+    Gang<GO> G{GO ? bp.gang : 1, it.member, (long long)k - it.member, gang + it.prob, gpart, 0u};
+    solve_one<T, GO>(bp, B, dsa + it.prob, res + it.prob, iter_budget, L, G);
+  }
+}
+
+__global__ void k_batch_clear(unsigned* next, GangCtl* gang, int nprob) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2) next[i] = 0u;
+  if (i < nprob) {
+    gang[i].arrive = 0u;
+    gang[i].err = 0;
+    gang[i].cand = 0ull;
   }
 }
 
 }  // namespace
 
-template <typename T>
-int launch_batch(hipStream_t s, int threads, int blocks, const BatchProb* probs, const int* order,
-                 int count, unsigned* next, BatchBufs<T> B, DevScalars* dsa, BatchResult* res,
+template <typename T, int NT>
+void launch_pair(hipStream_t s, int blocks_gang, int blocks_one, const BatchProb* probs,
+                 const BatchItem* items, const int* count, unsigned* next, GangCtl* gang,
+                 double* gpart, BatchBufs<T> B, DevScalars* dsa, BatchResult* res,
                  int iter_budget) {
+  if (blocks_gang > 0)
+    hipLaunchKernelGGL((k_batch<T, NT, true>), dim3(blocks_gang), dim3(NT), 0, s, probs, items,
+                       count, next, gang, gpart, B, dsa, res, iter_budget);
+  if (blocks_one > 0)
+    hipLaunchKernelGGL((k_batch<T, NT, false>), dim3(blocks_one), dim3(NT), 0, s, probs, items,
+                       count, next, gang, gpart, B, dsa, res, iter_budget);
+}
+
+template <typename T>
+int launch_batch(hipStream_t s, int threads, int blocks_gang, int blocks_one,
+                 const BatchProb* probs, const BatchItem* items, const int* count, unsigned* next,
+                 GangCtl* gang, double* gpart, BatchBufs<T> B, DevScalars* dsa,
+                 BatchResult* res, int iter_budget) {
   switch (threads) {
     case 256:
-      hipLaunchKernelGGL((k_batch<T, 256>), dim3(blocks), dim3(256), 0, s, probs, order, count,
-                         next, B, dsa, res, iter_budget);
+      launch_pair<T, 256>(s, blocks_gang, blocks_one, probs, items, count, next, gang, gpart, B,
+                          dsa, res, iter_budget);
       return 0;
     case 512:
-      hipLaunchKernelGGL((k_batch<T, 512>), dim3(blocks), dim3(512), 0, s, probs, order, count,
-                         next, B, dsa, res, iter_budget);
+      launch_pair<T, 512>(s, blocks_gang, blocks_one, probs, items, count, next, gang, gpart, B,
+                          dsa, res, iter_budget);
       return 0;
     default:
       return -1;
   }
 }
 
-template int launch_batch<float>(hipStream_t, int, int, const BatchProb*, const int*, int,
-                                 unsigned*, BatchBufs<float>, DevScalars*, BatchResult*, int);
-template int launch_batch<double>(hipStream_t, int, int, const BatchProb*, const int*, int,
-                                  unsigned*, BatchBufs<double>, DevScalars*, BatchResult*, int);
+void launch_batch_clear(hipStream_t s, unsigned* next, GangCtl* gang, int nprob) {
+  const int n = nprob > 1 ? nprob : 1;
+  hipLaunchKernelGGL(k_batch_clear, dim3((n + 255) / 256), dim3(256), 0, s, next, gang, nprob);
+}
+
+template <typename T>
+int batch_blocks_per_cu(int threads) {
+  int a = 0, b = 0;
+  hipError_t e1, e2;
+  if (threads == 512) {
+    e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batch<T, 512, true>, 512, 0);
+    e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batch<T, 512, false>, 512, 0);
+  } else {
+    e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batch<T, 256, true>, 256, 0);
+    e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batch<T, 256, false>, 256, 0);
+  }
+  if (e1 != hipSuccess || e2 != hipSuccess) return 0;
+  return a < b ? a : b;
+}
+
+template int launch_batch<float>(hipStream_t, int, int, int, const BatchProb*,
+                                 const BatchItem*, const int*, unsigned*, GangCtl*, double*,
+                                 BatchBufs<float>, DevScalars*, BatchResult*, int);
+template int launch_batch<double>(hipStream_t, int, int, int, const BatchProb*,
+                                  const BatchItem*, const int*, unsigned*, GangCtl*, double*,
+                                  BatchBufs<double>, DevScalars*, BatchResult*, int);
+template int batch_blocks_per_cu<float>(int);
+template int batch_blocks_per_cu<double>(int);
 
 }  // namespace lbfgsb

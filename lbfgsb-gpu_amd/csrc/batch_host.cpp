@@ -6,7 +6,10 @@
 // 16 x its offset).  A slot holds one problem from admission to removal;
 // `run` launches the kernel over every unfinished slot, largest first, either
 // to completion or for an iteration budget (the serving loop resumes the
-// suspended problems in its next launch).
+// suspended problems in its next launch).  With a segment length set, a
+// problem of n elements is solved by a gang of min(max_gang, ceil(n /
+// segment)) workgroups; the gang size depends on n and the configuration
+// only, so a problem's result does not depend on what else is in the batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,6 +46,8 @@ struct BatchBase {
   virtual long long free_elems() const = 0;
   virtual int results_all(double* f, int* iter, int* info, int* done) = 0;
   virtual int clocks(long long* start, long long* end) = 0;
+  virtual int configure(long long segment, int max_gang) = 0;
+  virtual int gang_of(int slot) const = 0;
   bool ok = false;
 };
 
@@ -66,8 +71,14 @@ struct Arena final : BatchBase {
   DevScalars* dsa = nullptr;
   BatchResult* res = nullptr;
   BatchProb* probs = nullptr;
-  int* order = nullptr;
+  BatchItem* items = nullptr;
+  int* count = nullptr;
+  GangCtl* gang = nullptr;
+  double* gpart = nullptr;
   unsigned* next = nullptr;
+  long long max_items = 0;
+  long long segment = 0;  // 0: one workgroup per problem
+  int max_gang = 1;
   std::vector<Slot> slots;
   std::vector<BatchProb> hprobs;
   std::vector<BatchResult> hres;
@@ -95,7 +106,12 @@ struct Arena final : BatchBase {
     for (T** p : vec)
       if (dm(*p, c)) return -1;
     if (dm(W, c * 16) || dm(nbd, c) || dm(iwhere, c) || dm(ca, c) || dm(cb, c)) return -1;
-    if (dm(dsa, maxp) || dm(res, maxp) || dm(probs, maxp) || dm(order, maxp) || dm(next, 1))
+    // queue entries: one per problem plus one per extra gang member (each
+    // member takes at least kMinSegment elements)
+    max_items = (long long)maxp + cap / kMinSegment + 1;
+    if (dm(dsa, maxp) || dm(res, maxp) || dm(probs, maxp) || dm(items, (size_t)max_items) ||
+        dm(count, 2) || dm(gang, maxp) || dm(gpart, (size_t)max_items * 2 * kGangPart) ||
+        dm(next, 2))
       return -1;
     BCHK(hipMemsetAsync(W, 0, c * 16 * sizeof(T), st));
     BCHK(hipStreamSynchronize(st));
@@ -103,8 +119,9 @@ struct Arena final : BatchBase {
   }
   ~Arena() override {
     hipStreamSynchronize(st);
-    void* ptrs[] = {x,   xb,  g,   gb,     l,  u,  W,   d,   z,     r,   dsub, ob,    oc,
-                    oa0, oa1, oa2, nbd,    iwhere, ca, cb, dsa, res, probs, order, next};
+    void* ptrs[] = {x,   xb,   g,      gb, l,  u,   W,   d,     z,     r,     dsub, ob,
+                    oc,  oa0,  oa1,    oa2, nbd, iwhere, ca, cb, dsa, res, probs, items,
+                    count, gang, gpart, next};
     for (void* p : ptrs)
       if (p) hipFree(p);
   }
@@ -193,6 +210,7 @@ struct Arena final : BatchBase {
     p.obj.a0 = oa0 + off;
     p.obj.a1 = oa1 + off;
     p.obj.a2 = oa2 + off;
+    p.gang = 1;
     // the state the host driver starts from (capi.cpp: drive)
     DevScalars ds;
     std::memset(&ds, 0, sizeof(ds));
@@ -211,29 +229,67 @@ struct Arena final : BatchBase {
     return slot;
   }
 
+  int configure(long long seg, int mg) override {
+    if (seg < 0 || (seg > 0 && seg < kMinSegment) || mg < 1) return -1;
+    segment = seg;
+    max_gang = mg;
+    return 0;
+  }
+  int gang_size(long long n) const {
+    if (segment <= 0) return 1;
+    const long long s = (n + segment - 1) / segment;
+    return (int)std::max<long long>(1, std::min<long long>(s, max_gang));
+  }
+  int gang_of(int slot) const override {
+    if (slot < 0 || slot >= maxp || !slots[slot].used) return -1;
+    return gang_size(slots[slot].n);
+  }
+
   int run(int threads, int blocks, int iter_budget) override {
     std::vector<int> ord;
     for (int k = 0; k < maxp; k++)
       if (slots[k].used && !slots[k].done) ord.push_back(k);
     if (ord.empty()) return 0;
-    // largest problems first: the tail of the persistent queue is short work
+    // largest problems first: the gangs start together and the tail of the
+    // persistent queue is short work
     std::stable_sort(ord.begin(), ord.end(),
                      [&](int a, int b) { return slots[a].n > slots[b].n; });
     if (threads != 256 && threads != 512) threads = 256;
-    if (blocks <= 0) {
-      int dev = 0, cus = 0;
-      hipGetDevice(&dev);
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      blocks = std::max(1, cus) * (threads == 256 ? 2 : 1);
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // every workgroup of the launch is resident at once (gang members wait
+    // for each other)
+    const int per_cu = batch_blocks_per_cu<T>(threads);
+    if (per_cu <= 0) return -2;
+    const int resident = std::max(1, cus) * per_cu;
+    if (blocks <= 0) blocks = std::max(1, cus) * (threads == 256 ? 2 : 1);
+    blocks = std::min(blocks, resident);
+    // gang entries first (members consecutive), then the one-workgroup ones
+    std::vector<BatchItem> q, solo;
+    for (int k : ord) {
+      const int S = gang_size(slots[k].n);
+      if (S > blocks) return -4;  // the gang cannot be resident at once
+      hprobs[k].gang = S;
+      if (S > 1)
+        for (int j = 0; j < S; j++) q.push_back(BatchItem{k, j});
+      else
+        solo.push_back(BatchItem{k, 0});
     }
-    blocks = std::min<int>(blocks, (int)ord.size());
+    const int cnt[2] = {(int)q.size(), (int)solo.size()};
+    q.insert(q.end(), solo.begin(), solo.end());
+    if ((long long)q.size() > max_items) return -2;
+    const int blocks_gang = std::min(blocks, cnt[0]);
+    const int blocks_one = std::min(blocks, cnt[1]);
     BCHK(hipMemcpyAsync(probs, hprobs.data(), sizeof(BatchProb) * maxp, hipMemcpyHostToDevice,
                         st));
-    BCHK(hipMemcpyAsync(order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, st));
-    BCHK(hipMemsetAsync(next, 0, sizeof(unsigned), st));
+    BCHK(hipMemcpyAsync(items, q.data(), sizeof(BatchItem) * q.size(), hipMemcpyHostToDevice,
+                        st));
+    BCHK(hipMemcpyAsync(count, cnt, sizeof(cnt), hipMemcpyHostToDevice, st));
+    launch_batch_clear(st, next, gang, maxp);
     BatchBufs<T> B{x, xb, g, gb, l, u, nbd, iwhere, W, d, z, r, dsub, ca, cb};
-    if (launch_batch<T>(st, threads, blocks, probs, order, (int)ord.size(), next, B, dsa, res,
-                        iter_budget) != 0)
+    if (launch_batch<T>(st, threads, blocks_gang, blocks_one, probs, items, count, next, gang,
+                        gpart, B, dsa, res, iter_budget) != 0)
       return -1;
     BCHK(hipGetLastError());
     BCHK(hipMemcpyAsync(hres.data(), res, sizeof(BatchResult) * maxp, hipMemcpyDeviceToHost,
@@ -369,5 +425,15 @@ int lbfgsb_batch_remove(lbfgsb_batch* b, int slot) {
 }
 
 long long lbfgsb_batch_free_elems(lbfgsb_batch* b) { return b ? b->b->free_elems() : -1; }
+
+int lbfgsb_batch_configure(lbfgsb_batch* b, long long segment, int max_gang) {
+  if (!b) return -1;
+  return b->b->configure(segment, max_gang);
+}
+
+int lbfgsb_batch_gang(lbfgsb_batch* b, int slot) {
+  if (!b) return -1;
+  return b->b->gang_of(slot);
+}
 
 }  // extern "C"

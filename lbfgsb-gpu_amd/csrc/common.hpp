@@ -32,7 +32,14 @@ template <typename T>
 struct Pair {
   T* b0;
   T* b1;
-  __host__ __device__ T* operator[](int s) const { return s ? b1 : b0; }
+  // a select of the two values as integers: `s ? b1 : b0` may be folded into
+  // a load through a selected address, which keeps the enclosing struct in
+  // scratch memory
+  __host__ __device__ T* operator[](int s) const {
+    const uintptr_t a = (uintptr_t)b0, b = (uintptr_t)b1;
+    const uintptr_t mask = (uintptr_t)0 - (uintptr_t)(s != 0);
+    return (T*)(a ^ ((a ^ b) & mask));
+  }
 };
 
 // ---------------------------------------------------------------------------

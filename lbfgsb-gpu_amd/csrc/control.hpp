@@ -409,7 +409,7 @@ __device__ inline void logic_update_gcp(DevScalars* s, int reduce, const double*
 // After the Cauchy-point pass: free/active bookkeeping
 // (lbfgsbcpu.cpp:1143-1161), formk, and the two triangular solves of subsm
 // (lbfgsbcpu.cpp:1459-1471).  r = the pass's sums (P4_* layout).
-__device__ inline void logic_formk_subsm(DevScalars* s, const double* r, bool flagged) {
+__device__ __attribute__((noinline)) void logic_formk_subsm(DevScalars* s, const double* r, bool flagged) {
   const long long n = s->n;
   const int col = s->col;
   if (!s->shortcut) {

@@ -319,22 +319,27 @@ __device__ inline Cand* block_sort_large(Cand* a, Cand* b, long long K, Cand* sc
   return src;
 }
 
-// Scan sorted candidates (LDS or global) in chunks of blockDim: prefix sums
-// of the contributions, the stop test at every position, and the round's
-// conclusion by thread 0.  L.gc must be staged.
+// Scan sorted candidates cands[c_lo .. c_hi) of a round of K (LDS or global)
+// in chunks of blockDim: prefix sums of the contributions from the state in
+// L.scarry and, when `test`, the stop test at every position, ending at the
+// first chunk that holds a stop.  On return L.sstop is the first stopping
+// position (or the largest long long) with its exclusive state in
+// L.sstop_state, L.scarry is the inclusive state after the last position
+// scanned, and L.slast the exclusive state of position K - 1 if scanned.
+// L.gc must be staged.
 template <typename T>
-__device__ inline void block_round_scan(DevScalars* ds, RoundLds& L, const Cand* cands,
-                                        long long K, const T* g, const T* W) {
+__device__ __forceinline__ void block_scan_range(const DevScalars* ds, RoundLds& L, const Cand* cands,
+                                        long long c_lo, long long c_hi, long long K, bool test,
+                                        const T* g, const T* W) {
   const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1);
   const double theta = ds->theta;
   const long long kpassed = ds->gcp_kpassed;
   const double t_prev0 = ds->gcp_t_prev;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
-  if (threadIdx.x < kNV) L.scarry[threadIdx.x] = ds->gcp_carry[threadIdx.x];
   if (threadIdx.x == 0) L.sstop = 0x7fffffffffffffffLL;
   __syncthreads();
-  for (long long c0 = 0; c0 < K; c0 += blockDim.x) {
+  for (long long c0 = c_lo; c0 < c_hi; c0 += blockDim.x) {
     const long long pos = c0 + threadIdx.x;
     double v[kNV], ex[kNV];
 #pragma unroll
@@ -343,7 +348,7 @@ __device__ inline void block_round_scan(DevScalars* ds, RoundLds& L, const Cand*
     c.t = 0;
     c.idx = 0;
     c.pad = 0;
-    if (pos < K) {
+    if (pos < c_hi) {
       c = cands[pos];
       contribution(c, g, W, rr, theta, v);
     }
@@ -366,7 +371,7 @@ __device__ inline void block_round_scan(DevScalars* ds, RoundLds& L, const Cand*
       ex[q] += off;
     }
     bool stop = false;
-    if (pos < K && kpassed + pos > 0) {
+    if (test && pos < c_hi && kpassed + pos > 0) {
       const double tp = pos > 0 ? cands[pos - 1].t : t_prev0;
       double out[3];
       stop = stop_test(ds, L.gc, ex, tp, c.t, true, out);
@@ -389,6 +394,14 @@ __device__ inline void block_round_scan(DevScalars* ds, RoundLds& L, const Cand*
     __syncthreads();
   }
   __syncthreads();
+}
+
+// A whole round's scan in one workgroup and its conclusion by thread 0.
+template <typename T>
+__device__ inline void block_round_scan(DevScalars* ds, RoundLds& L, const Cand* cands,
+                                        long long K, const T* g, const T* W) {
+  if (threadIdx.x < kNV) L.scarry[threadIdx.x] = ds->gcp_carry[threadIdx.x];
+  block_scan_range(ds, L, cands, 0, K, K, true, g, W);
   if (threadIdx.x == 0) {
     const bool stopped = L.sstop != 0x7fffffffffffffffLL;
     round_conclude(ds, L.gc, cands, K, stopped ? L.sstop : -1,

@@ -114,6 +114,8 @@ SIGNATURES = {
     "lbfgsb_batch_get_x": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "lbfgsb_batch_remove": (C.c_int, [C.c_void_p, C.c_int]),
     "lbfgsb_batch_free_elems": (C.c_longlong, [C.c_void_p]),
+    "lbfgsb_batch_configure": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int]),
+    "lbfgsb_batch_gang": (C.c_int, [C.c_void_p, C.c_int]),
     "lbfgsb_amd_version": (C.c_char_p, []),
 }
 

@@ -8,9 +8,10 @@ slots solved by the persistent per-problem kernel.
 
 `run(iter_budget=k)` advances each unfinished problem by at most k
 iterations and returns how many remain; the serving loop admits new problems
-between such calls.  `autotune()` picks the workgroup size (threads per
-problem) and the number of resident workgroups by timing whole solves of a
-sample of the workload.
+between such calls.  `configure(segment, max_gang)` lets large problems be
+solved by gangs of workgroups, one segment each.  `autotune()` picks the
+workgroup size, the number of resident workgroups and the segment length by
+timing whole solves of a sample of the workload.
 """
 from __future__ import annotations
 
@@ -44,6 +45,7 @@ class Batch:
         self.max_problems = int(max_problems)
         self.n = {}
         self.threads, self.blocks = 256, 0
+        self.segment, self.max_gang = 0, 1
 
     def _dev(self, a, tdt):
         return self.torch.from_numpy(np.ascontiguousarray(a)).to("cuda", tdt)
@@ -86,6 +88,8 @@ class Batch:
         th = self.threads if threads is None else threads
         bl = self.blocks if blocks is None else blocks
         left = self.L.lbfgsb_batch_run(self.h, int(th), int(bl), int(iter_budget))
+        if left == -4:
+            raise ValueError("a gang is wider than the launch (fewer workgroups than max_gang)")
         if left < 0:
             raise RuntimeError(f"lbfgsb_batch_run failed ({left})")
         return left
@@ -127,6 +131,17 @@ class Batch:
     def free_elems(self) -> int:
         return int(self.L.lbfgsb_batch_free_elems(self.h))
 
+    def configure(self, segment: int = 0, max_gang: int = 1):
+        """Gangs: a problem of n elements is solved by min(max_gang,
+        ceil(n / segment)) workgroups (segment >= 4096; 0 = one workgroup
+        per problem)."""
+        if self.L.lbfgsb_batch_configure(self.h, int(segment), int(max_gang)) != 0:
+            raise ValueError(f"bad gang configuration segment={segment} max_gang={max_gang}")
+        self.segment, self.max_gang = int(segment), int(max_gang)
+
+    def gang(self, slot: int) -> int:
+        return int(self.L.lbfgsb_batch_gang(self.h, slot))
+
     def close(self):
         if self.h:
             self.L.lbfgsb_batch_destroy(self.h)
@@ -139,28 +154,41 @@ class Batch:
             pass
 
 
-# (threads per problem, resident workgroups as a multiple of the CU count)
-TUNE_CANDIDATES = ((256, 2), (256, 1), (512, 1), (256, 3))
+# (threads per workgroup, resident workgroups as a multiple of the CU count,
+# segment length of the gangs (0 = one workgroup per problem))
+TUNE_CANDIDATES = ((256, 2, 0), (512, 1, 0), (512, 1, 131072), (512, 1, 65536),
+                   (512, 1, 16384))
+MAX_GANG = 64
 
 
-def autotune(sample, dtype, eps=1e-6, max_iter=1000, candidates=TUNE_CANDIDATES):
+def autotune(sample, dtype, eps=1e-6, max_iter=1000, candidates=TUNE_CANDIDATES,
+             max_gang=MAX_GANG):
     """Solve `sample` (a list of Problems drawn from the workload) once per
-    candidate shape and return the fastest as {"threads", "blocks", ...}."""
+    candidate shape and return the fastest as {"threads", "blocks",
+    "segment", "max_gang", ...}.  A shape that cannot hold the widest gang
+    on one launch is skipped."""
     torch = _torch()
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     cap = sum(p.n + 64 for p in sample)
     timings = []
-    for th, per_cu in candidates:
+    for th, per_cu, seg in candidates:
         b = Batch(dtype, cap, len(sample))
+        b.configure(seg, max_gang if seg else 1)
         for p in sample:
             if b.add(p, eps=eps, max_iter=max_iter) < 0:
                 raise RuntimeError("autotune: problem rejected")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        b.run(threads=th, blocks=cus * per_cu)
+        try:
+            b.run(threads=th, blocks=cus * per_cu)
+        except ValueError:
+            b.close()
+            continue
         dt = time.perf_counter() - t0
         b.close()
-        timings.append((dt, th, cus * per_cu))
-    dt, th, bl = min(timings)
-    return {"threads": th, "blocks": bl, "seconds": dt,
-            "tried": [{"threads": t, "blocks": b_, "seconds": d} for d, t, b_ in timings]}
+        timings.append((dt, th, cus * per_cu, seg))
+    dt, th, bl, seg = min(timings)
+    return {"threads": th, "blocks": bl, "segment": seg, "max_gang": max_gang if seg else 1,
+            "seconds": dt,
+            "tried": [{"threads": t, "blocks": b_, "segment": g, "seconds": d}
+                      for d, t, b_, g in timings]}
