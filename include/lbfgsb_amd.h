@@ -198,6 +198,8 @@ int lbfgsb_batch_clocks(lbfgsb_batch* b, long long* start, long long* end,
 int lbfgsb_batch_get_x(lbfgsb_batch* b, int slot, void* out);
 int lbfgsb_batch_remove(lbfgsb_batch* b, int slot);
 long long lbfgsb_batch_free_elems(lbfgsb_batch* b);
+/* Device bytes an arena of this dtype, capacity and slot count holds. */
+long long lbfgsb_batch_bytes(int dtype, long long capacity, int max_problems);
 /* Gangs: with segment > 0 (at least 4096 elements) a problem of n elements
  * is solved by min(max_gang, ceil(n / segment)) workgroups together, each
  * on one segment of its elements, reductions combined in member order; the
@@ -208,6 +210,84 @@ int lbfgsb_batch_configure(lbfgsb_batch* b, long long segment, int max_gang);
 /* Workgroups the slot's problem is solved by under the current
  * configuration (-1 for a free slot). */
 int lbfgsb_batch_gang(lbfgsb_batch* b, int slot);
+/* on != 0: every _run replays a captured HIP graph of its launches (a
+ * queue-clearing kernel and the two batch kernels at fixed grids; queue
+ * sizes are read on the device), captured again when threads, blocks or
+ * iter_budget change.  Results are identical with and without. */
+int lbfgsb_batch_use_graph(lbfgsb_batch* b, int on);
+
+/* ---- Generated problems (no counterpart in the reference): the benchmark
+ * objectives built on the device from a seed -- kind 1 grid quadratic,
+ * 2 grid cos (nx * ny == n), 3 extended Rosenbrock (even n),
+ * 4 pentadiagonal box QP -- with the arrays of lbfgsb_amd/problems.py
+ * (uniform draws and the arithmetic on them bit-identical; the normal
+ * draws through the device's log and cos).
+ * _add_generated admits such a problem into a slot (returns as _add);
+ * lbfgsb_generate writes one into device arrays of n elements (the
+ * objective arrays its kind uses; others may be NULL), returns 0, -1 on a
+ * bad spec, -2 on a device error. */
+int lbfgsb_batch_add_generated(lbfgsb_batch* b, int kind, long long n, long long nx, long long ny,
+                               unsigned long long seed, double eps_g, double eps_f, double eps_x,
+                               int max_iteration);
+int lbfgsb_generate(int dtype, int kind, long long n, long long nx, long long ny,
+                    unsigned long long seed, void* x0, int* nbd, void* l, void* u, void* b,
+                    void* c, void* a0, void* a1, void* a2, void* stream);
+
+/* ---- Serving loop (no counterpart in the reference).  A stream of solve
+ * requests with arrival times is served in real time: a request is admitted
+ * (its problem generated into a slot of the arena of its precision) as soon
+ * as it has arrived and the arena has a slot and a range free -- first come,
+ * first served per precision -- and every resident problem advances by
+ * `quantum` iterations per launch of the batch kernels (graph-replayed when
+ * use_graph), so requests that arrive while others run join the next
+ * launch.  The two arenas are sized so that all the server holds on the
+ * device stays within hbm_budget_bytes (fp32_share of it for fp32).  A
+ * problem's result is the one it has alone (the batched form's results do
+ * not depend on the other problems).  Times are host clock seconds from the
+ * start of the stream; requests are taken in arrival order. */
+typedef struct {
+  int kind;                 /* objective kind, as lbfgsb_generate */
+  int dtype;                /* 0 fp32 bulk, 1 fp64 */
+  long long n, nx, ny;      /* size (grid kinds: nx * ny == n) */
+  unsigned long long seed;  /* problem seed, as lbfgsb_generate */
+  double eps;               /* eps_g = eps_f = eps_x */
+  int max_iteration;
+  int pad;
+  double arrival;           /* seconds after the start of the stream */
+} lbfgsb_request;
+
+typedef struct {
+  double f;
+  int iter;
+  int info;                 /* summary info; -1 bad request, -3 larger than its arena */
+  int gang;                 /* workgroups that solved it */
+  int dtype;
+  double t_arrival, t_admit, t_done;
+} lbfgsb_response;
+
+typedef struct {
+  double hbm_budget_bytes;  /* device memory the two arenas may hold together */
+  double fp32_share;        /* fraction of it for the fp32 arena (0..1) */
+  int max_slots;            /* problems resident at once per arena */
+  int threads, blocks;      /* batch workgroup shape (0 = 256 and the device's capacity) */
+  long long segment;        /* gang segment (0 = one workgroup per problem) */
+  int max_gang;
+  int quantum;              /* iterations per launch (> 0) */
+  int use_graph;
+  int pad;
+} lbfgsb_serve_config;
+
+typedef struct {
+  double seconds;           /* stream start to last completion */
+  long long launches;       /* batch launches (one per arena per quantum) */
+  double arena_bytes;       /* device memory the arenas hold */
+  int max_waiting;          /* longest admission queue seen */
+  int max_resident;         /* most problems resident at once */
+} lbfgsb_serve_stats;
+
+/* Returns 0, -1 on a bad configuration, -2 on a device error. */
+int lbfgsb_serve_stream(const lbfgsb_serve_config* cfg, const lbfgsb_request* reqs, int count,
+                        lbfgsb_response* out, lbfgsb_serve_stats* stats, void* stream);
 
 /* Library version string. */
 const char* lbfgsb_amd_version(void);

@@ -21,6 +21,7 @@
 #include "../../include/lbfgsb_amd.h"
 #include "batch.hpp"
 #include "common.hpp"
+#include "generate.hpp"
 
 using namespace lbfgsb;
 
@@ -38,6 +39,8 @@ struct BatchBase {
                   int kind, long long nx, long long ny, double h, const void* b, const void* c,
                   const void* a0, const void* a1, const void* a2, double eps_g, double eps_f,
                   double eps_x, int max_iteration) = 0;
+  virtual int add_generated(const GenSpec& sp, double eps_g, double eps_f, double eps_x,
+                            int max_iteration) = 0;
   virtual int run(int threads, int blocks, int iter_budget) = 0;
   virtual int result(int slot, double* f, int* iter, int* info, double* residual_g,
                      int* done) = 0;
@@ -47,6 +50,7 @@ struct BatchBase {
   virtual int results_all(double* f, int* iter, int* info, int* done) = 0;
   virtual int clocks(long long* start, long long* end) = 0;
   virtual int configure(long long segment, int max_gang) = 0;
+  virtual void set_graph(bool on) = 0;
   virtual int gang_of(int slot) const = 0;
   bool ok = false;
 };
@@ -79,6 +83,12 @@ struct Arena final : BatchBase {
   long long max_items = 0;
   long long segment = 0;  // 0: one workgroup per problem
   int max_gang = 1;
+  // launches replayed from a captured graph (clear kernel + both batch
+  // kernels at fixed grids), re-captured when the shape changes
+  bool use_graph = false;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_threads = 0, g_blocks = 0, g_budget = -1;
   std::vector<Slot> slots;
   std::vector<BatchProb> hprobs;
   std::vector<BatchResult> hres;
@@ -119,6 +129,8 @@ struct Arena final : BatchBase {
   }
   ~Arena() override {
     hipStreamSynchronize(st);
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (cap_stream) hipStreamDestroy(cap_stream);
     void* ptrs[] = {x,   xb,   g,      gb, l,  u,   W,   d,     z,     r,     dsub, ob,
                     oc,  oa0,  oa1,    oa2, nbd, iwhere, ca, cb, dsa, res, probs, items,
                     count, gang, gpart, next};
@@ -158,17 +170,21 @@ struct Arena final : BatchBase {
     }
   }
 
-  int add(long long n, const void* x0, const int* nbd_in, const void* l_in, const void* u_in,
-          int kind, long long nx, long long ny, double h, const void* b, const void* c,
-          const void* a0, const void* a1, const void* a2, double eps_g, double eps_f,
-          double eps_x, int max_iteration) override {
-    // errclb's scalar checks (lbfgsbcpu.cpp:879-890) and the option checks
+  // errclb's scalar checks (lbfgsbcpu.cpp:879-890), the option checks and
+  // the objective's shape
+  int check(long long n, int kind, long long nx, long long ny, double eps_g, double eps_f,
+            double eps_x, int max_iteration) const {
     if (n <= 0 || m <= 0 || m > n || eps_f < 0 || eps_g < 0 || eps_x < 0 || max_iteration < 0)
       return -1;
     if (kind < OBJK_GRID_QUAD || kind > OBJK_PENTA) return -1;
     if ((kind == OBJK_GRID_QUAD || kind == OBJK_GRID_COS) && (nx <= 0 || ny <= 0 || nx * ny != n))
       return -1;
     if (n >= (1LL << 31)) return -1;  // candidate indices are 32-bit
+    return 0;
+  }
+
+  // A free slot and a range of npad elements, or -3.
+  int claim(long long n) {
     int slot = -1;
     for (int k = 0; k < maxp; k++)
       if (!slots[k].used) {
@@ -185,18 +201,18 @@ struct Arena final : BatchBase {
     s.off = off;
     s.n = n;
     s.npad = npad;
-    const size_t bytes = (size_t)n * sizeof(T);
-    auto cp = [&](T* dst, const void* src) -> int {
-      if (src) BCHK(hipMemcpyAsync(dst + off, src, bytes, hipMemcpyDeviceToDevice, st));
-      return 0;
-    };
-    if (cp(x, x0) || cp(l, l_in) || cp(u, u_in) || cp(ob, b) || cp(oc, c) || cp(oa0, a0) ||
-        cp(oa1, a1) || cp(oa2, a2))
-      return -2;
-    BCHK(hipMemcpyAsync(nbd + off, nbd_in, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, st));
+    return slot;
+  }
+
+  // Objective descriptor, zeroed memory pairs and the starting control
+  // state of a filled slot; waits for the slot's uploads.
+  int init_slot(int slot, int kind, long long nx, long long ny, double h, double eps_g,
+                double eps_f, double eps_x, int max_iteration) {
+    const Slot& sl = slots[slot];
+    const long long off = sl.off, n = sl.n;
     // the memory slots of a reused range may hold another problem's pairs:
     // they are multiplied by zero coefficients only, but keep them finite
-    BCHK(hipMemsetAsync(W + off * 16, 0, (size_t)npad * 16 * sizeof(T), st));
+    BCHK(hipMemsetAsync(W + off * 16, 0, (size_t)sl.npad * 16 * sizeof(T), st));
     BatchProb& p = hprobs[slot];
     p.off = off;
     p.n = n;
@@ -229,6 +245,52 @@ struct Arena final : BatchBase {
     return slot;
   }
 
+  int add(long long n, const void* x0, const int* nbd_in, const void* l_in, const void* u_in,
+          int kind, long long nx, long long ny, double h, const void* b, const void* c,
+          const void* a0, const void* a1, const void* a2, double eps_g, double eps_f,
+          double eps_x, int max_iteration) override {
+    if (check(n, kind, nx, ny, eps_g, eps_f, eps_x, max_iteration) != 0) return -1;
+    const int slot = claim(n);
+    if (slot < 0) return slot;
+    const long long off = slots[slot].off;
+    const size_t bytes = (size_t)n * sizeof(T);
+    auto cp = [&](T* dst, const void* src) -> int {
+      if (src && hipMemcpyAsync(dst + off, src, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return -2;
+      return 0;
+    };
+    if (cp(x, x0) || cp(l, l_in) || cp(u, u_in) || cp(ob, b) || cp(oc, c) || cp(oa0, a0) ||
+        cp(oa1, a1) || cp(oa2, a2) ||
+        hipMemcpyAsync(nbd + off, nbd_in, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, st) !=
+            hipSuccess) {
+      remove(slot);
+      return -2;
+    }
+    const int r = init_slot(slot, kind, nx, ny, h, eps_g, eps_f, eps_x, max_iteration);
+    if (r < 0) remove(slot);
+    return r;
+  }
+
+  int add_generated(const GenSpec& sp, double eps_g, double eps_f, double eps_x,
+                    int max_iteration) override {
+    if (gen_check(sp) != 0 ||
+        check(sp.n, sp.kind, sp.nx, sp.ny, eps_g, eps_f, eps_x, max_iteration) != 0)
+      return -1;
+    const int slot = claim(sp.n);
+    if (slot < 0) return slot;
+    const long long off = slots[slot].off;
+    if (launch_generate<T>(st, sp, x + off, nbd + off, l + off, u + off, ob + off, oc + off,
+                           oa0 + off, oa1 + off, oa2 + off) != 0) {
+      remove(slot);
+      return -2;
+    }
+    const bool grid = sp.kind == OBJK_GRID_QUAD || sp.kind == OBJK_GRID_COS;
+    const double h = grid ? 1.0 / (double)(sp.nx + 1) : 0.0;
+    const int r = init_slot(slot, sp.kind, sp.nx, sp.ny, h, eps_g, eps_f, eps_x, max_iteration);
+    if (r < 0) remove(slot);
+    return r;
+  }
+
   int configure(long long seg, int mg) override {
     if (seg < 0 || (seg > 0 && seg < kMinSegment) || mg < 1) return -1;
     segment = seg;
@@ -240,6 +302,7 @@ struct Arena final : BatchBase {
     const long long s = (n + segment - 1) / segment;
     return (int)std::max<long long>(1, std::min<long long>(s, max_gang));
   }
+  void set_graph(bool on) override { use_graph = on; }
   int gang_of(int slot) const override {
     if (slot < 0 || slot >= maxp || !slots[slot].used) return -1;
     return gang_size(slots[slot].n);
@@ -279,18 +342,49 @@ struct Arena final : BatchBase {
     const int cnt[2] = {(int)q.size(), (int)solo.size()};
     q.insert(q.end(), solo.begin(), solo.end());
     if ((long long)q.size() > max_items) return -2;
-    const int blocks_gang = std::min(blocks, cnt[0]);
-    const int blocks_one = std::min(blocks, cnt[1]);
     BCHK(hipMemcpyAsync(probs, hprobs.data(), sizeof(BatchProb) * maxp, hipMemcpyHostToDevice,
                         st));
     BCHK(hipMemcpyAsync(items, q.data(), sizeof(BatchItem) * q.size(), hipMemcpyHostToDevice,
                         st));
     BCHK(hipMemcpyAsync(count, cnt, sizeof(cnt), hipMemcpyHostToDevice, st));
-    launch_batch_clear(st, next, gang, maxp);
     BatchBufs<T> B{x, xb, g, gb, l, u, nbd, iwhere, W, d, z, r, dsub, ca, cb};
-    if (launch_batch<T>(st, threads, blocks_gang, blocks_one, probs, items, count, next, gang,
-                        gpart, B, dsa, res, iter_budget) != 0)
-      return -1;
+    if (use_graph) {
+      // the queue sizes are read from `count` on the device: one graph
+      // serves every launch of this shape (workgroups beyond a queue's
+      // length find it empty and exit)
+      if (!gexec || g_threads != threads || g_blocks != blocks || g_budget != iter_budget) {
+        if (gexec) {
+          BCHK(hipGraphExecDestroy(gexec));
+          gexec = nullptr;
+        }
+        if (!cap_stream) BCHK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+        hipGraph_t gr = nullptr;
+        BCHK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeRelaxed));
+        launch_batch_clear(cap_stream, next, gang, maxp);
+        const int lr = launch_batch<T>(cap_stream, threads, blocks, blocks, probs, items, count,
+                                       next, gang, gpart, B, dsa, res, iter_budget);
+        BCHK(hipStreamEndCapture(cap_stream, &gr));
+        if (lr != 0) {
+          hipGraphDestroy(gr);
+          return -1;
+        }
+        const hipError_t ie = hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (ie != hipSuccess) {
+          gexec = nullptr;
+          return -2;
+        }
+        g_threads = threads;
+        g_blocks = blocks;
+        g_budget = iter_budget;
+      }
+      BCHK(hipGraphLaunch(gexec, st));
+    } else {
+      launch_batch_clear(st, next, gang, maxp);
+      if (launch_batch<T>(st, threads, std::min(blocks, cnt[0]), std::min(blocks, cnt[1]), probs,
+                          items, count, next, gang, gpart, B, dsa, res, iter_budget) != 0)
+        return -1;
+    }
     BCHK(hipGetLastError());
     BCHK(hipMemcpyAsync(hres.data(), res, sizeof(BatchResult) * maxp, hipMemcpyDeviceToHost,
                         st));
@@ -352,6 +446,24 @@ struct Arena final : BatchBase {
 struct lbfgsb_batch {
   BatchBase* b;
 };
+
+namespace {
+
+// Device bytes an arena of `capacity` elements and max_problems slots
+// allocates (the allocations of Arena::alloc).
+long long arena_bytes(int dtype, long long capacity, int max_problems) {
+  const long long s = dtype == 1 ? 8 : 4;
+  const long long cap = ((capacity + 63) / 64) * 64;
+  const long long items = (long long)max_problems + cap / kMinSegment + 1;
+  long long b = cap * (15 * s + 16 * s + (long long)sizeof(int) + 1 + 2 * (long long)sizeof(Cand));
+  b += (long long)max_problems *
+       (long long)(sizeof(DevScalars) + sizeof(BatchResult) + sizeof(BatchProb) + sizeof(GangCtl));
+  b += items * ((long long)sizeof(BatchItem) + 2LL * kGangPart * (long long)sizeof(double));
+  b += 2 * (long long)sizeof(int) + 2 * (long long)sizeof(unsigned);
+  return b;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -426,6 +538,11 @@ int lbfgsb_batch_remove(lbfgsb_batch* b, int slot) {
 
 long long lbfgsb_batch_free_elems(lbfgsb_batch* b) { return b ? b->b->free_elems() : -1; }
 
+long long lbfgsb_batch_bytes(int dtype, long long capacity, int max_problems) {
+  if (capacity <= 0 || max_problems <= 0) return -1;
+  return arena_bytes(dtype, capacity, max_problems);
+}
+
 int lbfgsb_batch_configure(lbfgsb_batch* b, long long segment, int max_gang) {
   if (!b) return -1;
   return b->b->configure(segment, max_gang);
@@ -434,6 +551,36 @@ int lbfgsb_batch_configure(lbfgsb_batch* b, long long segment, int max_gang) {
 int lbfgsb_batch_gang(lbfgsb_batch* b, int slot) {
   if (!b) return -1;
   return b->b->gang_of(slot);
+}
+
+int lbfgsb_batch_use_graph(lbfgsb_batch* b, int on) {
+  if (!b) return -1;
+  b->b->set_graph(on != 0);
+  return 0;
+}
+
+int lbfgsb_batch_add_generated(lbfgsb_batch* b, int kind, long long n, long long nx, long long ny,
+                               unsigned long long seed, double eps_g, double eps_f, double eps_x,
+                               int max_iteration) {
+  if (!b) return -1;
+  GenSpec sp{kind, 0, n, nx, ny, seed};
+  return b->b->add_generated(sp, eps_g, eps_f, eps_x, max_iteration);
+}
+
+int lbfgsb_generate(int dtype, int kind, long long n, long long nx, long long ny,
+                    unsigned long long seed, void* x0, int* nbd, void* l, void* u, void* bv,
+                    void* c, void* a0, void* a1, void* a2, void* stream) {
+  GenSpec sp{kind, 0, n, nx, ny, seed};
+  hipStream_t st = (hipStream_t)stream;
+  int r;
+  if (dtype == 1)
+    r = launch_generate<double>(st, sp, (double*)x0, nbd, (double*)l, (double*)u, (double*)bv,
+                                (double*)c, (double*)a0, (double*)a1, (double*)a2);
+  else
+    r = launch_generate<float>(st, sp, (float*)x0, nbd, (float*)l, (float*)u, (float*)bv,
+                               (float*)c, (float*)a0, (float*)a1, (float*)a2);
+  if (r != 0) return r;
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : -2;
 }
 
 }  // extern "C"

@@ -54,6 +54,32 @@ class CallbacksF64(C.Structure):
 
 CallbacksF32 = CallbacksF64  # same layout (function pointers and contexts)
 
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("dtype", C.c_int), ("n", C.c_longlong),
+                ("nx", C.c_longlong), ("ny", C.c_longlong), ("seed", C.c_ulonglong),
+                ("eps", C.c_double), ("max_iteration", C.c_int), ("pad", C.c_int),
+                ("arrival", C.c_double)]
+
+
+class Response(C.Structure):
+    _fields_ = [("f", C.c_double), ("iter", C.c_int), ("info", C.c_int), ("gang", C.c_int),
+                ("dtype", C.c_int), ("t_arrival", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_double), ("fp32_share", C.c_double),
+                ("max_slots", C.c_int), ("threads", C.c_int), ("blocks", C.c_int),
+                ("segment", C.c_longlong), ("max_gang", C.c_int), ("quantum", C.c_int),
+                ("use_graph", C.c_int), ("pad", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("launches", C.c_longlong),
+                ("arena_bytes", C.c_double), ("max_waiting", C.c_int),
+                ("max_resident", C.c_int)]
+
 # Every exported symbol with its signature (restype, argtypes).
 SIGNATURES = {
     "lbfgsb_default_options_f64": (None, [C.POINTER(OptionsF64)]),
@@ -116,6 +142,15 @@ SIGNATURES = {
     "lbfgsb_batch_free_elems": (C.c_longlong, [C.c_void_p]),
     "lbfgsb_batch_configure": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int]),
     "lbfgsb_batch_gang": (C.c_int, [C.c_void_p, C.c_int]),
+    "lbfgsb_batch_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "lbfgsb_batch_bytes": (C.c_longlong, [C.c_int, C.c_longlong, C.c_int]),
+    "lbfgsb_batch_add_generated": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong, C.c_longlong,
+                                             C.c_longlong, C.c_ulonglong, C.c_double,
+                                             C.c_double, C.c_double, C.c_int]),
+    "lbfgsb_generate": (C.c_int, [C.c_int, C.c_int, C.c_longlong, C.c_longlong, C.c_longlong,
+                                  C.c_ulonglong] + [C.c_void_p] * 10),
+    "lbfgsb_serve_stream": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(Request), C.c_int,
+                                      C.POINTER(Response), C.POINTER(ServeStats), C.c_void_p]),
     "lbfgsb_amd_version": (C.c_char_p, []),
 }
 

@@ -84,6 +84,20 @@ class Batch:
             self.n[slot] = int(n)
         return slot
 
+    def add_generated(self, kind: int, n: int, nx: int = 0, ny: int = 0, seed: int = 0,
+                      eps: float = 1e-6, max_iter: int = 1000) -> int:
+        """Admit a problem built on the device by the generator (the arrays
+        problems.py builds for the same kind, shape and seed)."""
+        slot = self.L.lbfgsb_batch_add_generated(self.h, int(kind), int(n), int(nx), int(ny),
+                                                 int(seed), eps, eps, eps, int(max_iter))
+        if slot >= 0:
+            self.n[slot] = int(n)
+        return slot
+
+    def use_graph(self, on: bool = True):
+        """Replay every launch from a captured HIP graph."""
+        self.L.lbfgsb_batch_use_graph(self.h, 1 if on else 0)
+
     def run(self, iter_budget: int = 0, threads: int | None = None, blocks: int | None = None):
         th = self.threads if threads is None else threads
         bl = self.blocks if blocks is None else blocks

@@ -198,3 +198,59 @@ def batched_qps(count: int = 4096, lo: float = 1e3, hi: float = 2e5,
     [lo, hi] (stream 7 of `seed`), problem k seeded with seed + (k+1)*2^20."""
     sizes = batch_sizes(count, lo, hi, seed)
     return [penta_qp(int(n), seed=seed + ((k + 1) << 20)) for k, n in enumerate(sizes)]
+
+
+# Tolerances and iteration caps of the generators' own workloads, used by
+# the serving stream: (eps, max_iteration) per objective kind.
+KIND_SETTINGS = {OBJ_GRID_QUAD: (1e-8, 1000), OBJ_GRID_COS: (1e-7, 2000),
+                 OBJ_ROSEN: (1e-7, 1000), OBJ_PENTA: (1e-6, 1000)}
+
+
+def request_shape(kind: int, n: int):
+    """(n, nx, ny) of a request of about n variables: grids take
+    nx = floor(sqrt(n)), ny = n // nx; Rosenbrock an even n."""
+    if kind in (OBJ_GRID_QUAD, OBJ_GRID_COS):
+        nx = max(1, int(math.isqrt(int(n))))
+        ny = max(1, int(n) // nx)
+        return nx * ny, nx, ny
+    if kind == OBJ_ROSEN:
+        n = int(n) - int(n) % 2
+        return max(n, 2), 0, 0
+    return int(n), 0, 0
+
+
+def serving_stream(count: int = 20000, qps: float = 10.0, lo: float = 1e3, hi: float = 4e6,
+                   fp32_frac: float = 0.7, seed: int = SEED_BASE + 4) -> list:
+    """The serving workload: `count` requests with Poisson arrivals at `qps`
+    per second (stream 8), n log-uniform in [lo, hi] (stream 7), the
+    objective kind uniform over the four generators and the precision fp32
+    with probability fp32_frac (stream 9, counters 2k and 2k+1), and problem
+    k seeded with seed + (k+1)*2^20.  Each request keeps its generator's
+    tolerance and iteration cap (KIND_SETTINGS)."""
+    u = uniform(seed, STREAM_ARRIVAL, count)
+    arrivals = np.cumsum(-np.log1p(-u) / qps)
+    sizes = batch_sizes(count, lo, hi, seed)
+    mix = uniform(seed, STREAM_MIX, 2 * count)
+    kinds = 1 + np.minimum((mix[0::2] * 4).astype(np.int64), 3)
+    out = []
+    for k in range(count):
+        kind = int(kinds[k])
+        n, nx, ny = request_shape(kind, int(sizes[k]))
+        eps, mi = KIND_SETTINGS[kind]
+        out.append(dict(kind=kind, dtype=0 if mix[2 * k + 1] < fp32_frac else 1, n=n, nx=nx,
+                        ny=ny, seed=seed + ((k + 1) << 20), eps=eps, max_iteration=mi,
+                        arrival=float(arrivals[k])))
+    return out
+
+
+def problem_of(req: dict) -> Problem:
+    """The host copy of a request's problem (the device generator builds the
+    same arrays)."""
+    kind, s = req["kind"], req["seed"]
+    if kind == OBJ_GRID_QUAD:
+        return grid_quadratic(req["nx"], req["ny"], seed=s)
+    if kind == OBJ_GRID_COS:
+        return grid_cos(req["nx"], req["ny"], seed=s)
+    if kind == OBJ_ROSEN:
+        return rosenbrock(req["n"], seed=s)
+    return penta_qp(req["n"], seed=s)

@@ -54,3 +54,33 @@ def test_device_code_is_gfx950():
     assert ".hip_fatbin" in out
     blob = open(_lib.LIB_PATH, "rb").read()
     assert b"gfx950" in blob
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of the header's structs have the C layout (sizes
+    and field offsets from the host compiler)."""
+    structs = {"lbfgsb_options_f64": _lib.OptionsF64, "lbfgsb_options_f32": _lib.OptionsF32,
+               "lbfgsb_summary_f64": _lib.SummaryF64, "lbfgsb_summary_f32": _lib.SummaryF32,
+               "lbfgsb_callbacks_f64": _lib.CallbacksF64, "lbfgsb_request": _lib.Request,
+               "lbfgsb_response": _lib.Response, "lbfgsb_serve_config": _lib.ServeConfig,
+               "lbfgsb_serve_stats": _lib.ServeStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{_lib.HEADER}"',
+             "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    seen = {}
+    for ln in out.splitlines():
+        cname, field, val = ln.split()
+        seen[(cname, field)] = int(val)
+    for cname, py in structs.items():
+        assert seen[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert seen[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
