@@ -138,34 +138,36 @@ def run_cpu_baseline(n_target: int, n_sample: int, warm: int = 9, timed: int = 1
 
 def run_batched(count: int, plain_sample: int = 32):
     """Batched form on the pentadiagonal box-QP workload (count problems, n
-    log-uniform in [1e3, 2e5], tolerance 1e-6): autotuned shape, one
-    persistent launch per dtype timed with the problems resident in HBM, the
-    fp32 minima against the fp64 ones, and the plain form (one
-    lbfgsb_minimize call per problem) timed on a random sample."""
+    log-uniform in [1e3, 2e5], tolerance 1e-6), the problems generated in
+    their HBM slots: autotuned shape, one graph-replayed launch per dtype
+    timed with the problems resident, the fp32 minima against the fp64 ones,
+    and the plain form (one lbfgsb_minimize call per problem) timed on a
+    random sample."""
     import numpy as np
     import torch
     from lbfgsb_amd import batch as bt
     from lbfgsb_amd import problems as pb
     from lbfgsb_amd import solver
 
-    probs = pb.batched_qps(count)
-    print(f"batched: {count} problems generated", file=sys.stderr, flush=True)
+    specs = pb.batched_qp_specs(count)
     eps, max_iter = 1e-6, 1000
     rng = np.random.default_rng(5)
-    sample = [probs[k] for k in rng.choice(count, size=min(256, count), replace=False)]
-    tuned = bt.autotune(sample, np.float32, eps=eps, max_iter=max_iter)
+    # tuned on the workload itself: a sample's time is set by its largest
+    # problems, the whole batch's by throughput
+    tuned = bt.autotune(specs, np.float32, eps=eps, max_iter=max_iter)
     print(f"batched: autotuned {tuned['threads']} threads x {tuned['blocks']} workgroups, "
-          f"segment {tuned['segment']}",
-          file=sys.stderr, flush=True)
+          f"segment {tuned['segment']}", file=sys.stderr, flush=True)
+    total_n = int(sum(r["n"] for r in specs))
     out = {"workload": f"{count} pentadiagonal box QPs, n log-uniform in [1e3, 2e5], "
-                       f"tolerance 1e-6, m=8",
-           "problems": count, "total_n": int(sum(p.n for p in probs)), "autotune": tuned}
+                       f"tolerance 1e-6, m=8, one graph-replayed persistent launch",
+           "problems": count, "total_n": total_n, "autotune": tuned}
     fvals = {}
     for dtype, name in ((np.float32, "f32"), (np.float64, "f64")):
-        b = bt.Batch(dtype, sum(p.n + 64 for p in probs), count)
+        b = bt.Batch(dtype, sum(r["n"] + 64 for r in specs), count)
         b.configure(tuned["segment"], tuned["max_gang"])
-        for p in probs:
-            if b.add(p, eps=eps, max_iter=max_iter) < 0:
+        b.use_graph(True)
+        for r in specs:
+            if bt.add_any(b, r) < 0:
                 raise RuntimeError("batched problem rejected")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -175,18 +177,21 @@ def run_batched(count: int, plain_sample: int = 32):
         f, it, info, done = b.results()
         st, en = b.clocks()
         b.close()
-        lat = (en - st)[:count]
+        solve = (en - st)[:count]
+        finish = en[:count] - st[:count].min()
         s = 8 if dtype == np.float64 else 4
-        bytes_alg = sum(int(it[k]) * b_alg_per_iter(probs[k].n, s) for k in range(count))
+        bytes_alg = sum(int(it[k]) * b_alg_per_iter(specs[k]["n"], s) for k in range(count))
         fvals[name] = f[:count].copy()
         print(f"batched: {name} {count / wall:.1f} solves/s", file=sys.stderr, flush=True)
         out[name] = {"solves_per_s": count / wall, "seconds": wall, "unfinished": int(left),
                      "info_counts": {str(k): int(v) for k, v in
                                      zip(*np.unique(info[:count], return_counts=True))},
                      "iterations_mean": float(it[:count].mean()),
-                     "solve_time_p50_ms": float(np.percentile(lat, 50) * 1e3),
-                     "solve_time_p95_ms": float(np.percentile(lat, 95) * 1e3),
-                     "solve_time_p99_ms": float(np.percentile(lat, 99) * 1e3),
+                     "solve_time_p50_ms": float(np.percentile(solve, 50) * 1e3),
+                     "solve_time_p95_ms": float(np.percentile(solve, 95) * 1e3),
+                     "solve_time_p99_ms": float(np.percentile(solve, 99) * 1e3),
+                     "completion_p50_ms": float(np.percentile(finish, 50) * 1e3),
+                     "completion_p99_ms": float(np.percentile(finish, 99) * 1e3),
                      "alg_bytes_per_s_GB": bytes_alg / wall / 1e9,
                      "hbm_frac": bytes_alg / wall / 1e9 / HBM_PEAK_GBS}
     rel = np.abs(fvals["f32"] - fvals["f64"]) / np.maximum(np.abs(fvals["f64"]), 1e-300)
@@ -196,7 +201,8 @@ def run_batched(count: int, plain_sample: int = 32):
     idx = rng.choice(count, size=min(plain_sample, count), replace=False)
     t_plain = 0.0
     for k in idx:
-        dp = solver.DeviceProblem.create(probs[k], np.float32)
+        p = pb.penta_qp(specs[k]["n"], seed=specs[k]["seed"])
+        dp = solver.DeviceProblem.create(p, np.float32)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         solver.minimize(dp, eps=eps, max_iter=max_iter)
@@ -207,6 +213,63 @@ def run_batched(count: int, plain_sample: int = 32):
     out["plain_f32_solves_per_s"] = plain_rate
     out["plain_sample"] = int(len(idx))
     out["speedup_vs_plain_f32"] = out["f32"]["solves_per_s"] / plain_rate
+    return out
+
+
+def run_serving(count: int, qps: float, check: int = 64):
+    """Serving form: the first `count` requests of the 20,000-request stream
+    (Poisson arrivals at `qps`, n log-uniform in [1e3, 4e6], the four
+    generators, 70% fp32 / 30% fp64), served in real time with continuous
+    batching under a 200 GB HBM budget.  Then a sample of the fp32 requests
+    is solved again in fp64 to state the objective error of the fp32 answers."""
+    import numpy as np
+    from lbfgsb_amd import batch as bt
+    from lbfgsb_amd import problems as pb
+    from lbfgsb_amd.serve import latency_summary, serve
+
+    reqs = pb.serving_stream(20000, qps=qps)[:count]
+    cfg = dict(budget_bytes=200e9, fp32_share=0.7, max_slots=4096, threads=256, segment=65536,
+               max_gang=64, quantum=20, use_graph=True)
+    print(f"serving: {count} requests at {qps} per second", file=sys.stderr, flush=True)
+    res, st = serve(reqs, **cfg)
+    done = [r for r in res if r["info"] >= 0]
+    out = {"workload": f"first {count} of 20000 requests, Poisson {qps}/s, n log-uniform in "
+                       f"[1e3, 4e6], objectives of the four generators, 70% fp32 / 30% fp64",
+           "offered_qps": qps, "requests": count, "answered": len(done),
+           "solves_per_s": len(done) / st["seconds"], "seconds": st["seconds"],
+           "latency": latency_summary(res), "config": cfg, "stats": st,
+           "info_counts": {str(k): int(v) for k, v in
+                           zip(*np.unique([r["info"] for r in res], return_counts=True))}}
+    print(f"serving: {out['solves_per_s']:.2f} solves/s, latency {out['latency']}",
+          file=sys.stderr, flush=True)
+    # fp32 answers against fp64 solves of the same problems
+    idx = [k for k, q in enumerate(reqs) if q["dtype"] == 0][:check]
+    b = bt.Batch(np.float64, sum(reqs[k]["n"] + 64 for k in idx), len(idx))
+    b.configure(65536, 64)
+    b.use_graph(True)
+    slots = [bt.add_any(b, reqs[k]) for k in idx]
+    b.run()
+    f64, it64, info64, _ = b.results()
+    b.close()
+    rel, rel_conv = [], []
+    for k, s in zip(idx, slots):
+        r = abs(res[k]["f"] - f64[s]) / max(abs(f64[s]), 1.0)
+        rel.append(r)
+        if res[k]["info"] != 5 and info64[s] != 5:
+            rel_conv.append(r)
+    rel, rel_conv = np.array(rel), np.array(rel_conv)
+    kinds = np.array([reqs[k]["kind"] for k in idx])
+    out["f32_vs_f64_by_kind"] = {
+        str(kd): {"count": int((kinds == kd).sum()),
+                  "max": float(rel[kinds == kd].max()) if (kinds == kd).any() else None,
+                  "frac_le_1e-6": float(np.mean(rel[kinds == kd] <= 1e-6))
+                  if (kinds == kd).any() else None} for kd in (1, 2, 3, 4)}
+    out["f32_vs_f64_rel_error"] = {
+        "sample": len(idx), "max": float(rel.max()), "median": float(np.median(rel)),
+        "frac_le_1e-6": float(np.mean(rel <= 1e-6)),
+        "converged_sample": int(rel_conv.size),
+        "converged_max": float(rel_conv.max()) if rel_conv.size else None,
+        "converged_frac_le_1e-6": float(np.mean(rel_conv <= 1e-6)) if rel_conv.size else None}
     return out
 
 
@@ -221,6 +284,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2 ** 22)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-count", type=int, default=4096)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-count", type=int, default=400)
+    ap.add_argument("--serving-qps", type=float, default=10.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -276,6 +342,8 @@ def main():
                                       for k, v in r64["kern"].items()}}
     if not a.no_batched:
         out["batched"] = run_batched(a.batched_count)
+    if not a.no_serving:
+        out["serving"] = run_serving(a.serving_count, a.serving_qps)
     if not a.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline(n, min(a.cpu_sample, n))
     print(json.dumps(out), flush=True)

@@ -175,21 +175,32 @@ TUNE_CANDIDATES = ((256, 2, 0), (512, 1, 0), (512, 1, 131072), (512, 1, 65536),
 MAX_GANG = 64
 
 
+def add_any(b: Batch, item, eps=1e-6, max_iter=1000) -> int:
+    """Admit a Problem (uploaded) or a request dict (generated on the
+    device, with its own tolerance and iteration cap)."""
+    if isinstance(item, Problem):
+        return b.add(item, eps=eps, max_iter=max_iter)
+    return b.add_generated(item["kind"], item["n"], item.get("nx", 0), item.get("ny", 0),
+                           item["seed"], eps=item.get("eps", eps),
+                           max_iter=item.get("max_iteration", max_iter))
+
+
 def autotune(sample, dtype, eps=1e-6, max_iter=1000, candidates=TUNE_CANDIDATES,
-             max_gang=MAX_GANG):
-    """Solve `sample` (a list of Problems drawn from the workload) once per
-    candidate shape and return the fastest as {"threads", "blocks",
+             max_gang=MAX_GANG, graph=True):
+    """Solve `sample` (Problems or request dicts drawn from the workload) once
+    per candidate shape and return the fastest as {"threads", "blocks",
     "segment", "max_gang", ...}.  A shape that cannot hold the widest gang
     on one launch is skipped."""
     torch = _torch()
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    cap = sum(p.n + 64 for p in sample)
+    cap = sum(p.n + 64 if isinstance(p, Problem) else p["n"] + 64 for p in sample)
     timings = []
     for th, per_cu, seg in candidates:
         b = Batch(dtype, cap, len(sample))
         b.configure(seg, max_gang if seg else 1)
+        b.use_graph(graph)
         for p in sample:
-            if b.add(p, eps=eps, max_iter=max_iter) < 0:
+            if add_any(b, p, eps, max_iter) < 0:
                 raise RuntimeError("autotune: problem rejected")
         torch.cuda.synchronize()
         t0 = time.perf_counter()

@@ -192,12 +192,21 @@ def penta_matvec(p: Problem, x: np.ndarray) -> np.ndarray:
     return ax
 
 
+def batched_qp_specs(count: int = 4096, lo: float = 1e3, hi: float = 2e5,
+                     seed: int = SEED_BASE + 3) -> list:
+    """Specs of the batched workload: `count` pentadiagonal box QPs, n
+    log-uniform in [lo, hi] (stream 7 of `seed`), problem k seeded with
+    seed + (k+1)*2^20, tolerance 1e-6, at most 1000 iterations (the request
+    dicts of serving_stream, without arrival times)."""
+    sizes = batch_sizes(count, lo, hi, seed)
+    return [dict(kind=OBJ_PENTA, dtype=0, n=int(n), nx=0, ny=0, seed=seed + ((k + 1) << 20),
+                 eps=1e-6, max_iteration=1000) for k, n in enumerate(sizes)]
+
+
 def batched_qps(count: int = 4096, lo: float = 1e3, hi: float = 2e5,
                 seed: int = SEED_BASE + 3) -> list:
-    """The batched workload: `count` pentadiagonal box QPs, n log-uniform in
-    [lo, hi] (stream 7 of `seed`), problem k seeded with seed + (k+1)*2^20."""
-    sizes = batch_sizes(count, lo, hi, seed)
-    return [penta_qp(int(n), seed=seed + ((k + 1) << 20)) for k, n in enumerate(sizes)]
+    """The batched workload as host problems (see batched_qp_specs)."""
+    return [penta_qp(r["n"], seed=r["seed"]) for r in batched_qp_specs(count, lo, hi, seed)]
 
 
 # Tolerances and iteration caps of the generators' own workloads, used by
