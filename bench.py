@@ -39,14 +39,14 @@ def kernel_bytes_per_elem(s: int) -> dict:
     (col = m = 8, one memory update per iteration; nbd int32, iwhere 1 byte).
       pass_a_fused: read x, x_old, g, g_old, l, u, d (7s), nbd, iwhere, the 14
                     retained W entries (14s); write the new S/Y pair (2s),
-                    iwhere
+                    iwhere and the breakpoint t (s) the Cauchy search selects from
       p4:           read x, g, l, u (4s), nbd, iwhere, W (16s); write z, r
                     (2s), iwhere
       p5a:          read r, z, l, u (4s), nbd, iwhere, W (16s); write d_sub (s)
       p5b:          read x, g, z, d_sub, l, u (6s), nbd, iwhere; write z, d (2s)
       trial:        read z, write x_trial (2s, unit step)
       gd:           read g, d (2s)"""
-    return {"pass_a_fused": 23 * s + 6, "p4": 22 * s + 6, "p5a": 21 * s + 5,
+    return {"pass_a_fused": 24 * s + 6, "p4": 22 * s + 6, "p5a": 21 * s + 5,
             "p5b": 8 * s + 5, "trial": 2 * s, "gd": 2 * s}
 
 

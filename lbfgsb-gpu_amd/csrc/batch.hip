@@ -773,6 +773,7 @@ __device__ __forceinline__ void b_step(int step, double f, const Prob<T>& P, Bat
       }
       if (t0) logic_update_gcp(s, 1, L.red, L.rv[1], L.ri[0]);
       __syncthreads();
+      post_update_block(s);
       build_middle(s);
       break;
     case K_STOP:
@@ -782,6 +783,7 @@ __device__ __forceinline__ void b_step(int step, double f, const Prob<T>& P, Bat
       }
       if (t0) logic_stop_tests(s, L.red, L.rv[0], L.rv[1], L.ri[0]);
       __syncthreads();
+      post_update_block(s);
       build_middle(s);
       break;
     case K_UPD:
@@ -792,6 +794,7 @@ __device__ __forceinline__ void b_step(int step, double f, const Prob<T>& P, Bat
       if (s->upd_done) break;
       if (t0) logic_update_gcp(s, 0, L.red, 0.0, 0);
       __syncthreads();
+      post_update_block(s);
       build_middle(s);
       break;
     case K_GCP:
@@ -802,7 +805,7 @@ __device__ __forceinline__ void b_step(int step, double f, const Prob<T>& P, Bat
       break;
     case K_FORMK:
       if (s->halt) break;
-      if (t0) logic_formk_subsm(s, L.red, s->p4_flagged != 0);
+      logic_formk_subsm(s, L.red, s->p4_flagged != 0);
       break;
     case K_P5A:
       b_p5a(P, L, G);

@@ -130,12 +130,13 @@ __global__ void __launch_bounds__(kBlock)
 k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
          const T* __restrict__ u, const int* __restrict__ nbd,
          uint8_t* __restrict__ iwhere, T* __restrict__ W, const T* __restrict__ d,
-         DevScalars* ds, double* part, double* pmax, double* pmin, long long* pmini) {
+         T* __restrict__ tbrk, DevScalars* ds, double* part, double* pmax, double* pmin,
+         long long* pmini) {
   __shared__ double sh[kWavesPerBlock * PA_COUNT];
   if (ds->halt) return;
   using Op = PassAOp<T, FUSED>;
   Op op;
-  op.init(ds, X, Gv, l, u, nbd, iwhere, W, d);
+  op.init(ds, X, Gv, l, u, nbd, iwhere, W, d, tbrk);
   if constexpr (sizeof(T) == 4) {
     // FP32: two consecutive runs of 256 elements per block step, loads of
     // both issued before either is processed
@@ -371,14 +372,14 @@ void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g,
 template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X,
                    Pair<T> Gv, const T* l, const T* u, const int* nbd,
-                   uint8_t* iwhere, T* W, const T* d, DevScalars* ds, double* part,
-                   double* pmax, double* pmin, long long* pmini) {
+                   uint8_t* iwhere, T* W, const T* d, T* tbrk, DevScalars* ds,
+                   double* part, double* pmax, double* pmin, long long* pmini) {
   if (fused)
     hipLaunchKernelGGL((k_pass_a<T, true>), dim3(G), dim3(kBlock), 0, s, n, X, Gv,
-                       l, u, nbd, iwhere, W, d, ds, part, pmax, pmin, pmini);
+                       l, u, nbd, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini);
   else
     hipLaunchKernelGGL((k_pass_a<T, false>), dim3(G), dim3(kBlock), 0, s, n, X,
-                       Gv, l, u, nbd, iwhere, W, d, ds, part, pmax, pmin, pmini);
+                       Gv, l, u, nbd, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini);
 }
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
@@ -423,7 +424,7 @@ void launch_gd(hipStream_t s, int G, long long n, Pair<T> Gv, const T* d,
                                  const T*, const T*, const int*, double*);          \
   template void launch_pass_a<T>(hipStream_t, int, bool, long long, Pair<T>,      \
                                  Pair<T>, const T*, const T*, const int*,         \
-                                 uint8_t*, T*, const T*, DevScalars*, double*,      \
+                                 uint8_t*, T*, const T*, T*, DevScalars*, double*,  \
                                  double*, double*, long long*);                     \
   template void launch_p4<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,     \
                              const T*, const T*, const int*, uint8_t*, const T*,    \

@@ -145,9 +145,10 @@ __device__ inline void build_middle(DevScalars* ds) {
   }
 }
 
-// matupd's small part + formt (lbfgsbcpu.cpp:1365-1386, 1105-1133).
-__device__ inline void memory_update(DevScalars* ds) {
-  if (!ds->updatd) return;
+// matupd's small part (lbfgsbcpu.cpp:1365-1386); formt follows in
+// formt_block.  Returns whether a pair was added.
+__device__ inline bool memory_update_pre(DevScalars* ds) {
+  if (!ds->updatd) return false;
   const int m = ds->m, col = ds->col;
   ds->rr = ds->pa[PA_RR];
   ds->theta = ds->rr / ds->dr;
@@ -163,89 +164,195 @@ __device__ inline void memory_update(DevScalars* ds) {
   }
   A1(ds->ss, 8, col, col) = ds->stp == 1.0 ? ds->dtd : ds->stp * ds->stp * ds->dtd;
   A1(ds->sy, 8, col, col) = ds->dr;
-  if (!d_formt(ds->wt, ds->sy, ds->ss, col, ds->theta)) reset_memory(ds);
+  return true;
 }
 
-// formk (lbfgsbcpu.cpp:907-1103) from the inner products of k_p4.
-__device__ inline bool formk_device(DevScalars* ds, const double* nc, const double* corr) {
+// formt (lbfgsbcpu.cpp:1105-1133) by the whole workgroup: the entries of
+// T = theta S'S + L D^{-1} L' one per thread (each summed in the reference
+// order), then the Cholesky factor by thread 0 in registers.  A failed
+// factorisation resets the memory (thread 0).
+__device__ inline void formt_block(DevScalars* ds) {
+  const int col = ds->col;
+  const double theta = ds->theta;
+  const int t = threadIdx.x;
+  if (t < 64) {
+    const int i = t / 8 + 1, j = t % 8 + 1;
+    if (i <= j && j <= col) {
+      if (i == 1) {
+        A1(ds->wt, 8, 1, j) = theta * A1(ds->ss, 8, 1, j);
+      } else {
+        const int k1 = i - 1;  // min(i, j) - 1
+        double ddum = 0.0;
+        for (int k = 1; k <= k1; k++)
+          ddum = ddum + A1(ds->sy, 8, i, k) * A1(ds->sy, 8, j, k) / A1(ds->sy, 8, k, k);
+        A1(ds->wt, 8, i, j) = ddum + theta * A1(ds->ss, 8, i, j);
+      }
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    double a[8][8];
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+#pragma unroll
+      for (int c = 0; c < 8; c++) a[r][c] = (r <= c && c < col) ? A1(ds->wt, 8, r + 1, c + 1) : 0.0;
+    const bool ok = d_dpofa_r<8>(a, col);
+    if (ok) {
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+          if (r <= c && c < col) A1(ds->wt, 8, r + 1, c + 1) = a[r][c];
+    } else {
+      // dpofa leaves the factor partly written; the memory is discarded
+      reset_memory(ds);
+    }
+  }
+  __syncthreads();
+}
+
+// formk (lbfgsbcpu.cpp:907-1103) from the inner products of k_p4, by the
+// whole workgroup.  Every entry of wn1 and wn is formed by one thread with
+// the reference's operations in the reference's order; the Cholesky
+// factorisations run on thread 0 in registers and the triangular solves of
+// the off-diagonal block one column per thread.  Returns false (internalinfo
+// set) when a factorisation fails.
+__device__ inline bool formk_block(DevScalars* ds, const double* nc, const double* corr) {
   const int m = ds->m, col = ds->col;
   double* wn1 = ds->wn1;
   double* wn = ds->wn;
   const double theta = ds->theta;
-  int upcl;
-  if (ds->updatd) {
-    if (ds->iupdat > m) {
-      for (int jy = 1; jy <= m - 1; jy++) {
-        const int js = m + jy;
-        for (int i = 0; i < m - jy; i++) A1(wn1, 16, jy + i, jy) = A1(wn1, 16, jy + 1 + i, jy + 1);
-        for (int i = 0; i < m - jy; i++) A1(wn1, 16, js + i, js) = A1(wn1, 16, js + 1 + i, js + 1);
-        for (int i = 0; i < m - 1; i++) A1(wn1, 16, m + 1 + i, jy) = A1(wn1, 16, m + 2 + i, jy + 1);
-      }
+  const int t = threadIdx.x;
+  const bool upd = ds->updatd != 0;
+  const int upcl = upd ? col - 1 : col;
+  if (upd && ds->iupdat > m) {
+    // the ring dropped its oldest pair: the three lower blocks of wn1 move
+    // one step up the diagonal (every destination takes its source's value
+    // from before the move, as the reference's loop order gives)
+    double v = 0.0;
+    bool dest = false;
+    int r = 0, c = 0;
+    if (t < 256) {
+      r = t / 16 + 1;
+      c = t % 16 + 1;
+      const bool yy = c <= m - 1 && r >= c && r <= m - 1;
+      const bool ssb = c >= m + 1 && c <= 2 * m - 1 && r >= c && r <= 2 * m - 1;
+      const bool sy = c <= m - 1 && r >= m + 1 && r <= 2 * m - 1;
+      dest = yy || ssb || sy;
+      if (dest) v = A1(wn1, 16, r + 1, c + 1);
     }
-    for (int jy = 1; jy <= col; jy++) {
+    __syncthreads();
+    if (dest) A1(wn1, 16, r, c) = v;
+    __syncthreads();
+  }
+  if (upd) {
+    if (t < col) {
+      const int jy = t + 1;
       A1(wn1, 16, col, jy) = nc[jy - 1];
       A1(wn1, 16, m + col, m + jy) = nc[8 + jy - 1];
       A1(wn1, 16, m + col, jy) = nc[16 + jy - 1];
     }
-    for (int i = 1; i <= col; i++) A1(wn1, 16, m + i, col) = nc[24 + i - 1];
-    upcl = col - 1;
-  } else {
-    upcl = col;
+    __syncthreads();
+    if (t < col) A1(wn1, 16, m + t + 1, col) = nc[24 + t];
+    __syncthreads();
   }
   if (corr) {
-    for (int iy = 1; iy <= upcl; iy++) {
-      for (int jy = 1; jy <= iy; jy++) {
+    if (t < 64) {
+      const int iy = t / 8 + 1, jy = t % 8 + 1;
+      if (iy <= upcl && jy <= iy) {
         const int q = (iy - 1) * iy / 2 + (jy - 1);
         A1(wn1, 16, iy, jy) = A1(wn1, 16, iy, jy) + corr[q];
         A1(wn1, 16, m + iy, m + jy) = A1(wn1, 16, m + iy, m + jy) - corr[36 + q];
       }
-    }
-    for (int i = 1; i <= upcl; i++) {
-      for (int jy = 1; jy <= upcl; jy++) {
-        const double v = corr[72 + (i - 1) * 8 + (jy - 1)];
-        if (i <= jy)
-          A1(wn1, 16, m + i, jy) = A1(wn1, 16, m + i, jy) + v;
+      if (iy <= upcl && jy <= upcl) {
+        const double v = corr[72 + (iy - 1) * 8 + (jy - 1)];
+        if (iy <= jy)
+          A1(wn1, 16, m + iy, jy) = A1(wn1, 16, m + iy, jy) + v;
         else
-          A1(wn1, 16, m + i, jy) = A1(wn1, 16, m + i, jy) - v;
+          A1(wn1, 16, m + iy, jy) = A1(wn1, 16, m + iy, jy) - v;
       }
     }
+    __syncthreads();
   }
-  for (int iy = 1; iy <= col; iy++) {
-    const int iis = col + iy, is1 = m + iy;
-    for (int jy = 1; jy <= iy; jy++) {
-      const int js = col + jy, js1 = m + jy;
-      A1(wn, 16, jy, iy) = A1(wn1, 16, iy, jy) / theta;
-      A1(wn, 16, js, iis) = A1(wn1, 16, is1, js1) * theta;
+  // wn from wn1, entry by entry
+  if (t < 64) {
+    const int iy = t / 8 + 1, jy = t % 8 + 1;
+    if (iy <= col && jy <= col) {
+      const int iis = col + iy, is1 = m + iy;
+      if (jy <= iy) {
+        double v = A1(wn1, 16, iy, jy) / theta;
+        if (jy == iy) v = v + A1(ds->sy, 8, iy, iy);
+        A1(wn, 16, jy, iy) = v;
+        A1(wn, 16, col + jy, iis) = A1(wn1, 16, is1, m + jy) * theta;
+      }
+      A1(wn, 16, jy, iis) = jy < iy ? -A1(wn1, 16, is1, jy) : A1(wn1, 16, is1, jy);
     }
-    for (int jy = 1; jy <= iy - 1; jy++) A1(wn, 16, jy, iis) = -A1(wn1, 16, is1, jy);
-    for (int jy = iy; jy <= col; jy++) A1(wn, 16, jy, iis) = A1(wn1, 16, is1, jy);
-    A1(wn, 16, iy, iy) = A1(wn, 16, iy, iy) + A1(ds->sy, 8, iy, iy);
   }
-  if (!d_dpofa(wn, 16, col)) {
-    ds->internalinfo = -1;
-    return false;
+  __syncthreads();
+  // LEL': the upper-left block's factor
+  if (t == 0) {
+    double a[8][8];
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+#pragma unroll
+      for (int c = 0; c < 8; c++) a[r][c] = (r <= c && c < col) ? A1(wn, 16, r + 1, c + 1) : 0.0;
+    const bool ok = d_dpofa_r<8>(a, col);
+    if (ok) {
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+          if (r <= c && c < col) A1(wn, 16, r + 1, c + 1) = a[r][c];
+    } else {
+      ds->internalinfo = -1;
+    }
+    ds->blk_flag = ok ? 1 : 0;
   }
-  const int col2 = 2 * col;
-  double* work = ds->tmp;  // 8 doubles
-  for (int js = col + 1; js <= col2; js++) {
-    for (int i = 1; i <= col; i++) work[i - 1] = A1(wn, 16, i, js);
-    d_dtrsl(wn, 16, col, work, 11);
-    for (int i = 1; i <= col; i++) A1(wn, 16, i, js) = work[i - 1];
+  __syncthreads();
+  if (!ds->blk_flag) return false;
+  // L^{-1} (-L_a'+R_z') one column per thread
+  if (t < col) {
+    const int js = col + 1 + t;
+    double b[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) b[i] = i < col ? A1(wn, 16, i + 1, js) : 0.0;
+    d_dtrsl_r<8>(wn, 16, col, b, 11);
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      if (i < col) A1(wn, 16, i + 1, js) = b[i];
   }
-  for (int iis = col + 1; iis <= col2; iis++)
-    for (int js = iis; js <= col2; js++)
+  __syncthreads();
+  if (t < 64) {
+    const int a = t / 8, b = t % 8;
+    if (a <= b && b < col) {
+      const int iis = col + 1 + a, js = col + 1 + b;
       A1(wn, 16, iis, js) = A1(wn, 16, iis, js) +
                             ddot(&A1(wn, 16, 1, iis), 16, &A1(wn, 16, 1, js), 16, col);
-  double* wm = ds->tmp + 16;  // 8 x 8
-  for (int i = 1; i <= col; i++)
-    for (int k = 1; k <= col; k++) A1(wm, 8, i, k) = A1(wn, 16, col + i, col + k);
-  if (!d_dpofa(wm, 8, col)) {
-    ds->internalinfo = -2;
-    return false;
+    }
   }
-  for (int i = 1; i <= col; i++)
-    for (int k = 1; k <= col; k++) A1(wn, 16, col + i, col + k) = A1(wm, 8, i, k);
-  return true;
+  __syncthreads();
+  // the lower-right block's factor
+  if (t == 0) {
+    double a[8][8];
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+#pragma unroll
+      for (int c = 0; c < 8; c++)
+        a[r][c] = (r <= c && c < col) ? A1(wn, 16, col + r + 1, col + c + 1) : 0.0;
+    const bool ok = d_dpofa_r<8>(a, col);
+    if (ok) {
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+          if (r <= c && c < col) A1(wn, 16, col + r + 1, col + c + 1) = a[r][c];
+    } else {
+      ds->internalinfo = -2;
+    }
+    ds->blk_flag = ok ? 1 : 0;
+  }
+  __syncthreads();
+  return ds->blk_flag != 0;
 }
 
 __device__ inline void request_trial(DevScalars* ds) {
@@ -383,9 +490,7 @@ __device__ inline void logic_stop_tests(DevScalars* s, const double* pa, double 
     s->status = ST_CHECKED;
     s->halt = 1;
   } else {
-    memory_update(s);
-    gcp_controller(s);
-    s->upd_done = 1;
+    s->post_update = 1;  // memory update + Cauchy controller (post_update_block)
   }
 }
 
@@ -399,54 +504,86 @@ __device__ inline void logic_update_gcp(DevScalars* s, int reduce, const double*
     for (int v = 0; v < PA_COUNT; v++) s->pa[v] = pa[v];
     s->gcp_bkmin = bk;
     s->gcp_ibkmin = ibk;
+    s->post_update = 2;  // the Cauchy controller only
   } else {
-    memory_update(s);
+    s->post_update = 1;  // memory update + Cauchy controller
   }
-  gcp_controller(s);
-  s->upd_done = 1;
 }
 
-// After the Cauchy-point pass: free/active bookkeeping
-// (lbfgsbcpu.cpp:1143-1161), formk, and the two triangular solves of subsm
-// (lbfgsbcpu.cpp:1459-1471).  r = the pass's sums (P4_* layout).
-__device__ __attribute__((noinline)) void logic_formk_subsm(DevScalars* s, const double* r, bool flagged) {
-  const long long n = s->n;
-  const int col = s->col;
-  if (!s->shortcut) {
-    s->nfree = (int)r[P4_NFREE];
-    if (s->iter > 0 && s->cnstnd) {
-      s->nenter = (int)r[P4_NENTER];
-      s->nleave = (int)r[P4_NLEAVE];
-    } else {
-      s->nenter = 0;
-      s->nleave = 0;
-    }
-    s->ileave = (int)(n + 1 - s->nleave);
-    s->wrk = s->nleave > 0 || s->nenter > 0 || s->updatd;
+// The step a one-thread decision requested in post_update, by the whole
+// workgroup (the stopping tests and logic_update_gcp end with it; the
+// caller synchronises before): the memory update, formt, and the Cauchy
+// controller.
+__device__ inline void post_update_block(DevScalars* s) {
+  const int mode = s->post_update;
+  if (!mode) return;
+  __syncthreads();
+  if (mode == 1) {
+    if (threadIdx.x == 0) s->blk_flag = memory_update_pre(s) ? 1 : 0;
+    __syncthreads();
+    if (s->blk_flag) formt_block(s);
   }
-  s->skip_subsm = (s->nfree == 0 || col == 0);
+  if (threadIdx.x == 0) {
+    gcp_controller(s);
+    s->upd_done = 1;
+    s->post_update = 0;
+  }
+  __syncthreads();
+}
+
+// After the Cauchy-point pass, by the whole workgroup: free/active
+// bookkeeping (lbfgsbcpu.cpp:1143-1161) on thread 0, formk, and the two
+// triangular solves of subsm (lbfgsbcpu.cpp:1459-1471) on thread 0 with the
+// vector in registers.  r = the pass's sums (P4_* layout).  The caller
+// synchronises before.
+__device__ inline void logic_formk_subsm(DevScalars* s, const double* r, bool flagged) {
+  const int col = s->col;
+  if (threadIdx.x == 0) {
+    const long long n = s->n;
+    if (!s->shortcut) {
+      s->nfree = (int)r[P4_NFREE];
+      if (s->iter > 0 && s->cnstnd) {
+        s->nenter = (int)r[P4_NENTER];
+        s->nleave = (int)r[P4_NLEAVE];
+      } else {
+        s->nenter = 0;
+        s->nleave = 0;
+      }
+      s->ileave = (int)(n + 1 - s->nleave);
+      s->wrk = s->nleave > 0 || s->nenter > 0 || s->updatd;
+    }
+    s->skip_subsm = (s->nfree == 0 || col == 0);
+  }
+  __syncthreads();
   if (s->skip_subsm) return;
   bool fail = false;
-  if (s->wrk) fail = !formk_device(s, r + P4_NC, flagged ? r + P4_CORR : nullptr);
-  if (!fail) {
-    const double theta = s->theta;
-    double* wv = s->wv;
-    for (int i = 0; i < col; i++) {
-      wv[i] = r[P4_WV + i];
-      wv[col + i] = theta * r[P4_WV + 8 + i];
+  if (s->wrk) fail = !formk_block(s, r + P4_NC, flagged ? r + P4_CORR : nullptr);
+  if (threadIdx.x == 0) {
+    if (!fail) {
+      const double theta = s->theta;
+      double b[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++)
+        b[q] = q < col ? r[P4_WV + q] : (q < 2 * col ? theta * r[P4_WV + 8 + q - col] : 0.0);
+      int info = d_dtrsl_r<16>(s->wn, 16, 2 * col, b, 11);
+      if (info == 0) {
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+          if (q < col) b[q] = -b[q];
+        info = d_dtrsl_r<16>(s->wn, 16, 2 * col, b, 1);
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++)
+        if (q < 2 * col) s->wv[q] = b[q];
+      fail = info != 0;
     }
-    int info = d_dtrsl(s->wn, 16, 2 * col, wv, 11);
-    if (info == 0) {
-      for (int i = 0; i < col; i++) wv[i] = -wv[i];
-      info = d_dtrsl(s->wn, 16, 2 * col, wv, 1);
+    if (fail) {
+      reset_memory(s);
+      s->status = ST_RESTART;
+      s->halt = 1;
     }
-    fail = info != 0;
   }
-  if (fail) {
-    reset_memory(s);
-    s->status = ST_RESTART;
-    s->halt = 1;
-  }
+  __syncthreads();
 }
 
 // Backtrack length of subsm (lbfgsbcpu.cpp:1484-1538): the smallest

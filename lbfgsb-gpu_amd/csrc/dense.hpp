@@ -28,90 +28,153 @@ __device__ inline double ddot(const double* a, int sa, const double* b, int sb, 
   return r;
 }
 
-// LINPACK dpofa, upper Cholesky in place (lbfgsbcpu.cpp:1876-1908).
-__device__ inline bool d_dpofa(double* a, int lda, int n) {
-  for (int j = 1; j <= n; j++) {
+// ---------------------------------------------------------------------------
+// LINPACK dpofa / dtrsl and bmv for one thread, with the vector (or matrix)
+// being solved in registers: every loop is unrolled to a fixed maximum size
+// and masked by the actual size, so each entry sees the reference's
+// operations in the reference's order without a dependent LDS round trip
+// per operation.
+
+// ALGLIB blocked dot of column j of t (rows 1 .. len) with b[0 .. len).
+template <int NMAX>
+__device__ __forceinline__ double ddot_col_r(const double* __restrict__ t, int ldt, int j,
+                                             const double (&b)[NMAX], int len) {
+  double r = 0.0;
+#pragma unroll
+  for (int blk = 0; blk < NMAX / 4; blk++) {
+    if (4 * blk + 4 <= len) {
+      const int i = 4 * blk + 1;
+      r += A1(t, ldt, i, j) * b[i - 1] + A1(t, ldt, i + 1, j) * b[i] +
+           A1(t, ldt, i + 2, j) * b[i + 1] + A1(t, ldt, i + 3, j) * b[i + 2];
+    }
+  }
+#pragma unroll
+  for (int i = 1; i <= NMAX; i++) {
+    if (i > (len / 4) * 4 && i <= len) r += A1(t, ldt, i, j) * b[i - 1];
+  }
+  return r;
+}
+
+// dtrsl (lbfgsbcpu.cpp:1910-1986) on a register vector b[0 .. n), n <= NMAX.
+template <int NMAX>
+__device__ __forceinline__ int d_dtrsl_r(const double* __restrict__ t, int ldt, int n,
+                                         double (&b)[NMAX], int job) {
+  for (int j = 1; j <= n; j++)
+    if (A1(t, ldt, j, j) == 0.0) return j;
+  if (job == 1) {
+#pragma unroll
+    for (int k = 0; k < NMAX; k++)
+      if (k == n - 1) b[k] = b[k] / A1(t, ldt, n, n);
+#pragma unroll
+    for (int j = NMAX - 1; j >= 1; j--) {
+      if (j <= n - 1) {
+        const double temp = -b[j];
+#pragma unroll
+        for (int i = 1; i <= j; i++) b[i - 1] += temp * A1(t, ldt, i, j + 1);
+        b[j - 1] = b[j - 1] / A1(t, ldt, j, j);
+      }
+    }
+  } else {  // job 11
+    b[0] = b[0] / A1(t, ldt, 1, 1);
+#pragma unroll
+    for (int j = 2; j <= NMAX; j++) {
+      if (j <= n) {
+        const double v = ddot_col_r<NMAX>(t, ldt, j, b, j - 1);
+        b[j - 1] = b[j - 1] - v;
+        b[j - 1] = b[j - 1] / A1(t, ldt, j, j);
+      }
+    }
+  }
+  return 0;
+}
+
+// dpofa (lbfgsbcpu.cpp:1876-1908) on the upper triangle of a register
+// matrix a[r][c] (0-based), n <= NMAX.
+template <int NMAX>
+__device__ __forceinline__ bool d_dpofa_r(double (&a)[NMAX][NMAX], int n) {
+#pragma unroll
+  for (int j = 1; j <= NMAX; j++) {
+    if (j > n) continue;
     double s = 0.0;
-    for (int k = 1; k <= j - 1; k++) {
-      double v = ddot(&A1(a, lda, 1, k), lda, &A1(a, lda, 1, j), lda, k - 1);
-      double t = A1(a, lda, k, j) - v;
-      t = t / A1(a, lda, k, k);
-      A1(a, lda, k, j) = t;
+#pragma unroll
+    for (int k = 1; k < NMAX; k++) {
+      if (k > j - 1) continue;
+      // ddot(a(1:k-1, k), a(1:k-1, j)) in the blocked order
+      double v = 0.0;
+#pragma unroll
+      for (int blk = 0; blk < NMAX / 4; blk++) {
+        if (4 * blk + 4 <= k - 1) {
+          const int i = 4 * blk;
+          v += a[i][k - 1] * a[i][j - 1] + a[i + 1][k - 1] * a[i + 1][j - 1] +
+               a[i + 2][k - 1] * a[i + 2][j - 1] + a[i + 3][k - 1] * a[i + 3][j - 1];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NMAX; i++)
+        if (i >= ((k - 1) / 4) * 4 && i < k - 1) v += a[i][k - 1] * a[i][j - 1];
+      double t = a[k - 1][j - 1] - v;
+      t = t / a[k - 1][k - 1];
+      a[k - 1][j - 1] = t;
       s = s + t * t;
     }
-    s = A1(a, lda, j, j) - s;
+    s = a[j - 1][j - 1] - s;
     if (s <= 0.0) return false;
-    A1(a, lda, j, j) = sqrt(s);
+    a[j - 1][j - 1] = sqrt(s);
   }
   return true;
 }
 
-// LINPACK dtrsl for upper-triangular t: job 1 solves T x = b, job 11 solves
-// T' x = b (lbfgsbcpu.cpp:1910-1986).  Returns 0 or the zero-pivot index.
-__device__ inline int d_dtrsl(const double* t, int ldt, int n, double* b, int job) {
-  for (int j = 1; j <= n; j++)
-    if (A1(t, ldt, j, j) == 0.0) return j;
-  if (job == 1) {
-    b[n - 1] = b[n - 1] / A1(t, ldt, n, n);
-    for (int jj = 2; jj <= n; jj++) {
-      int j = n - jj + 1;
-      double temp = -b[j];
-      for (int i = 1; i <= j; i++) b[i - 1] += temp * A1(t, ldt, i, j + 1);
-      b[j - 1] = b[j - 1] / A1(t, ldt, j, j);
-    }
-  } else {  // job 11
-    b[0] = b[0] / A1(t, ldt, 1, 1);
-    for (int j = 2; j <= n; j++) {
-      double v = ddot(&A1(t, ldt, 1, j), ldt, b, 1, j - 1);
-      b[j - 1] = b[j - 1] - v;
-      b[j - 1] = b[j - 1] / A1(t, ldt, j, j);
-    }
-  }
-  return 0;
-}
-
 // p = M v with the 2col x 2col middle matrix through sy and the Cholesky
-// factor of T in wt (lbfgsbcpu.cpp:517-565).  sy, wt have ld 8.
-__device__ inline int d_bmv(const double* sy, const double* wt, int col, const double* v,
-                            double* p) {
+// factor of T in wt (lbfgsbcpu.cpp:517-565).  sy, wt have ld 8.  The two
+// halves of v and p (Y part, S part) are kept in registers.
+__device__ inline int d_bmv(const double* __restrict__ sy, const double* __restrict__ wt,
+                            int col, const double* v, double* p) {
   if (col == 0) return 0;
-  p[col] = v[col];
-  for (int i = 2; i <= col; i++) {
-    int i2 = col + i;
-    double s = 0.0;
-    for (int k = 1; k <= i - 1; k++) s = s + A1(sy, 8, i, k) * v[k - 1] / A1(sy, 8, k, k);
-    p[i2 - 1] = v[i2 - 1] + s;
+  double vy[8], vs[8], py[8], ps[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    vy[q] = q < col ? v[q] : 0.0;
+    vs[q] = q < col ? v[col + q] : 0.0;
+    py[q] = 0.0;
+    ps[q] = 0.0;
   }
-  // the two triangular solves work in place on p[col .. 2col-1] (no
-  // per-thread temporaries: those would live in scratch memory)
-  int info = d_dtrsl(wt, 8, col, p + col, 11);
-  if (info != 0) return info;
-  for (int i = 1; i <= col; i++) p[i - 1] = v[i - 1] / sqrt(A1(sy, 8, i, i));
-  info = d_dtrsl(wt, 8, col, p + col, 1);
-  if (info != 0) return info;
-  for (int i = 1; i <= col; i++) p[i - 1] = -p[i - 1] / sqrt(A1(sy, 8, i, i));
-  for (int i = 1; i <= col; i++) {
+  ps[0] = vs[0];
+#pragma unroll
+  for (int i = 2; i <= 8; i++) {
+    if (i > col) continue;
     double s = 0.0;
-    for (int k = i + 1; k <= col; k++) s = s + A1(sy, 8, k, i) * p[col + k - 1] / A1(sy, 8, i, i);
-    p[i - 1] = p[i - 1] + s;
+#pragma unroll
+    for (int k = 1; k < 8; k++)
+      if (k <= i - 1) s = s + A1(sy, 8, i, k) * vy[k - 1] / A1(sy, 8, k, k);
+    ps[i - 1] = vs[i - 1] + s;
   }
-  return 0;
-}
-
-// T = theta S'S + L D^{-1} L' and its Cholesky factor (lbfgsbcpu.cpp:1105-1133).
-__device__ inline bool d_formt(double* wt, const double* sy, const double* ss, int col,
-                               double theta) {
-  for (int j = 1; j <= col; j++) A1(wt, 8, 1, j) = theta * A1(ss, 8, 1, j);
-  for (int i = 2; i <= col; i++) {
-    for (int j = i; j <= col; j++) {
-      int k1 = (i < j ? i : j) - 1;
-      double ddum = 0.0;
-      for (int k = 1; k <= k1; k++)
-        ddum = ddum + A1(sy, 8, i, k) * A1(sy, 8, j, k) / A1(sy, 8, k, k);
-      A1(wt, 8, i, j) = ddum + theta * A1(ss, 8, i, j);
+  int info = d_dtrsl_r<8>(wt, 8, col, ps, 11);
+  if (info != 0) return info;
+#pragma unroll
+  for (int i = 1; i <= 8; i++)
+    if (i <= col) py[i - 1] = vy[i - 1] / sqrt(A1(sy, 8, i, i));
+  info = d_dtrsl_r<8>(wt, 8, col, ps, 1);
+  if (info != 0) return info;
+#pragma unroll
+  for (int i = 1; i <= 8; i++)
+    if (i <= col) py[i - 1] = -py[i - 1] / sqrt(A1(sy, 8, i, i));
+#pragma unroll
+  for (int i = 1; i <= 8; i++) {
+    if (i > col) continue;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 2; k <= 8; k++)
+      if (k >= i + 1 && k <= col) s = s + A1(sy, 8, k, i) * ps[k - 1] / A1(sy, 8, i, i);
+    py[i - 1] = py[i - 1] + s;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    if (q < col) {
+      p[q] = py[q];
+      p[col + q] = ps[q];
     }
   }
-  return d_dpofa(wt, 8, col);
+  return 0;
 }
 
 __device__ inline double dmax_(double a, double b) { return a > b ? a : b; }

@@ -60,6 +60,7 @@ struct Engine final : EngineBase {
   const T* u;
   const int* nbd;
   T *W = nullptr, *d = nullptr, *z = nullptr, *r = nullptr, *dsub = nullptr;
+  T* tbrk = nullptr;  // breakpoint t per variable, written by pass A
   uint8_t* iwhere = nullptr;
   DevScalars* ds = nullptr;
   DevScalars* hs = nullptr;
@@ -106,7 +107,7 @@ struct Engine final : EngineBase {
     const size_t nn = (size_t)n;
     if (dmalloc(X[1], nn) || dmalloc(Gr[0], nn) || dmalloc(Gr[1], nn)) return -1;
     if (dmalloc(W, wsize(nn)) || dmalloc(d, nn) || dmalloc(z, nn) || dmalloc(r, nn) ||
-        dmalloc(dsub, nn) || dmalloc(iwhere, nn))
+        dmalloc(dsub, nn) || dmalloc(tbrk, nn) || dmalloc(iwhere, nn))
       return -1;
     HIPCHK(hipMemsetAsync(W, 0, wsize(nn) * sizeof(T), st));
     if (dmalloc(ds, 1)) return -1;
@@ -175,7 +176,7 @@ struct Engine final : EngineBase {
     hipStreamSynchronize(st);
     drain_events();
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
-    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, z, r, dsub};
+    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, z, r, dsub, tbrk};
     for (T* p : bufs) hipFree(p);
     hipFree(iwhere);
     hipFree(ds);
@@ -264,19 +265,19 @@ struct Engine final : EngineBase {
         launch_resume(st, ds, 0);
         break;
       case K_PASSA_SA:
-        launch_pass_a<T>(st, G, false, n, dX, dG, l, u, nbd, iwhere, W, d, ds, part, pmax, pmin,
-                         pmini);
+        launch_pass_a<T>(st, G, false, n, dX, dG, l, u, nbd, iwhere, W, d, tbrk, ds, part, pmax,
+                         pmin, pmini);
         break;
       case K_UPD_REDUCE:
         launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0);
         launch_update_gcp(st, ds, part, pmin, pmini, G, 1);
         break;
       case K_PASSA_FUSED:
-        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, nbd, iwhere, W, d, ds, part, pmax, pmin,
-                         pmini);
+        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, nbd, iwhere, W, d, tbrk, ds, part, pmax,
+                         pmin, pmini);
         break;
       case K_GCP:
-        launch_gcp_select<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, ca, n);
+        launch_gcp_select<T>(st, G, n, tbrk, ds, ca, n);
         launch_gcp_round_small<T>(st, ca, dG, W, ds);
         break;
       case K_STOP:
@@ -333,7 +334,7 @@ struct Engine final : EngineBase {
       // the previous kernel halted the chain: clear it for the round kernels
       launch_resume(st, ds, 0);
       if (hs->gcp_need_select) {
-        launch_gcp_select<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, ca, n);
+        launch_gcp_select<T>(st, G, n, tbrk, ds, ca, n);
         HIPCHK(hipGetLastError());
         continue;  // size the round with the count just selected
       }
@@ -348,7 +349,18 @@ struct Engine final : EngineBase {
   }
 
   int run(int program, int start_step, int stop_step, double f) override {
-    const std::vector<int> steps = program_steps(program);
+    std::vector<int> steps = program_steps(program);
+    if (!hs->has_custom_stop && program != PROG_AFTER_CHECK && start_step != K_UPD &&
+        stop_step != K_UPD) {
+      // without a customised stopping test, the stopping tests already did
+      // the memory update and the Cauchy controller (or ended the chain):
+      // the update step would find nothing to do
+      for (size_t k = 0; k + 1 < steps.size(); k++)
+        if (steps[k] == K_UPD && steps[k + 1] == K_GCP && k > 0 && steps[k - 1] == K_STOP) {
+          steps.erase(steps.begin() + k);
+          break;
+        }
+    }
     size_t i = 0;
     if (start_step >= 0 && start_step != K_END) {
       while (i < steps.size() && steps[i] != start_step) i++;

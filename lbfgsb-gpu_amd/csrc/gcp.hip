@@ -16,8 +16,9 @@
 // (4 col + 1)-vectors [d w, t d w, d^2], and test every k in parallel; the
 // first k that stops ends the search.  The first range is [0, 2 dtm_0) with
 // dtm_0 the minimiser of the model before any breakpoint; later ranges grow
-// by 4x.  The candidates are recomputed from x, g, l, u (no breakpoint list
-// is kept by the hot pass), and round 0 runs entirely on the device with its
+// by 4x.  Pass A leaves every variable's breakpoint t (+inf for none) in an
+// n-vector, so a selection reads one value per variable, and round 0 runs
+// entirely on the device with its
 // size read on the device when one workgroup can sort it (2048 candidates);
 // larger or later rounds are
 // sized by the host.
@@ -29,13 +30,9 @@ namespace lbfgsb {
 // entries beyond `cap` are not written).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_gcp_select(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
-             const T* __restrict__ u, const int* __restrict__ nbd,
-             const uint8_t* __restrict__ iwhere, DevScalars* ds, Cand* cand, long long cap) {
+k_gcp_select(long long n, const T* __restrict__ tbrk, DevScalars* ds, Cand* cand,
+             long long cap) {
   if (!ds->gcp_pending) return;
-  const int cur = ds->phase & 1;
-  const T* __restrict__ x = X[cur];
-  const T* __restrict__ g = Gv[cur];
   const double lo = ds->gcp_tau_lo, hi = ds->gcp_tau_hi;
   const int lane = threadIdx.x & 63;
   // no block reads the flag; the host reads it after the launch
@@ -43,14 +40,9 @@ k_gcp_select(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
   for (long long i0 = blockIdx.x * (long long)kBlock; i0 < n;
        i0 += (long long)gridDim.x * kBlock) {
     const long long i = i0 + threadIdx.x;
-    bool take = false;
-    T tb = 0;
-    if (i < n) {
-      if (breakpoint_of(x[i], g[i], l[i], u[i], nbd[i], iw_get(iwhere[i]), tb)) {
-        const double t = (double)tb;
-        take = t >= lo && t < hi;
-      }
-    }
+    const T tb = i < n ? tbrk[i] : (T)__builtin_inf();
+    const double t = (double)tb;
+    const bool take = t >= lo && t < hi;  // +inf (no breakpoint) is never taken
     const unsigned long long mask = __ballot(take);
     if (mask) {
       unsigned long long base = 0;
@@ -323,11 +315,9 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
 
 // ---------------------------------------------------------------------------
 template <typename T>
-void launch_gcp_select(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
-                       const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
-                       DevScalars* ds, Cand* ca, long long buf_cap) {
-  hipLaunchKernelGGL(k_gcp_select<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd, iwhere,
-                     ds, ca, buf_cap);
+void launch_gcp_select(hipStream_t s, int G, long long n, const T* tbrk, DevScalars* ds,
+                       Cand* ca, long long buf_cap) {
+  hipLaunchKernelGGL(k_gcp_select<T>, dim3(G), dim3(kBlock), 0, s, n, tbrk, ds, ca, buf_cap);
 }
 
 // Sort + scan + round end for at most `cap` candidates (the device count
@@ -369,9 +359,8 @@ void launch_gcp_round_small(hipStream_t s, const Cand* ca, Pair<T> Gv, const T* 
 #define LBFGSB_INST_GCP(T)                                                                  \
   template void launch_gcp_round_small<T>(hipStream_t, const Cand*, Pair<T>, const T*,   \
                                           DevScalars*);                                    \
-  template void launch_gcp_select<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
-                                     const T*, const T*, const int*, const uint8_t*,       \
-                                     DevScalars*, Cand*, long long);                       \
+  template void launch_gcp_select<T>(hipStream_t, int, long long, const T*, DevScalars*, \
+                                     Cand*, long long);                                    \
   template void launch_gcp_sortscan<T>(hipStream_t, Pair<T>, const T*, DevScalars*,      \
                                        Cand*, Cand*, long long, const GcpScratch&);
 

@@ -27,10 +27,12 @@ void launch_active(hipStream_t s, int G, long long n, T* x, const T* l, const T*
 template <typename T>
 void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g, const T* l,
                    const T* u, const int* nbd, double* pmax);
+// tbrk receives every variable's breakpoint t (+inf: none) for the
+// Cauchy search's selections.
 template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X, Pair<T> Gv,
                    const T* l, const T* u, const int* nbd, uint8_t* iwhere, T* W, const T* d,
-                   DevScalars* ds, double* part, double* pmax, double* pmin,
+                   T* tbrk, DevScalars* ds, double* part, double* pmax, double* pmin,
                    long long* pmini);
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
@@ -80,9 +82,8 @@ struct GcpScratch {
   double* last_state;
 };
 template <typename T>
-void launch_gcp_select(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
-                       const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
-                       DevScalars* ds, Cand* ca, long long buf_cap);
+void launch_gcp_select(hipStream_t s, int G, long long n, const T* tbrk, DevScalars* ds,
+                       Cand* ca, long long buf_cap);
 template <typename T>
 void launch_gcp_round_small(hipStream_t s, const Cand* ca, Pair<T> Gv, const T* W,
                             DevScalars* ds);

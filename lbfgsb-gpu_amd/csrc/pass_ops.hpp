@@ -31,6 +31,7 @@ struct PassAOp {
   uint8_t* __restrict__ iwhere;
   T* __restrict__ W;
   const T* __restrict__ d;
+  T* __restrict__ tout;  // breakpoint t of every variable (+inf: none), or null
   int col, itail;
   bool upd, stp1;
   T stp;
@@ -50,8 +51,9 @@ struct PassAOp {
 
   __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
                               const T* u_, const int* nbd_, uint8_t* iwhere_, T* W_,
-                              const T* d_) {
+                              const T* d_, T* tout_ = nullptr) {
     const int cur = ds->phase & 1;
+    tout = tout_;
     x = X[cur];
     g = Gv[cur];
     xold = X[cur ^ 1];
@@ -150,6 +152,9 @@ struct PassAOp {
       }
     }
     iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb));
+    // the Cauchy search selects its candidates from these (the same t as
+    // breakpoint_of computes from x, g, l, u and the new iwhere)
+    if (tout) tout[i] = isbrk ? tbrk : (T)__builtin_inf();
   }
   __device__ __forceinline__ void finish() {
     if (upd) {

@@ -3,8 +3,9 @@
 // on the device (lbfgsbcpu.cpp:279-462), so that a whole iteration can be
 // enqueued without a host round trip.  One launch = one block of 256
 // threads.  Each kernel copies the device state block into LDS, runs the
-// scalar logic there (thread 0) and writes it back, so the dependent chains
-// of the small dense algebra see LDS rather than HBM latency.
+// scalar logic there and writes it back: the decisions on thread 0, the
+// small dense algebra shared out over the workgroup where its entries are
+// independent and kept in registers where it is a dependent chain.
 #include "common.hpp"
 #include "dense.hpp"
 #include "control.hpp"
@@ -164,18 +165,21 @@ k_rowsum(const double* part, int G, int v0, DevScalars* ds, int p4_rows) {
   }
 }
 
-__global__ void s_resume(DevScalars* ds, int reset_bp) {
+__global__ void __launch_bounds__(kBlock)
+s_resume(DevScalars* ds, int reset_bp) {
   if (threadIdx.x == 0) logic_resume(ds, reset_bp);
 }
 
-__global__ void s_setup_errclb(DevScalars* ds, const double* part, int G) {
+__global__ void __launch_bounds__(kBlock)
+s_setup_errclb(DevScalars* ds, const double* part, int G) {
   __shared__ double r[2];
   reduce_rows(part, G, 0, 2, r);
   __syncthreads();
   if (threadIdx.x == 0) logic_setup_errclb(ds, r[0], r[1]);
 }
 
-__global__ void s_setup_active(DevScalars* ds, const double* part, int G) {
+__global__ void __launch_bounds__(kBlock)
+s_setup_active(DevScalars* ds, const double* part, int G) {
   __shared__ double r[3];
   reduce_rows(part, G, 0, 3, r);
   __syncthreads();
@@ -183,7 +187,8 @@ __global__ void s_setup_active(DevScalars* ds, const double* part, int G) {
 }
 
 // First projected-gradient test after the initial evaluation.
-__global__ void s_setup_projgr(DevScalars* ds, const double* pmax, int G, double f) {
+__global__ void __launch_bounds__(kBlock)
+s_setup_projgr(DevScalars* ds, const double* pmax, int G, double f) {
   __shared__ double sh[kWavesPerBlock];
   double m = reduce_max(pmax, G, sh);
   if (threadIdx.x == 0) logic_setup_projgr(ds, m, f);
@@ -192,7 +197,8 @@ __global__ void s_setup_projgr(DevScalars* ds, const double* pmax, int G, double
 // Stopping tests after an accepted step (lbfgsbcpu.cpp:394-429); without a
 // customised stopping callback this kernel also runs the memory update and
 // the next Cauchy controller.
-__global__ void s_stop_tests(DevScalars* ds, const double* part, const double* pmax,
+__global__ void __launch_bounds__(kBlock)
+s_stop_tests(DevScalars* ds, const double* part, const double* pmax,
                              const double* pmin, const long long* pmini, int G) {
   __shared__ DevScalars sds;
   __shared__ double sh[kWavesPerBlock];
@@ -211,13 +217,15 @@ __global__ void s_stop_tests(DevScalars* ds, const double* part, const double* p
   // s->red holds the row sums of the pass-A partials (k_rowsum)
   if (threadIdx.x == 0) logic_stop_tests(s, s->red, sbg, bk, ibk);
   __syncthreads();
+  post_update_block(s);
   build_middle(s);
   ds_store(ds, s);
 }
 
 // Memory update + Cauchy controller.  `reduce` selects the standalone path
 // (iteration start / restart) that reduces the classification partials here.
-__global__ void s_update_gcp(DevScalars* ds, const double* part, const double* pmin,
+__global__ void __launch_bounds__(kBlock)
+s_update_gcp(DevScalars* ds, const double* part, const double* pmin,
                              const long long* pmini, int G, int reduce) {
   __shared__ DevScalars sds;
   __shared__ double sh[kWavesPerBlock];
@@ -235,13 +243,15 @@ __global__ void s_update_gcp(DevScalars* ds, const double* part, const double* p
   (void)part;
   if (threadIdx.x == 0) logic_update_gcp(s, reduce, s->red, bk, ibk);
   __syncthreads();
+  post_update_block(s);
   build_middle(s);
   ds_store(ds, s);
 }
 
 // After k_p4: free/active bookkeeping (lbfgsbcpu.cpp:1143-1161), formk,
 // and the two triangular solves of subsm (lbfgsbcpu.cpp:1459-1471).
-__global__ void s_formk_subsm(DevScalars* ds, const double* part, int G) {
+__global__ void __launch_bounds__(kBlock)
+s_formk_subsm(DevScalars* ds, const double* part, int G) {
   __shared__ DevScalars sds;
   __shared__ int flags[1];
   if (ds->halt) return;
@@ -252,12 +262,13 @@ __global__ void s_formk_subsm(DevScalars* ds, const double* part, int G) {
   const double* r = s->red;  // row sums of the k_p4 partials (k_rowsum)
   if (threadIdx.x == 0) flags[0] = s->p4_flagged;
   __syncthreads();
-  if (threadIdx.x == 0) logic_formk_subsm(s, r, flags[0] != 0);
+  logic_formk_subsm(s, r, flags[0] != 0);
   ds_store(ds, s);
 }
 
 // Backtrack length of subsm (lbfgsbcpu.cpp:1484-1538).
-__global__ void s_alpha(DevScalars* ds, const double* pmin, const long long* pmini, int G) {
+__global__ void __launch_bounds__(kBlock)
+s_alpha(DevScalars* ds, const double* pmin, const long long* pmini, int G) {
   __shared__ double sh[kWavesPerBlock];
   __shared__ long long shi[kWavesPerBlock];
   if (ds->halt || ds->skip_subsm) return;
@@ -268,7 +279,8 @@ __global__ void s_alpha(DevScalars* ds, const double* pmin, const long long* pmi
 }
 
 // First call of lnsrlb for an iteration (lbfgsbcpu.cpp:1263-1339).
-__global__ void s_ls_init(DevScalars* ds, const double* part, const double* pmin, int G) {
+__global__ void __launch_bounds__(kBlock)
+s_ls_init(DevScalars* ds, const double* part, const double* pmin, int G) {
   __shared__ DevScalars sds;
   __shared__ double r[P5_COUNT];
   __shared__ double sh[kWavesPerBlock];
@@ -284,7 +296,8 @@ __global__ void s_ls_init(DevScalars* ds, const double* part, const double* pmin
 
 // Later calls of lnsrlb, after an evaluation with value f
 // (lbfgsbcpu.cpp:1312-1339 and the loop exit at :352).
-__global__ void s_ls_step(DevScalars* ds, const double* part, int G, double f) {
+__global__ void __launch_bounds__(kBlock)
+s_ls_step(DevScalars* ds, const double* part, int G, double f) {
   __shared__ DevScalars sds;
   __shared__ double r[1];
   reduce_rows(part, G, 0, 1, r);

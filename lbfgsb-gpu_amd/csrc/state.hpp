@@ -167,6 +167,10 @@ struct DevScalars {
   int p4_flagged;    // k_p4 saw entering/leaving variables
   int gcp_rounds_run;  // statistics: Cauchy searches that needed sorted rounds
   long long gcp_passed_total;  // statistics: breakpoints passed in those
+  // workgroup-wide steps requested by the one-thread decisions: 1 = memory
+  // update then the Cauchy controller, 2 = the controller only
+  int post_update;
+  int blk_flag;  // a workgroup-wide step's outcome, handed between its phases
 
   // ---- resumable solve of the batched form: set up done, next program of
   // the chain and the function value that program takes
