@@ -39,7 +39,7 @@ __device__ inline void set_done(DevScalars* ds, int info) {
 // Cauchy-search controller (lbfgsbcpu.cpp:620-724 and the first step of the
 // breakpoint loop :725-747).  Thread 0.  Sets gcp_pending when the sorted
 // breakpoint rounds must run.
-__device__ inline void gcp_controller(DevScalars* ds) {
+__device__ __forceinline__ void gcp_controller(DevScalars* ds) {
   ds->gcp_pending = 0;
   ds->p4_flagged = 0;
   for (;;) {

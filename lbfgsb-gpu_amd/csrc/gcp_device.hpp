@@ -120,7 +120,7 @@ __device__ inline bool stop_test(const DevScalars* ds, const GcpConst& gc,
   return dtm < t_next - t_prev;
 }
 
-__device__ inline void gcp_done(DevScalars* ds) {
+__device__ __forceinline__ void gcp_done(DevScalars* ds) {
   const int col = ds->col;
   if (col > 0 && d_bmv(ds->sy, ds->wt, col, ds->c, ds->mc) != 0) ds->internalinfo = -8;
   ds->nintol += ds->nint;
@@ -133,7 +133,7 @@ __device__ inline void gcp_done(DevScalars* ds) {
 // search stops (its exclusive state in `st`), or -1 when every candidate
 // passed (then `st` is the inclusive total).  last_state = exclusive state
 // before the last candidate.
-__device__ inline void round_conclude(DevScalars* ds, const GcpConst& gc, const Cand* cand, long long K,
+__device__ __forceinline__ void round_conclude(DevScalars* ds, const GcpConst& gc, const Cand* cand, long long K,
                                long long stop_pos, const double* stv, const double* last_state) {
   const int col = ds->col;
   // the state stays where the caller keeps it (LDS or global), read in place
