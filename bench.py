@@ -38,16 +38,17 @@ def kernel_bytes_per_elem(s: int) -> dict:
     """Algorithmic HBM bytes per element of each bulk pass at steady state
     (col = m = 8, one memory update per iteration; nbd int32, iwhere 1 byte).
       pass_a_fused: read x, x_old, g, g_old, l, u, d (7s), nbd, iwhere, the 14
-                    retained W entries (14s); write the new S/Y pair (2s),
-                    iwhere and the breakpoint t (s) the Cauchy search selects from
-      p4:           read x, g, l, u (4s), nbd, iwhere, W (16s); write z, r
-                    (2s), iwhere
-      p5a:          read r, z, l, u (4s), nbd, iwhere, W (16s); write d_sub (s)
-      p5b:          read x, g, z, d_sub, l, u (6s), nbd, iwhere; write z, d (2s)
+                    retained W entries (14s); write the new S/Y pair (2s) and
+                    the breakpoint t (s) the Cauchy search selects from
+      p4:           read x, g, l, u (4s), nbd, iwhere, W (16s); write iwhere
+      p5a:          read x, g, l, u (4s), nbd, iwhere, W (16s); write d_sub (s)
+      p5b:          read x, g, d_sub, l, u (5s), nbd, iwhere; write z, d (2s)
       trial:        read z, write x_trial (2s, unit step)
-      gd:           read g, d (2s)"""
-    return {"pass_a_fused": 24 * s + 6, "p4": 22 * s + 6, "p5a": 21 * s + 5,
-            "p5b": 8 * s + 5, "trial": 2 * s, "gd": 2 * s}
+      gd:           read g, d (2s)
+    x_cp and r are recomputed per element by the passes that need them (from
+    x, g, l, u, the class and W) instead of being stored and read back."""
+    return {"pass_a_fused": 24 * s + 5, "p4": 20 * s + 6, "p5a": 21 * s + 5,
+            "p5b": 7 * s + 5, "trial": 2 * s, "gd": 2 * s}
 
 
 # HBM bytes per launch at steady state for the default workload (n = 16,777,216,
@@ -59,8 +60,9 @@ MEASURED_TRAFFIC_16M_FP32 = {"pass_a_fused": 1649638490, "p4": 1589936986, "p5a"
 
 
 def b_alg_per_iter(n: int, s: int) -> float:
-    """Canonical compulsory bytes of one iteration, n (76 s + 32)."""
-    return float(n) * (76 * s + 32)
+    """Algorithmic bytes of one steady-state iteration with one trial point:
+    the sum of kernel_bytes_per_elem, n (76 s + 21)."""
+    return float(n) * sum(kernel_bytes_per_elem(s).values())
 
 
 def run_gpu(n: int, f64: bool, warmup: int, steps: int):

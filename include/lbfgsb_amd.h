@@ -113,11 +113,12 @@ void* lbfgsb_engine_g(lbfgsb_engine* e, int which);
 /* Named scalar of the device state (see DevScalars), as double. */
 int lbfgsb_engine_get_scalar(lbfgsb_engine* e, const char* name, double* value);
 int lbfgsb_engine_set_scalar(lbfgsb_engine* e, const char* name, double value);
-/* Named array copied to / from host memory.  Names: x, g, xb, gb, z, d, r,
- * dsub (bulk type, n); iwhere (int32, n); iwbits (uint8, n); ws, wy (bulk
- * type, n x m row-major); index, indx2 (int32, n, 1-based as the
- * reference); sy, ss, wt (m x m), wn, snd (2m x 2m), p, c, mc, wv, pa
- * (double). */
+/* Named array copied to / from host memory.  Names: x, g, xb, gb, z, d,
+ * dsub (bulk type, n); xcp (bulk type, n; read only: the Cauchy point
+ * computed from the current state, which the iteration does not store);
+ * iwhere (int32, n); iwbits (uint8, n); ws, wy (bulk type, n x m
+ * row-major); index, indx2 (int32, n, 1-based as the reference); sy, ss,
+ * wt (m x m), wn, snd (2m x 2m), p, c, mc, wv, pa (double). */
 int lbfgsb_engine_get_array(lbfgsb_engine* e, const char* name, void* out, long long count);
 int lbfgsb_engine_set_array(lbfgsb_engine* e, const char* name, const void* in,
                             long long count);

@@ -65,7 +65,6 @@ struct Prob {
   T* W;
   T* d;
   T* z;
-  T* r;
   T* dsub;
   Cand* ca;
   Cand* cb;
@@ -370,7 +369,7 @@ template <typename T, bool GO>
 __device__ __forceinline__ void b_p4(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt) return;
   P4Op<T> op;
-  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.W, P.z, P.r);
+  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.W);
   const long long hi = P.hi;
   for (long long i0 = P.lo; i0 < hi; i0 += blockDim.x) {
     const long long i = i0 + threadIdx.x;
@@ -402,7 +401,7 @@ template <typename T, bool GO>
 __device__ __forceinline__ void b_p5a(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt || L.s.skip_subsm) return;
   P5aOp<T> op;
-  op.init(&L.s, P.l, P.u, P.nbd, P.iwhere, P.W, P.z, P.r, P.dsub);
+  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.W, P.dsub);
   for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     typename P5aOp<T>::In e;
     op.load(e, i);
@@ -887,7 +886,6 @@ __device__ __forceinline__ void solve_one(const BatchProb& bp, const BatchBufs<T
   P.W = B.W + bp.woff;
   P.d = B.d + bp.off;
   P.z = B.z + bp.off;
-  P.r = B.r + bp.off;
   P.dsub = B.dsub + bp.off;
   P.ca = B.ca + bp.off;
   P.cb = B.cb + bp.off;

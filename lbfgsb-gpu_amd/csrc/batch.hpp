@@ -60,7 +60,6 @@ struct BatchBufs {
   T* W;
   T* d;
   T* z;
-  T* r;
   T* dsub;
   Cand* ca;  // breakpoint candidates, n per problem
   Cand* cb;

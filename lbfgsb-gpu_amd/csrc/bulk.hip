@@ -138,17 +138,25 @@ k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
   Op op;
   op.init(ds, X, Gv, l, u, nbd, iwhere, W, d, tbrk);
   if constexpr (sizeof(T) == 4) {
-    // FP32: two consecutive runs of 256 elements per block step, loads of
-    // both issued before either is processed
-    for (long long i0 = blockIdx.x * (long long)(2 * kBlock); i0 < n;
-         i0 += (long long)gridDim.x * (2 * kBlock)) {
-      const long long ia = i0 + threadIdx.x, ib = ia + kBlock;
-      typename Op::In ea, eb;
+    // FP32: two consecutive runs of 256 elements per block step, software
+    // pipelined: the loads of one run are in flight while the other run is
+    // processed
+    const long long step = (long long)gridDim.x * (2 * kBlock);
+    long long i0 = blockIdx.x * (long long)(2 * kBlock);
+    typename Op::In ea, eb;
+    if (i0 < n) {
+      const long long ia = i0 + threadIdx.x;
       op.load(ea, ia < n ? ia : n - 1);
+    }
+    for (; i0 < n; i0 += step) {
+      const long long ia = i0 + threadIdx.x, ib = ia + kBlock, na = ia + step;
       op.load(eb, ib < n ? ib : n - 1);
       op.keep(ea);
-      op.keep(eb);
       if (ia < n) op.element(ia, ea);
+      // unconditional (clamped past the end, unused there): a branch around
+      // it would make the wait for eb's loads cover these too
+      op.load(ea, na < n ? na : n - 1);
+      op.keep(eb);
       if (ib < n) op.element(ib, eb);
     }
   } else {
@@ -192,27 +200,31 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
      const T* __restrict__ u, const int* __restrict__ nbd,
-     uint8_t* __restrict__ iwhere, const T* __restrict__ W, T* __restrict__ z,
-     T* __restrict__ r, DevScalars* ds, double* part) {
+     uint8_t* __restrict__ iwhere, const T* __restrict__ W, DevScalars* ds, double* part) {
   __shared__ double sh[kWavesPerBlock * P4_CORR];
   __shared__ double s_corr[kWavesPerBlock][136];
   if (ds->halt) return;
   P4Op<T> op;
-  op.init(ds, X, Gv, l, u, nbd, iwhere, W, z, r);
+  op.init(ds, X, Gv, l, u, nbd, iwhere, W);
   if constexpr (sizeof(T) == 4) {
-    // FP32: each block takes two consecutive runs of 256 elements per step;
-    // the loads of both are issued before either is processed, so every wave
-    // has two elements' worth of requests in flight (the register footprint
-    // leaves two waves per SIMD)
-    for (long long i0 = blockIdx.x * (long long)(2 * kBlock); i0 < n;
-         i0 += (long long)gridDim.x * (2 * kBlock)) {
-      const long long ia = i0 + threadIdx.x, ib = ia + kBlock;
-      typename P4Op<T>::In ea, eb;
+    // FP32: each block takes two consecutive runs of 256 elements per step,
+    // software pipelined: one run's loads are in flight while the other run
+    // is processed (the register footprint leaves two waves per SIMD, so a
+    // wave cannot rely on others to cover its arithmetic)
+    const long long step = (long long)gridDim.x * (2 * kBlock);
+    long long i0 = blockIdx.x * (long long)(2 * kBlock);
+    typename P4Op<T>::In ea, eb;
+    if (i0 < n) {
+      const long long ia = i0 + threadIdx.x;
       op.load(ea, ia < n ? ia : n - 1);
+    }
+    for (; i0 < n; i0 += step) {
+      const long long ia = i0 + threadIdx.x, ib = ia + kBlock, na = ia + step;
       op.load(eb, ib < n ? ib : n - 1);
       op.keep(ea);
-      op.keep(eb);
       op.corr_step(ia, ia < n ? op.element(ia, ea) : 0);
+      op.load(ea, na < n ? na : n - 1);  // unconditional, as in k_pass_a
+      op.keep(eb);
       op.corr_step(ib, ib < n ? op.element(ib, eb) : 0);
     }
   } else {
@@ -242,17 +254,37 @@ k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
   }
 }
 
+// x_cp of every variable from the current x, g, class and search outcome:
+// what k_p4, k_p5a and k_p5b compute per element (an inspection aid; the
+// iteration stores no x_cp).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_cauchy_point(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
+               const T* __restrict__ u, const int* __restrict__ nbd,
+               const uint8_t* __restrict__ iwhere, const DevScalars* ds, T* __restrict__ xcp) {
+  CauchyOutcome<T> co;
+  co.init(ds);
+  const int cur = ds->phase & 1;
+  const T* __restrict__ x = X[cur];
+  const T* __restrict__ g = Gv[cur];
+  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    int iw = iw_get(iwhere[i]);
+    xcp[i] = co.point(i, x[i], g[i], l[i], u[i], nbd[i], iw);
+  }
+}
+
 // k_p5a: subspace Newton direction over the free variables and the
 // step-to-box ratio (lbfgsbcpu.cpp:1472-1516).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_p5a(long long n, const T* __restrict__ l, const T* __restrict__ u,
+k_p5a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __restrict__ u,
       const int* __restrict__ nbd, const uint8_t* __restrict__ iwhere,
-      const T* __restrict__ W, const T* __restrict__ z, const T* __restrict__ r,
-      T* __restrict__ dsub, DevScalars* ds, double* pmin, long long* pmini) {
+      const T* __restrict__ W, T* __restrict__ dsub, DevScalars* ds, double* pmin,
+      long long* pmini) {
   if (ds->halt || ds->skip_subsm) return;
   P5aOp<T> op;
-  op.init(ds, l, u, nbd, iwhere, W, z, r, dsub);
+  op.init(ds, X, Gv, l, u, nbd, iwhere, W, dsub);
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
     typename P5aOp<T>::In e;
@@ -384,17 +416,23 @@ void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X,
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
                const T* l, const T* u, const int* nbd, uint8_t* iwhere,
-               const T* W, T* z, T* r, DevScalars* ds, double* part) {
+               const T* W, DevScalars* ds, double* part) {
   hipLaunchKernelGGL(k_p4<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd,
-                     iwhere, W, z, r, ds, part);
+                     iwhere, W, ds, part);
 }
 template <typename T>
-void launch_p5a(hipStream_t s, int G, long long n, const T* l, const T* u,
-                const int* nbd, const uint8_t* iwhere, const T* W, const T* z,
-                const T* r, T* dsub, DevScalars* ds, double* pmin,
-                long long* pmini) {
-  hipLaunchKernelGGL(k_p5a<T>, dim3(G), dim3(kBlock), 0, s, n, l, u, nbd, iwhere,
-                     W, z, r, dsub, ds, pmin, pmini);
+void launch_p5a(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
+                const T* u, const int* nbd, const uint8_t* iwhere, const T* W, T* dsub,
+                DevScalars* ds, double* pmin, long long* pmini) {
+  hipLaunchKernelGGL(k_p5a<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd, iwhere,
+                     W, dsub, ds, pmin, pmini);
+}
+template <typename T>
+void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
+                         const T* u, const int* nbd, const uint8_t* iwhere,
+                         const DevScalars* ds, T* xcp) {
+  hipLaunchKernelGGL(k_cauchy_point<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd, iwhere,
+                     ds, xcp);
 }
 template <typename T>
 void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
@@ -428,10 +466,13 @@ void launch_gd(hipStream_t s, int G, long long n, Pair<T> Gv, const T* d,
                                  double*, double*, long long*);                     \
   template void launch_p4<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,     \
                              const T*, const T*, const int*, uint8_t*, const T*,    \
-                             T*, T*, DevScalars*, double*);                         \
-  template void launch_p5a<T>(hipStream_t, int, long long, const T*, const T*,      \
-                              const int*, const uint8_t*, const T*, const T*,       \
+                             DevScalars*, double*);                                 \
+  template void launch_p5a<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,      \
+                              const T*, const T*, const int*, const uint8_t*,       \
                               const T*, T*, DevScalars*, double*, long long*);      \
+  template void launch_cauchy_point<T>(hipStream_t, int, long long, Pair<T>,      \
+                                       Pair<T>, const T*, const T*, const int*,    \
+                                       const uint8_t*, const DevScalars*, T*);      \
   template void launch_p5b<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
                               const T*, const T*, const int*, const uint8_t*, T*,   \
                               const T*, T*, DevScalars*, double*, double*);         \

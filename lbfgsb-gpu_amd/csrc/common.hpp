@@ -195,25 +195,41 @@ __device__ inline RingReads ring_reads(int head, int m, int col, int skip_slot) 
   return r;
 }
 
+// The retained Y and S entries of element i in ring order, as loaded: an
+// unused position holds the entry of the first used one.  The 16 loads are
+// unconditional and issued ahead of any use, so a wave has the whole W row
+// in flight at once (one 256 B segment per slot in FP32) instead of waiting
+// on each pair.
+template <typename T>
+__device__ inline void load_ring_raw(const T* __restrict__ W, long long i, const RingReads& rr,
+                                     T (&yv)[kMaxM], T (&sv)[kMaxM]) {
+  const T* __restrict__ row = W + widx((size_t)i, 0);
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    yv[j] = row[rr.rs[j] * kTile];
+    sv[j] = row[(8 + rr.rs[j]) * kTile];
+  }
+}
+
+// Zeros the unused positions of a row from load_ring_raw.  Kept apart from
+// the loads so that a loop can issue the next element's loads and process
+// the current one before waiting for them.
+template <typename T>
+__device__ inline void mask_ring(const RingReads& rr, T (&yv)[kMaxM], T (&sv)[kMaxM]) {
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    yv[j] = rr.use[j] ? yv[j] : (T)0;
+    sv[j] = rr.use[j] ? sv[j] : (T)0;
+  }
+}
+
 // The retained Y and S entries of element i in ring order (0 for unused
-// positions).  The 16 loads are unconditional and issued ahead of any use,
-// so a wave has the whole W row in flight at once (one 256 B segment per
-// slot in FP32) instead of waiting on each pair.
+// positions).
 template <typename T>
 __device__ inline void load_ring(const T* __restrict__ W, long long i, const RingReads& rr,
                                  T (&yv)[kMaxM], T (&sv)[kMaxM]) {
-  const T* __restrict__ row = W + widx((size_t)i, 0);
-  T ly[kMaxM], ls[kMaxM];
-#pragma unroll
-  for (int j = 0; j < kMaxM; j++) {
-    ly[j] = row[rr.rs[j] * kTile];
-    ls[j] = row[(8 + rr.rs[j]) * kTile];
-  }
-#pragma unroll
-  for (int j = 0; j < kMaxM; j++) {
-    yv[j] = rr.use[j] ? ly[j] : (T)0;
-    sv[j] = rr.use[j] ? ls[j] : (T)0;
-  }
+  load_ring_raw(W, i, rr, yv, sv);
+  mask_ring(rr, yv, sv);
 }
 
 template <typename T>

@@ -59,8 +59,9 @@ struct Engine final : EngineBase {
   const T* l;
   const T* u;
   const int* nbd;
-  T *W = nullptr, *d = nullptr, *z = nullptr, *r = nullptr, *dsub = nullptr;
+  T *W = nullptr, *d = nullptr, *z = nullptr, *dsub = nullptr;
   T* tbrk = nullptr;  // breakpoint t per variable, written by pass A
+  T* xcp = nullptr;   // x_cp on request (named array "xcp"); the iteration keeps none
   uint8_t* iwhere = nullptr;
   DevScalars* ds = nullptr;
   DevScalars* hs = nullptr;
@@ -106,8 +107,8 @@ struct Engine final : EngineBase {
   int alloc() {
     const size_t nn = (size_t)n;
     if (dmalloc(X[1], nn) || dmalloc(Gr[0], nn) || dmalloc(Gr[1], nn)) return -1;
-    if (dmalloc(W, wsize(nn)) || dmalloc(d, nn) || dmalloc(z, nn) || dmalloc(r, nn) ||
-        dmalloc(dsub, nn) || dmalloc(tbrk, nn) || dmalloc(iwhere, nn))
+    if (dmalloc(W, wsize(nn)) || dmalloc(d, nn) || dmalloc(z, nn) || dmalloc(dsub, nn) ||
+        dmalloc(tbrk, nn) || dmalloc(iwhere, nn))
       return -1;
     HIPCHK(hipMemsetAsync(W, 0, wsize(nn) * sizeof(T), st));
     if (dmalloc(ds, 1)) return -1;
@@ -176,7 +177,7 @@ struct Engine final : EngineBase {
     hipStreamSynchronize(st);
     drain_events();
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
-    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, z, r, dsub, tbrk};
+    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, z, dsub, tbrk, xcp};
     for (T* p : bufs) hipFree(p);
     hipFree(iwhere);
     hipFree(ds);
@@ -288,14 +289,14 @@ struct Engine final : EngineBase {
         launch_update_gcp(st, ds, part, pmin, pmini, G, 0);
         break;
       case K_P4:
-        launch_p4<T>(st, G, n, dX, dG, l, u, nbd, iwhere, W, z, r, ds, part);
+        launch_p4<T>(st, G, n, dX, dG, l, u, nbd, iwhere, W, ds, part);
         break;
       case K_FORMK:
         launch_rowsum(st, part, G, 0, P4_COUNT, ds, 1);
         launch_formk_subsm(st, ds, part, G);
         break;
       case K_P5A:
-        launch_p5a<T>(st, G, n, l, u, nbd, iwhere, W, z, r, dsub, ds, pmin, pmini);
+        launch_p5a<T>(st, G, n, dX, dG, l, u, nbd, iwhere, W, dsub, ds, pmin, pmini);
         break;
       case K_ALPHA:
         launch_alpha(st, ds, pmin, pmini, G);
@@ -421,7 +422,14 @@ struct Engine final : EngineBase {
     if (name == "gb") return dcopy(Gr[c ^ 1], nn * sizeof(T));
     if (name == "z") return dcopy(z, nn * sizeof(T));
     if (name == "d") return dcopy(d, nn * sizeof(T));
-    if (name == "r") return dcopy(r, nn * sizeof(T));
+    if (name == "xcp") {
+      // the Cauchy point as the passes after the search compute it, from
+      // the current x, g, class and search outcome
+      if (!xcp && dmalloc(xcp, nn)) return -1;
+      launch_cauchy_point<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, xcp);
+      HIPCHK(hipGetLastError());
+      return dcopy(xcp, nn * sizeof(T));
+    }
     if (name == "dsub") return dcopy(dsub, nn * sizeof(T));
     if (name == "iwbits") return dcopy(iwhere, nn);
     if (name == "iwhere") {

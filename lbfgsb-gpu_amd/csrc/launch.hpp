@@ -36,12 +36,17 @@ void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X, Pai
                    long long* pmini);
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-               const T* u, const int* nbd, uint8_t* iwhere, const T* W, T* z, T* r,
-               DevScalars* ds, double* part);
+               const T* u, const int* nbd, uint8_t* iwhere, const T* W, DevScalars* ds,
+               double* part);
 template <typename T>
-void launch_p5a(hipStream_t s, int G, long long n, const T* l, const T* u, const int* nbd,
-                const uint8_t* iwhere, const T* W, const T* z, const T* r, T* dsub,
+void launch_p5a(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
+                const T* u, const int* nbd, const uint8_t* iwhere, const T* W, T* dsub,
                 DevScalars* ds, double* pmin, long long* pmini);
+// x_cp of every variable from the current state (inspection only).
+template <typename T>
+void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
+                         const T* u, const int* nbd, const uint8_t* iwhere,
+                         const DevScalars* ds, T* xcp);
 template <typename T>
 void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                 const T* u, const int* nbd, const uint8_t* iwhere, T* z, const T* dsub, T* d,

@@ -88,12 +88,13 @@ struct PassAOp {
     e.xo = FUSED ? xold[i] : (T)0;
     e.go = upd ? gold[i] : (T)0;
     e.dd = upd ? d[i] : (T)0;
-    load_ring(W, i, rr, e.yv, e.sv);
+    load_ring_raw(W, i, rr, e.yv, e.sv);
   }
   __device__ __forceinline__ void keep(const In& e) const {
     keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.xo, e.go, e.dd, e.yv, e.sv);
   }
   __device__ __forceinline__ void element(long long i, In& e) {
+    mask_ring(rr, e.yv, e.sv);
     const T xi = e.x, gi = e.g, li = e.l, ui = e.u;
     const int nb = e.nb;
     const uint8_t iwb = e.iwb;
@@ -111,7 +112,7 @@ struct PassAOp {
         W[widx(i, 8 + itail)] = ss;
       }
     }
-    // Unused positions hold zeros (load_ring), and with an update the
+    // Unused positions hold zeros (mask_ring), and with an update the
     // newest position col - 1 (slot itail, not read) is zero too: the sums
     // run over all positions without guards, and the new pair's share of p
     // goes to pny / pns (added at position col - 1 by finish()).
@@ -151,10 +152,16 @@ struct PassAOp {
         if (fabs(neggi) > 0) acc[PA_UNBND] = 1.0;  // an OR, kept 0/1 through the sum
       }
     }
-    iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb));
-    // the Cauchy search selects its candidates from these (the same t as
-    // breakpoint_of computes from x, g, l, u and the new iwhere)
-    if (tout) tout[i] = isbrk ? tbrk : (T)__builtin_inf();
+    if (tout) {
+      // the Cauchy search selects its candidates from these (the same t as
+      // breakpoint_of computes from x, g, l, u and the new class); the
+      // class itself is not stored: the Cauchy-point pass reclassifies from
+      // the stored one (classification is idempotent), so this pass writes
+      // one stream less
+      tout[i] = isbrk ? tbrk : (T)__builtin_inf();
+    } else {
+      iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb));
+    }
   }
   __device__ __forceinline__ void finish() {
     if (upd) {
@@ -168,116 +175,41 @@ struct PassAOp {
 };
 
 // ---------------------------------------------------------------------------
-// Cauchy-point pass: x_cp from the search's outcome, free / entering /
-// leaving sets, the new formk column, the signed entering/leaving Gram
-// pairs, r = -theta (z - x) - g + W M c and W'r over the free set
-// (lbfgsbcpu.cpp:705-817, 819-869, 907-1054, 1135-1173, 1446-1458).
+// The Cauchy point of one variable from the search's outcome
+// (lbfgsbcpu.cpp:705-817) and the reduced gradient there
+// (lbfgsbcpu.cpp:1446-1458).  The classification is idempotent and the
+// outcome (mode, tsum, last passed breakpoint, M c, theta) is fixed from the
+// end of the search to the line search, so the Cauchy-point pass and the
+// subspace passes after it compute the same x_cp and r from x, g, l, u, the
+// stored class and W, bit for bit: neither vector is stored.
 template <typename T>
-struct P4Op {
-  static constexpr int kAcc = P4_CORR;  // 3 counts, 32 new-column sums, 16 W'r
-  const T* __restrict__ x;
-  const T* __restrict__ g;
-  const T* __restrict__ l;
-  const T* __restrict__ u;
-  const int* __restrict__ nbd;
-  uint8_t* __restrict__ iwhere;
-  const T* __restrict__ W;
-  T* __restrict__ z;
-  T* __restrict__ r;
-  DevScalars* ds;
-  int col, mode;
-  bool shortcut, track, newcol;
+struct CauchyOutcome {
+  int mode;
+  bool shortcut;
   T tsum;
-  double t_last, theta;
+  double t_last;
   long long i_last;
-  RingReads rr;
+  double theta;
   double mcy[kMaxM], mcs[kMaxM];
-  double acc[kAcc];
-  // signed Gram pairs over entering (+1) / leaving (-1) variables: lane L of
-  // every wave owns pairs L, L+64, L+128 (< 136); wa/wb are the W columns
-  // (0..7 = Y slots, 8..15 = S slots) of the pair's two factors
-  double corr[3];
-  int wa[3], wb[3];
-  bool pact[3];
 
-  struct In {
-    T x, g, l, u;
-    int nb;
-    uint8_t iwb;
-    T yv[kMaxM], sv[kMaxM];
-    T yn, sn;  // the newest pair again (a cache hit): no select over positions
-  };
-
-  __device__ __forceinline__ void init(DevScalars* ds_, Pair<T> X, Pair<T> Gv, const T* l_, const T* u_,
-                              const int* nbd_, uint8_t* iwhere_, const T* W_, T* z_, T* r_) {
-    ds = ds_;
-    const int cur = ds->phase & 1;
-    x = X[cur];
-    g = Gv[cur];
-    l = l_;
-    u = u_;
-    nbd = nbd_;
-    iwhere = iwhere_;
-    W = W_;
-    z = z_;
-    r = r_;
-    const int m = ds->m, head = ds->head;
-    col = ds->col;
+  __device__ __forceinline__ void init(const DevScalars* ds) {
     shortcut = ds->shortcut;
     mode = ds->gcp_mode;
     tsum = (T)ds->gcp_tsum;
     t_last = ds->gcp_last_t;
     i_last = ds->gcp_last_idx;
-    track = ds->iter > 0 && ds->cnstnd && !shortcut;
-    newcol = ds->updatd && col > 0;
     theta = ds->theta;
-    const int upcl = ds->updatd ? col - 1 : col;
-    rr = ring_reads(head, m, col, -1);
+    const int col = ds->col;
 #pragma unroll
     for (int j = 0; j < kMaxM; j++) {
       mcy[j] = j < col ? ds->mc[j] : 0.0;
       mcs[j] = j < col ? theta * ds->mc[col + j] : 0.0;
     }
-#pragma unroll
-    for (int v = 0; v < kAcc; v++) acc[v] = 0.0;
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-      corr[q] = 0.0;
-      const int t = lane + 64 * q;
-      int cat = 0, a = 0, b = 0;
-      if (t < 136) corr_pair(t, cat, a, b);
-      pact[q] = t < 136 && a < upcl && b < upcl;
-      int sa = head + a, sb = head + b;
-      if (sa >= m) sa -= m;
-      if (sb >= m) sb -= m;
-      wa[q] = cat == 0 ? sa : 8 + sa;
-      wb[q] = cat == 1 ? 8 + sb : sb;
-    }
   }
-  __device__ __forceinline__ void load(In& e, long long i) const {
-    e.x = x[i];
-    e.g = g[i];
-    e.l = l[i];
-    e.u = u[i];
-    e.nb = nbd[i];
-    e.iwb = iwhere[i];
-    load_ring(W, i, rr, e.yv, e.sv);
-    const T* __restrict__ row = W + widx((size_t)i, 0);
-    e.yn = row[rr.newest * kTile];
-    e.sn = row[(8 + rr.newest) * kTile];
-  }
-  __device__ __forceinline__ void keep(const In& e) const {
-    keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.yv, e.sv, e.yn, e.sn);
-  }
-  // Returns +1 / -1 for an entering / leaving variable, 0 otherwise.
-  __device__ __forceinline__ int element(long long i, const In& e) {
-    int flag = 0;
-    const T xi = e.x, gi = e.g, li = e.l, ui = e.u;
-    const int nb = e.nb;
-    const uint8_t iwb = e.iwb;
-    const bool wasfree = iw_wasfree(iwb);
-    int iw = iw_get(iwb);
+  // x_cp of variable i; iw: the stored class in, the class after the
+  // search out (unchanged under the unconstrained shortcut).
+  __device__ __forceinline__ T point(long long i, T xi, T gi, T li, T ui, int nb,
+                                     int& iw) const {
     T zi = xi;
     if (!shortcut) {
       T neggi, tb;
@@ -303,6 +235,122 @@ struct P4Op {
         }
       }
     }
+    return zi;
+  }
+  // r_i = -theta (x_cp - x) - g + (W M c)_i in FP64 (-g under the shortcut),
+  // over a masked row (unused positions zero); callers round it to T.
+  __device__ __forceinline__ double reduced(T xi, T gi, T zi, const T (&yv)[kMaxM],
+                                            const T (&sv)[kMaxM]) const {
+    double ri = -(double)gi;
+    if (!shortcut) {
+      ri = -theta * (double)(zi - xi) - (double)gi;
+#pragma unroll
+      for (int j = 0; j < kMaxM; j++) {
+        ri = __builtin_fma((double)yv[j], mcy[j], ri);
+        ri = __builtin_fma((double)sv[j], mcs[j], ri);
+      }
+    }
+    return ri;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Cauchy-point pass: x_cp from the search's outcome, free / entering /
+// leaving sets, the new formk column, the signed entering/leaving Gram
+// pairs, r = -theta (z - x) - g + W M c and W'r over the free set
+// (lbfgsbcpu.cpp:705-817, 819-869, 907-1054, 1135-1173, 1446-1458).
+template <typename T>
+struct P4Op {
+  static constexpr int kAcc = P4_CORR;  // 3 counts, 32 new-column sums, 16 W'r
+  const T* __restrict__ x;
+  const T* __restrict__ g;
+  const T* __restrict__ l;
+  const T* __restrict__ u;
+  const int* __restrict__ nbd;
+  uint8_t* __restrict__ iwhere;
+  const T* __restrict__ W;
+  DevScalars* ds;
+  int col;
+  bool shortcut, track, newcol;
+  CauchyOutcome<T> co;
+  RingReads rr;
+  double acc[kAcc];
+  // signed Gram pairs over entering (+1) / leaving (-1) variables: lane L of
+  // every wave owns pairs L, L+64, L+128 (< 136); wa/wb are the W columns
+  // (0..7 = Y slots, 8..15 = S slots) of the pair's two factors
+  double corr[3];
+  int wa[3], wb[3];
+  bool pact[3];
+
+  struct In {
+    T x, g, l, u;
+    int nb;
+    uint8_t iwb;
+    T yv[kMaxM], sv[kMaxM];
+    T yn, sn;  // the newest pair again (a cache hit): no select over positions
+  };
+
+  __device__ __forceinline__ void init(DevScalars* ds_, Pair<T> X, Pair<T> Gv, const T* l_, const T* u_,
+                              const int* nbd_, uint8_t* iwhere_, const T* W_) {
+    ds = ds_;
+    const int cur = ds->phase & 1;
+    x = X[cur];
+    g = Gv[cur];
+    l = l_;
+    u = u_;
+    nbd = nbd_;
+    iwhere = iwhere_;
+    W = W_;
+    const int m = ds->m, head = ds->head;
+    col = ds->col;
+    shortcut = ds->shortcut;
+    co.init(ds);
+    track = ds->iter > 0 && ds->cnstnd && !shortcut;
+    newcol = ds->updatd && col > 0;
+    const int upcl = ds->updatd ? col - 1 : col;
+    rr = ring_reads(head, m, col, -1);
+#pragma unroll
+    for (int v = 0; v < kAcc; v++) acc[v] = 0.0;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      corr[q] = 0.0;
+      const int t = lane + 64 * q;
+      int cat = 0, a = 0, b = 0;
+      if (t < 136) corr_pair(t, cat, a, b);
+      pact[q] = t < 136 && a < upcl && b < upcl;
+      int sa = head + a, sb = head + b;
+      if (sa >= m) sa -= m;
+      if (sb >= m) sb -= m;
+      wa[q] = cat == 0 ? sa : 8 + sa;
+      wb[q] = cat == 1 ? 8 + sb : sb;
+    }
+  }
+  __device__ __forceinline__ void load(In& e, long long i) const {
+    e.x = x[i];
+    e.g = g[i];
+    e.l = l[i];
+    e.u = u[i];
+    e.nb = nbd[i];
+    e.iwb = iwhere[i];
+    load_ring_raw(W, i, rr, e.yv, e.sv);
+    const T* __restrict__ row = W + widx((size_t)i, 0);
+    e.yn = row[rr.newest * kTile];
+    e.sn = row[(8 + rr.newest) * kTile];
+  }
+  __device__ __forceinline__ void keep(const In& e) const {
+    keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.yv, e.sv, e.yn, e.sn);
+  }
+  // Returns +1 / -1 for an entering / leaving variable, 0 otherwise.
+  __device__ __forceinline__ int element(long long i, In& e) {
+    mask_ring(rr, e.yv, e.sv);
+    int flag = 0;
+    const T xi = e.x, gi = e.g, li = e.l, ui = e.u;
+    const int nb = e.nb;
+    const uint8_t iwb = e.iwb;
+    const bool wasfree = iw_wasfree(iwb);
+    int iw = iw_get(iwb);
+    const T zi = co.point(i, xi, gi, li, ui, nb, iw);
     const bool freenow = iw <= 0;
     if (!shortcut) {
       acc[P4_NFREE] += freenow ? 1.0 : 0.0;
@@ -317,9 +365,8 @@ struct P4Op {
         }
       }
       iwhere[i] = iw_make(iw, freenow, wasfree);
-      z[i] = zi;
     }
-    // Positions j >= col hold zeros (load_ring), so the sums below run over
+    // Positions j >= col hold zeros (mask_ring), so the sums below run over
     // all kMaxM positions without guards: unused accumulators collect exact
     // zeros and are never read.
     if (col > 0) {
@@ -337,17 +384,7 @@ struct P4Op {
         }
       }
       // r over the free set; 0 elsewhere, so W'r needs no branch
-      double ri = -(double)gi;
-      if (!shortcut) {
-        ri = -theta * (double)(zi - xi) - (double)gi;
-#pragma unroll
-        for (int j = 0; j < kMaxM; j++) {
-          ri = __builtin_fma((double)e.yv[j], mcy[j], ri);
-          ri = __builtin_fma((double)e.sv[j], mcs[j], ri);
-        }
-      }
-      const T rt = freenow ? (T)ri : (T)0;
-      if (freenow) r[i] = rt;
+      const T rt = freenow ? (T)co.reduced(xi, gi, zi, e.yv, e.sv) : (T)0;
 #pragma unroll
       for (int j = 0; j < kMaxM; j++) {
         acc_prod(acc[P4_WV + j], e.yv[j], rt);
@@ -384,43 +421,46 @@ struct P4Op {
 // (lbfgsbcpu.cpp:1472-1516).
 template <typename T>
 struct P5aOp {
+  const T* __restrict__ x;
+  const T* __restrict__ g;
   const T* __restrict__ l;
   const T* __restrict__ u;
   const int* __restrict__ nbd;
   const uint8_t* __restrict__ iwhere;
   const T* __restrict__ W;
-  const T* __restrict__ z;
-  const T* __restrict__ r;
   T* __restrict__ dsub;
   int col;
   bool shortcut;
   double theta;
+  CauchyOutcome<T> co;
   RingReads rr;
   double wy[kMaxM], wsv[kMaxM];
   double amin;
   long long ai;
 
   struct In {
-    T r, z, l, u;
+    T x, g, l, u;
     int nb;
     uint8_t iwb;
     T yv[kMaxM], sv[kMaxM];
   };
 
-  __device__ __forceinline__ void init(const DevScalars* ds, const T* l_, const T* u_, const int* nbd_,
-                              const uint8_t* iwhere_, const T* W_, const T* z_, const T* r_,
-                              T* dsub_) {
+  __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
+                                       const T* u_, const int* nbd_, const uint8_t* iwhere_,
+                                       const T* W_, T* dsub_) {
+    const int cur = ds->phase & 1;
+    x = X[cur];
+    g = Gv[cur];
     l = l_;
     u = u_;
     nbd = nbd_;
     iwhere = iwhere_;
     W = W_;
-    z = z_;
-    r = r_;
     dsub = dsub_;
     col = ds->col;
     shortcut = ds->shortcut;
     theta = ds->theta;
+    co.init(ds);
 #pragma unroll
     for (int j = 0; j < kMaxM; j++) {
       wy[j] = j < col ? ds->wv[j] / theta : 0.0;
@@ -432,20 +472,25 @@ struct P5aOp {
   }
   __device__ __forceinline__ void load(In& e, long long i) const {
     e.iwb = iwhere[i];
-    e.r = r[i];
-    e.z = z[i];
+    e.x = x[i];
+    e.g = g[i];
     e.l = l[i];
     e.u = u[i];
     e.nb = nbd[i];
-    load_ring(W, i, rr, e.yv, e.sv);
+    load_ring_raw(W, i, rr, e.yv, e.sv);
   }
   __device__ __forceinline__ void keep(const In& e) const {
-    keep_loads(e.r, e.z, e.l, e.u, e.nb, e.iwb, e.yv, e.sv);
+    keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.yv, e.sv);
   }
-  __device__ __forceinline__ void element(long long i, const In& e) {
-    const bool freenow = shortcut || iw_get(e.iwb) <= 0;
+  __device__ __forceinline__ void element(long long i, In& e) {
+    int iw = iw_get(e.iwb);
+    const bool freenow = shortcut || iw <= 0;
     if (!freenow) return;
-    double di = (double)e.r;
+    mask_ring(rr, e.yv, e.sv);
+    // x_cp and r as the Cauchy-point pass formed them (r rounded to T)
+    const T zi = co.point(i, e.x, e.g, e.l, e.u, e.nb, iw);
+    const T rt = (T)co.reduced(e.x, e.g, zi, e.yv, e.sv);
+    double di = (double)rt;
 #pragma unroll
     for (int j = 0; j < kMaxM; j++)
       if (j < col) di += (double)e.yv[j] * wy[j] + (double)e.sv[j] * wsv[j];
@@ -455,13 +500,13 @@ struct P5aOp {
     if (nb != 0) {
       double cand = 2.0;
       if (dk < 0 && nb <= 2) {
-        const T t2 = e.l - e.z;
+        const T t2 = e.l - zi;
         if (t2 >= 0)
           cand = 0.0;
         else if (dk < t2)
           cand = (double)(t2 / dk);
       } else if (dk > 0 && nb >= 2) {
-        const T t2 = e.u - e.z;
+        const T t2 = e.u - zi;
         if (t2 <= 0)
           cand = 0.0;
         else if (dk > t2)
@@ -493,11 +538,12 @@ struct P5bOp {
   bool sub, shortcut, pin, want_stpmx;
   T alpha;
   long long ibd;
+  CauchyOutcome<T> co;
   double acc[P5_COUNT];
   double smin;
 
   struct In {
-    T x, g, l, u, z, dsub;
+    T x, g, l, u, dsub;
     uint8_t iwb;
     int nb;
   };
@@ -520,6 +566,7 @@ struct P5bOp {
     alpha = (T)ds->alpha;
     pin = ds->alpha < 1.0;
     ibd = ds->ibd;
+    co.init(ds);
     want_stpmx = ds->cnstnd && ds->iter > 0;
     acc[0] = acc[1] = 0.0;
     smin = 1.0e10;
@@ -529,19 +576,20 @@ struct P5bOp {
     e.g = g[i];
     e.l = l[i];
     e.u = u[i];
-    e.z = shortcut ? (T)0 : z[i];
     e.iwb = iwhere[i];
     e.dsub = sub ? dsub[i] : (T)0;
     e.nb = nbd[i];
   }
   __device__ __forceinline__ void keep(const In& e) const {
-    keep_loads(e.x, e.g, e.l, e.u, e.z, e.iwb, e.dsub, e.nb);
+    keep_loads(e.x, e.g, e.l, e.u, e.iwb, e.dsub, e.nb);
   }
   __device__ __forceinline__ void element(long long i, const In& e) {
     const T big = (T)1.0e10;
     const T xi = e.x;
-    T zi = shortcut ? xi : e.z;
-    if (sub && (shortcut || iw_get(e.iwb) <= 0)) {
+    int iw = iw_get(e.iwb);
+    const bool freenow = shortcut || iw <= 0;
+    T zi = co.point(i, xi, e.g, e.l, e.u, e.nb, iw);  // x_cp
+    if (sub && freenow) {
       T dk = e.dsub;
       if (pin && i == ibd) {
         if (dk > 0) {
@@ -553,10 +601,8 @@ struct P5bOp {
         }
       }
       zi = zi + alpha * dk;
-      z[i] = zi;
-    } else if (shortcut) {
-      z[i] = zi;
     }
+    z[i] = zi;  // the subspace point (x_cp where there is no subspace step)
     const T di = zi - xi;
     d[i] = di;
     acc[P5_DTD] += (double)di * (double)di;
