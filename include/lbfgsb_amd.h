@@ -139,6 +139,10 @@ int lbfgsb_engine_solve_f32(lbfgsb_engine* e, const lbfgsb_callbacks_f32* cb,
  * once the iteration count reaches from_iter (-1 = off; counters reset).
  * step 16 = the sorted breakpoint rounds of the Cauchy search. */
 void lbfgsb_engine_set_profiling(lbfgsb_engine* e, int from_iter);
+/* The same for the steps whose bit (1 << step) is set in step_mask only.
+ * Each timed step costs two event records on the stream (a few us each),
+ * so timing every step slows the iteration; timing one step does not. */
+void lbfgsb_engine_set_profiling_steps(lbfgsb_engine* e, int from_iter, unsigned step_mask);
 int lbfgsb_engine_step_time(lbfgsb_engine* e, int step, double* total_ms, long long* count);
 
 /* ---- Built-in objectives (the callback side; no counterpart in the

@@ -363,7 +363,12 @@ int lbfgsb_engine_solve_f32(lbfgsb_engine* e, const lbfgsb_callbacks_f32* cb,
   return drive<float>(e->e, cb, opt, summary, (void*)e->e->stream(), iter_clock, clock_cap);
 }
 
-void lbfgsb_engine_set_profiling(lbfgsb_engine* e, int from_iter) { e->e->set_profiling(from_iter); }
+void lbfgsb_engine_set_profiling(lbfgsb_engine* e, int from_iter) {
+  e->e->set_profiling(from_iter, ~0u);
+}
+void lbfgsb_engine_set_profiling_steps(lbfgsb_engine* e, int from_iter, unsigned step_mask) {
+  e->e->set_profiling(from_iter, step_mask);
+}
 
 int lbfgsb_engine_step_time(lbfgsb_engine* e, int step, double* total_ms, long long* count) {
   e->e->step_time(step, total_ms, count);

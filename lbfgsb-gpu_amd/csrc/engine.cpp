@@ -84,6 +84,7 @@ struct Engine final : EngineBase {
     hipEvent_t a, b;
   };
   int profile_from = -1;  // record kernels once the iteration count reaches this
+  unsigned profile_mask = ~0u;  // steps that are recorded (bit per step id)
   std::vector<hipEvent_t> ev_pool;
   std::vector<Pending> pending;
   double step_ms[kNumSteps] = {};
@@ -152,10 +153,11 @@ struct Engine final : EngineBase {
     }
     pending.clear();
   }
-  void set_profiling(int from_iter) override {
+  void set_profiling(int from_iter, unsigned step_mask) override {
     hipStreamSynchronize(st);
     drain_events();
     profile_from = from_iter;
+    profile_mask = step_mask;
     for (int i = 0; i < kNumSteps; i++) {
       step_ms[i] = 0;
       step_n[i] = 0;
@@ -248,7 +250,8 @@ struct Engine final : EngineBase {
   }
 
   void enqueue(int step, double f) {
-    if (profile_from < 0 || hs->iter < profile_from) return launch_step(step, f);
+    if (profile_from < 0 || hs->iter < profile_from || !((profile_mask >> step) & 1u))
+      return launch_step(step, f);
     hipEvent_t a = take_event(), b = take_event();
     if (!a || !b) return launch_step(step, f);
     hipEventRecord(a, st);
@@ -373,7 +376,10 @@ struct Engine final : EngineBase {
     HIPCHK(hipGetLastError());
     if (read_scalars()) return -1;
     if (hs->status == ST_GCP_ROUNDS) {
-      hipEvent_t ea = (profile_from >= 0 && hs->iter >= profile_from) ? take_event() : nullptr;
+      hipEvent_t ea = (profile_from >= 0 && hs->iter >= profile_from &&
+                       ((profile_mask >> K_END) & 1u))
+                          ? take_event()
+                          : nullptr;
       if (ea) hipEventRecord(ea, st);
       if (run_rounds()) return -1;
       if (ea) {

@@ -32,7 +32,7 @@ struct EngineBase {
   virtual hipStream_t stream() const = 0;
   virtual long long size() const = 0;
   // Per-step kernel timing with HIP events on the engine's stream.
-  virtual void set_profiling(int from_iter) = 0;  // -1 = off
+  virtual void set_profiling(int from_iter, unsigned step_mask) = 0;  // -1 = off
   virtual void step_time(int step, double* ms, long long* count) = 0;
   hipError_t last_error = hipSuccess;
 };

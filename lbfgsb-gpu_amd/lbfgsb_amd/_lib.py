@@ -111,6 +111,7 @@ SIGNATURES = {
                                           C.POINTER(OptionsF32), C.POINTER(SummaryF32),
                                           C.c_void_p, C.c_int]),
     "lbfgsb_engine_set_profiling": (None, [C.c_void_p, C.c_int]),
+    "lbfgsb_engine_set_profiling_steps": (None, [C.c_void_p, C.c_int, C.c_uint]),
     "lbfgsb_engine_step_time": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
                                           C.POINTER(C.c_longlong)]),
     "lbfgsb_problem_create": (C.c_void_p, [C.c_int, C.c_int, C.c_longlong, C.c_longlong,
