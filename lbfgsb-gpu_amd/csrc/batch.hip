@@ -64,7 +64,6 @@ struct Prob {
   uint8_t* iwhere;
   T* W;
   T* d;
-  T* z;
   T* dsub;
   Cand* ca;
   Cand* cb;
@@ -423,7 +422,7 @@ template <typename T, bool GO>
 __device__ __forceinline__ void b_p5b(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt) return;
   P5bOp<T> op;
-  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.z, P.dsub, P.d);
+  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.dsub, P.d);
   for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     typename P5bOp<T>::In e;
     op.load(e, i);
@@ -437,17 +436,28 @@ __device__ __forceinline__ void b_p5b(const Prob<T>& P, BatchLds& L, Gang<GO>& G
   gang_combine(G, L, P5_COUNT, 0, 0, 0, 2, &L.gflag);
 }
 
-// Trial point x_b = stp d + x_a, or the subspace point when stp == 1
-// (lbfgsbcpu.cpp:1329-1335).
+// Trial point x_b = stp d + x_a, or the subspace point z when stp == 1
+// (lbfgsbcpu.cpp:1329-1335).  The subspace pass leaves z in the trial
+// buffer, so a unit step only gets here (write_trial) when a trial since has
+// overwritten it; z is then formed again from the state it came from.
 template <typename T>
 __device__ __forceinline__ void b_trial(const Prob<T>& P, BatchLds& L) {
   if (!L.s.write_trial) return;
   const T* xa = P.X[L.s.trial_src];
   T* xb = P.X[L.s.trial_dst];
   const double stp = L.s.stp;
-  const bool one = stp == 1.0;
-  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
-    xb[i] = one ? P.z[i] : (T)(stp * (double)P.d[i] + (double)xa[i]);
+  if (stp == 1.0) {
+    P5bOp<T> op;
+    op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.dsub, nullptr);
+    for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
+      typename P5bOp<T>::In e;
+      op.load(e, i);
+      xb[i] = op.subspace_point(i, e);
+    }
+  } else {
+    for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
+      xb[i] = (T)(stp * (double)P.d[i] + (double)xa[i]);
+  }
   __syncthreads();
 }
 
@@ -885,7 +895,6 @@ __device__ __forceinline__ void solve_one(const BatchProb& bp, const BatchBufs<T
   P.iwhere = B.iwhere + bp.off;
   P.W = B.W + bp.woff;
   P.d = B.d + bp.off;
-  P.z = B.z + bp.off;
   P.dsub = B.dsub + bp.off;
   P.ca = B.ca + bp.off;
   P.cb = B.cb + bp.off;

@@ -59,7 +59,6 @@ struct BatchBufs {
   uint8_t* iwhere;
   T* W;
   T* d;
-  T* z;
   T* dsub;
   Cand* ca;  // breakpoint candidates, n per problem
   Cand* cb;

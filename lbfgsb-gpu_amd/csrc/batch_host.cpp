@@ -67,7 +67,7 @@ struct Arena final : BatchBase {
   long long cap;  // elements, multiple of 64
   int maxp;
   T *x = nullptr, *xb = nullptr, *g = nullptr, *gb = nullptr, *l = nullptr, *u = nullptr;
-  T *W = nullptr, *d = nullptr, *z = nullptr, *dsub = nullptr;
+  T *W = nullptr, *d = nullptr, *dsub = nullptr;
   T *ob = nullptr, *oc = nullptr, *oa0 = nullptr, *oa1 = nullptr, *oa2 = nullptr;
   int* nbd = nullptr;
   uint8_t* iwhere = nullptr;
@@ -112,7 +112,7 @@ struct Arena final : BatchBase {
   }
   int alloc() {
     const size_t c = (size_t)cap;
-    T** vec[] = {&x, &xb, &g, &gb, &l, &u, &d, &z, &dsub, &ob, &oc, &oa0, &oa1, &oa2};
+    T** vec[] = {&x, &xb, &g, &gb, &l, &u, &d, &dsub, &ob, &oc, &oa0, &oa1, &oa2};
     for (T** p : vec)
       if (dm(*p, c)) return -1;
     if (dm(W, c * 16) || dm(nbd, c) || dm(iwhere, c) || dm(ca, c) || dm(cb, c)) return -1;
@@ -131,7 +131,7 @@ struct Arena final : BatchBase {
     hipStreamSynchronize(st);
     if (gexec) hipGraphExecDestroy(gexec);
     if (cap_stream) hipStreamDestroy(cap_stream);
-    void* ptrs[] = {x,   xb,   g,      gb, l,  u,   W,   d,     z,     dsub, ob,
+    void* ptrs[] = {x,   xb,   g,      gb, l,  u,   W,   d,     dsub, ob,
                     oc,  oa0,  oa1,    oa2, nbd, iwhere, ca, cb, dsa, res, probs, items,
                     count, gang, gpart, next};
     for (void* p : ptrs)
@@ -347,7 +347,7 @@ struct Arena final : BatchBase {
     BCHK(hipMemcpyAsync(items, q.data(), sizeof(BatchItem) * q.size(), hipMemcpyHostToDevice,
                         st));
     BCHK(hipMemcpyAsync(count, cnt, sizeof(cnt), hipMemcpyHostToDevice, st));
-    BatchBufs<T> B{x, xb, g, gb, l, u, nbd, iwhere, W, d, z, dsub, ca, cb};
+    BatchBufs<T> B{x, xb, g, gb, l, u, nbd, iwhere, W, d, dsub, ca, cb};
     if (use_graph) {
       // the queue sizes are read from `count` on the device: one graph
       // serves every launch of this shape (workgroups beyond a queue's
@@ -455,7 +455,7 @@ long long arena_bytes(int dtype, long long capacity, int max_problems) {
   const long long s = dtype == 1 ? 8 : 4;
   const long long cap = ((capacity + 63) / 64) * 64;
   const long long items = (long long)max_problems + cap / kMinSegment + 1;
-  long long b = cap * (14 * s + 16 * s + (long long)sizeof(int) + 1 + 2 * (long long)sizeof(Cand));
+  long long b = cap * (13 * s + 16 * s + (long long)sizeof(int) + 1 + 2 * (long long)sizeof(Cand));
   b += (long long)max_problems *
        (long long)(sizeof(DevScalars) + sizeof(BatchResult) + sizeof(BatchProb) + sizeof(GangCtl));
   b += items * ((long long)sizeof(BatchItem) + 2LL * kGangPart * (long long)sizeof(double));

@@ -315,19 +315,19 @@ k_p5a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __re
   }
 }
 
-// k_p5b: apply the backtrack (pinning the limiting variable), d = z - x,
-// and the line-search set-up sums d'd, g'd and the nbd-aware max step.
+// k_p5b: apply the backtrack (pinning the limiting variable), z into the
+// trial buffer, d = z - x, and the line-search set-up sums d'd, g'd and the
+// nbd-aware max step.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_p5b(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
       const T* __restrict__ u, const int* __restrict__ nbd,
-      const uint8_t* __restrict__ iwhere, T* __restrict__ z,
-      const T* __restrict__ dsub, T* __restrict__ d, DevScalars* ds,
-      double* part, double* pmin) {
+      const uint8_t* __restrict__ iwhere, const T* __restrict__ dsub, T* __restrict__ d,
+      DevScalars* ds, double* part, double* pmin) {
   __shared__ double sh[kWavesPerBlock * P5_COUNT];
   if (ds->halt) return;
   P5bOp<T> op;
-  op.init(ds, X, Gv, l, u, nbd, iwhere, z, dsub, d);
+  op.init(ds, X, Gv, l, u, nbd, iwhere, dsub, d);
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
     typename P5bOp<T>::In e;
@@ -347,24 +347,56 @@ k_p5b(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
   }
 }
 
-// Trial point x_b = stp*d + x_a, or the subspace point when stp == 1
-// (lbfgsbcpu.cpp:1329-1335).
+// The subspace point z of every variable, as k_p5b forms it, into out.
+template <typename T>
+__device__ __forceinline__ void subspace_points(long long n, Pair<T> X, Pair<T> Gv,
+                                                const T* __restrict__ l,
+                                                const T* __restrict__ u,
+                                                const int* __restrict__ nbd,
+                                                const uint8_t* __restrict__ iwhere,
+                                                const T* __restrict__ dsub,
+                                                const DevScalars* ds, T* __restrict__ out) {
+  P5bOp<T> op;
+  op.init(ds, X, Gv, l, u, nbd, iwhere, dsub, nullptr);
+  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock) {
+    typename P5bOp<T>::In e;
+    op.load(e, i);
+    out[i] = op.subspace_point(i, e);
+  }
+}
+
+// Trial point x_b = stp*d + x_a, or the subspace point z when stp == 1
+// (lbfgsbcpu.cpp:1329-1335).  k_p5b leaves z in the trial buffer, so a unit
+// step only gets here (write_trial) when a trial since has overwritten it;
+// z is then formed again from the state it came from.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_trial(long long n, Pair<T> X, const T* __restrict__ z, const T* __restrict__ d,
-        const DevScalars* ds) {
+k_trial(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __restrict__ u,
+        const int* __restrict__ nbd, const uint8_t* __restrict__ iwhere,
+        const T* __restrict__ dsub, const T* __restrict__ d, const DevScalars* ds) {
   if (!ds->write_trial) return;
   const T* __restrict__ xa = X[ds->trial_src];
   T* __restrict__ xb = X[ds->trial_dst];
   const double stp = ds->stp;
-  const bool one = stp == 1.0;
-  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
-       i += (long long)gridDim.x * kBlock) {
-    if (one)
-      xb[i] = z[i];
-    else
-      xb[i] = (T)(stp * (double)d[i] + (double)xa[i]);
+  if (stp == 1.0) {
+    subspace_points(n, X, Gv, l, u, nbd, iwhere, dsub, ds, xb);
+    return;
   }
+  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    xb[i] = (T)(stp * (double)d[i] + (double)xa[i]);
+}
+
+// z of every variable into out (an inspection aid: the iteration keeps z
+// only in the trial buffer).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_subspace_point(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
+                 const T* __restrict__ u, const int* __restrict__ nbd,
+                 const uint8_t* __restrict__ iwhere, const T* __restrict__ dsub,
+                 const DevScalars* ds, T* __restrict__ out) {
+  subspace_points(n, X, Gv, l, u, nbd, iwhere, dsub, ds, out);
 }
 
 // g_b . d after an evaluation (lbfgsbcpu.cpp:1312).
@@ -437,15 +469,23 @@ void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> G
 template <typename T>
 void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
                 const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
-                T* z, const T* dsub, T* d, DevScalars* ds, double* part,
-                double* pmin) {
+                const T* dsub, T* d, DevScalars* ds, double* part, double* pmin) {
   hipLaunchKernelGGL(k_p5b<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd,
-                     iwhere, z, dsub, d, ds, part, pmin);
+                     iwhere, dsub, d, ds, part, pmin);
 }
 template <typename T>
-void launch_trial(hipStream_t s, int G, long long n, Pair<T> X, const T* z,
+void launch_trial(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
+                  const T* u, const int* nbd, const uint8_t* iwhere, const T* dsub,
                   const T* d, const DevScalars* ds) {
-  hipLaunchKernelGGL(k_trial<T>, dim3(G), dim3(kBlock), 0, s, n, X, z, d, ds);
+  hipLaunchKernelGGL(k_trial<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd, iwhere,
+                     dsub, d, ds);
+}
+template <typename T>
+void launch_subspace_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
+                           const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
+                           const T* dsub, const DevScalars* ds, T* out) {
+  hipLaunchKernelGGL(k_subspace_point<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd,
+                     iwhere, dsub, ds, out);
 }
 template <typename T>
 void launch_gd(hipStream_t s, int G, long long n, Pair<T> Gv, const T* d,
@@ -474,10 +514,15 @@ void launch_gd(hipStream_t s, int G, long long n, Pair<T> Gv, const T* d,
                                        Pair<T>, const T*, const T*, const int*,    \
                                        const uint8_t*, const DevScalars*, T*);      \
   template void launch_p5b<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
-                              const T*, const T*, const int*, const uint8_t*, T*,   \
+                              const T*, const T*, const int*, const uint8_t*,       \
                               const T*, T*, DevScalars*, double*, double*);         \
-  template void launch_trial<T>(hipStream_t, int, long long, Pair<T>, const T*,   \
-                                const T*, const DevScalars*);                       \
+  template void launch_trial<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
+                                const T*, const T*, const int*, const uint8_t*,     \
+                                const T*, const T*, const DevScalars*);             \
+  template void launch_subspace_point<T>(hipStream_t, int, long long, Pair<T>,      \
+                                         Pair<T>, const T*, const T*, const int*,  \
+                                         const uint8_t*, const T*,                  \
+                                         const DevScalars*, T*);                    \
   template void launch_gd<T>(hipStream_t, int, long long, Pair<T>, const T*,      \
                              const DevScalars*, double*);
 

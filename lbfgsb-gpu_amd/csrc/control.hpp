@@ -355,14 +355,22 @@ __device__ inline bool formk_block(DevScalars* ds, const double* nc, const doubl
   return ds->blk_flag != 0;
 }
 
+// The trial point x = z for a unit step, x = stp d + x otherwise
+// (lbfgsbcpu.cpp:1329-1335).  The subspace pass has left z in the trial
+// buffer: a unit step while it is still there needs no write.
 __device__ inline void request_trial(DevScalars* ds) {
   ds->task = 1;
   ds->ifun = ds->ifun + 1;
   ds->nfgv = ds->nfgv + 1;
   ds->iback = ds->ifun - 1;
-  ds->write_trial = 1;
   ds->trial_src = ds->phase & 1;
   ds->trial_dst = (ds->phase & 1) ^ 1;
+  if (ds->stp == 1.0 && ds->trial_has_z) {
+    ds->write_trial = 0;
+  } else {
+    ds->write_trial = 1;
+    ds->trial_has_z = 0;  // k_trial overwrites it
+  }
 }
 
 // Step accepted: the reference increments iter and, after its stopping
@@ -620,6 +628,7 @@ __device__ inline void logic_ls_init(DevScalars* s, double dtd, double gd, doubl
   s->ifun = 0;
   s->iback = 0;
   s->csave = 0;
+  s->trial_has_z = 1;  // the subspace pass wrote z into the trial buffer
   s->gd = gd;
   s->gdold = s->gd;
   if (s->gd >= 0) {

@@ -59,9 +59,11 @@ struct Engine final : EngineBase {
   const T* l;
   const T* u;
   const int* nbd;
-  T *W = nullptr, *d = nullptr, *z = nullptr, *dsub = nullptr;
+  T *W = nullptr, *d = nullptr, *dsub = nullptr;
   T* tbrk = nullptr;  // breakpoint t per variable, written by pass A
-  T* xcp = nullptr;   // x_cp on request (named array "xcp"); the iteration keeps none
+  // x_cp or z on request (named arrays "xcp", "z"): the iteration stores no
+  // x_cp, and z only in the trial buffer until a trial overwrites it
+  T* insp = nullptr;
   uint8_t* iwhere = nullptr;
   DevScalars* ds = nullptr;
   DevScalars* hs = nullptr;
@@ -108,7 +110,7 @@ struct Engine final : EngineBase {
   int alloc() {
     const size_t nn = (size_t)n;
     if (dmalloc(X[1], nn) || dmalloc(Gr[0], nn) || dmalloc(Gr[1], nn)) return -1;
-    if (dmalloc(W, wsize(nn)) || dmalloc(d, nn) || dmalloc(z, nn) || dmalloc(dsub, nn) ||
+    if (dmalloc(W, wsize(nn)) || dmalloc(d, nn) || dmalloc(dsub, nn) ||
         dmalloc(tbrk, nn) || dmalloc(iwhere, nn))
       return -1;
     HIPCHK(hipMemsetAsync(W, 0, wsize(nn) * sizeof(T), st));
@@ -179,7 +181,7 @@ struct Engine final : EngineBase {
     hipStreamSynchronize(st);
     drain_events();
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
-    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, z, dsub, tbrk, xcp};
+    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, dsub, tbrk, insp};
     for (T* p : bufs) hipFree(p);
     hipFree(iwhere);
     hipFree(ds);
@@ -305,13 +307,13 @@ struct Engine final : EngineBase {
         launch_alpha(st, ds, pmin, pmini, G);
         break;
       case K_P5B:
-        launch_p5b<T>(st, G, n, dX, dG, l, u, nbd, iwhere, z, dsub, d, ds, part, pmin);
+        launch_p5b<T>(st, G, n, dX, dG, l, u, nbd, iwhere, dsub, d, ds, part, pmin);
         break;
       case K_LSINIT:
         launch_ls_init(st, ds, part, pmin, G);
         break;
       case K_TRIAL:
-        launch_trial<T>(st, G, n, dX, z, d, ds);
+        launch_trial<T>(st, G, n, dX, dG, l, u, nbd, iwhere, dsub, d, ds);
         break;
       case K_GD:
         launch_gd<T>(st, G, n, dG, d, ds, part);
@@ -426,15 +428,18 @@ struct Engine final : EngineBase {
     if (name == "g") return dcopy(Gr[c], nn * sizeof(T));
     if (name == "xb") return dcopy(X[c ^ 1], nn * sizeof(T));
     if (name == "gb") return dcopy(Gr[c ^ 1], nn * sizeof(T));
-    if (name == "z") return dcopy(z, nn * sizeof(T));
     if (name == "d") return dcopy(d, nn * sizeof(T));
-    if (name == "xcp") {
-      // the Cauchy point as the passes after the search compute it, from
-      // the current x, g, class and search outcome
-      if (!xcp && dmalloc(xcp, nn)) return -1;
-      launch_cauchy_point<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, xcp);
+    if (name == "xcp" || name == "z") {
+      // the Cauchy point or the subspace point as the passes after the
+      // search compute them, from the current x, g, class, search outcome
+      // (and d_sub, alpha for z)
+      if (!insp && dmalloc(insp, nn)) return -1;
+      if (name == "xcp")
+        launch_cauchy_point<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, insp);
+      else
+        launch_subspace_point<T>(st, G, n, dX, dG, l, u, nbd, iwhere, dsub, ds, insp);
       HIPCHK(hipGetLastError());
-      return dcopy(xcp, nn * sizeof(T));
+      return dcopy(insp, nn * sizeof(T));
     }
     if (name == "dsub") return dcopy(dsub, nn * sizeof(T));
     if (name == "iwbits") return dcopy(iwhere, nn);
@@ -504,7 +509,6 @@ struct Engine final : EngineBase {
     if (name == "g") return hcopy(Gr[c], nn * sizeof(T));
     if (name == "xb") return hcopy(X[c ^ 1], nn * sizeof(T));
     if (name == "gb") return hcopy(Gr[c ^ 1], nn * sizeof(T));
-    if (name == "z") return hcopy(z, nn * sizeof(T));
     if (name == "d") return hcopy(d, nn * sizeof(T));
     if (name == "iwbits") return hcopy(iwhere, nn);
     if (name == "ws" || name == "wy") {

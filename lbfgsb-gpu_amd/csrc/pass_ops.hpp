@@ -523,7 +523,9 @@ struct P5aOp {
 // ---------------------------------------------------------------------------
 // Backtrack (pinning the limiting variable), d = z - x, and the line-search
 // set-up sums d'd, g'd and the nbd-aware max step (lbfgsbcpu.cpp:1517-1533,
-// 343-345, 1263-1299).
+// 343-345, 1263-1299).  z goes into the trial buffer X[cur ^ 1], where it is
+// the first trial point of a unit step; the trial pass recomputes it with
+// subspace_point() if a later unit-step trial finds it overwritten.
 template <typename T>
 struct P5bOp {
   const T* __restrict__ x;
@@ -549,7 +551,7 @@ struct P5bOp {
   };
 
   __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
-                              const T* u_, const int* nbd_, const uint8_t* iwhere_, T* z_,
+                              const T* u_, const int* nbd_, const uint8_t* iwhere_,
                               const T* dsub_, T* d_) {
     const int cur = ds->phase & 1;
     x = X[cur];
@@ -558,7 +560,7 @@ struct P5bOp {
     u = u_;
     nbd = nbd_;
     iwhere = iwhere_;
-    z = z_;
+    z = X[cur ^ 1];
     dsub = dsub_;
     d = d_;
     sub = !ds->skip_subsm;
@@ -583,12 +585,11 @@ struct P5bOp {
   __device__ __forceinline__ void keep(const In& e) const {
     keep_loads(e.x, e.g, e.l, e.u, e.iwb, e.dsub, e.nb);
   }
-  __device__ __forceinline__ void element(long long i, const In& e) {
-    const T big = (T)1.0e10;
-    const T xi = e.x;
+  // The subspace point of variable i (x_cp where there is no subspace step).
+  __device__ __forceinline__ T subspace_point(long long i, const In& e) const {
     int iw = iw_get(e.iwb);
     const bool freenow = shortcut || iw <= 0;
-    T zi = co.point(i, xi, e.g, e.l, e.u, e.nb, iw);  // x_cp
+    T zi = co.point(i, e.x, e.g, e.l, e.u, e.nb, iw);  // x_cp
     if (sub && freenow) {
       T dk = e.dsub;
       if (pin && i == ibd) {
@@ -602,7 +603,13 @@ struct P5bOp {
       }
       zi = zi + alpha * dk;
     }
-    z[i] = zi;  // the subspace point (x_cp where there is no subspace step)
+    return zi;
+  }
+  __device__ __forceinline__ void element(long long i, const In& e) {
+    const T big = (T)1.0e10;
+    const T xi = e.x;
+    const T zi = subspace_point(i, e);
+    z[i] = zi;
     const T di = zi - xi;
     d[i] = di;
     acc[P5_DTD] += (double)di * (double)di;

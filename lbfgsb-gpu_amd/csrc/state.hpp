@@ -152,8 +152,10 @@ struct DevScalars {
   long long ibd;
 
   // ---- trial point bookkeeping: k_trial writes X[trial_dst] from
-  // X[trial_src] when write_trial is set
-  int write_trial, trial_src, trial_dst;
+  // X[trial_src] when write_trial is set.  The subspace pass leaves the
+  // subspace point z in the trial buffer (trial_has_z), which is the trial
+  // point of a unit step.
+  int write_trial, trial_src, trial_dst, trial_has_z;
   int has_after_iteration, has_custom_stop;
 
   // ---- append counters (breakpoint list, round candidates)
