@@ -42,13 +42,15 @@ def kernel_bytes_per_elem(s: int) -> dict:
                     the breakpoint t (s) the Cauchy search selects from
       p4:           read x, g, l, u (4s), nbd, iwhere, W (16s); write iwhere
       p5a:          read x, g, l, u (4s), nbd, iwhere, W (16s); write d_sub (s)
-      p5b:          read x, g, d_sub, l, u (5s), nbd, iwhere; write z, d (2s)
-      trial:        read z, write x_trial (2s, unit step)
+      p5b:          read x, g, d_sub, l, u (5s), nbd, iwhere; write z into the
+                    trial buffer, d (2s)
+      trial:        nothing: the first trial is a unit step, whose trial point
+                    z is already in place (a non-unit step moves 3s)
       gd:           read g, d (2s)
     x_cp and r are recomputed per element by the passes that need them (from
     x, g, l, u, the class and W) instead of being stored and read back."""
     return {"pass_a_fused": 24 * s + 5, "p4": 20 * s + 6, "p5a": 21 * s + 5,
-            "p5b": 7 * s + 5, "trial": 2 * s, "gd": 2 * s}
+            "p5b": 7 * s + 5, "trial": 0, "gd": 2 * s}
 
 
 # HBM bytes per launch at steady state for the default workload (n = 16,777,216,
@@ -60,8 +62,8 @@ MEASURED_TRAFFIC_16M_FP32 = {"pass_a_fused": 1649638490, "p4": 1589936986, "p5a"
 
 
 def b_alg_per_iter(n: int, s: int) -> float:
-    """Algorithmic bytes of one steady-state iteration with one trial point:
-    the sum of kernel_bytes_per_elem, n (76 s + 21)."""
+    """Algorithmic bytes of one steady-state iteration with one (unit-step)
+    trial point: the sum of kernel_bytes_per_elem, n (74 s + 21)."""
     return float(n) * sum(kernel_bytes_per_elem(s).values())
 
 
