@@ -27,10 +27,21 @@ k_obj(ObjDesc o, long long n, const T* __restrict__ x, T* __restrict__ g, double
 __global__ void s_obj_finish(const double* part, int G, double* f_out) {
   const int lane = threadIdx.x & 63;
   if (threadIdx.x >= 64) return;
+  // every partial of the lane (b = lane + 64 k < G <= kMaxBlocks) loaded
+  // before the first two-sum, then summed in order of b
+  constexpr int kPer = kMaxBlocks / kWave;
+  double pv[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int b = lane + k * kWave;
+    pv[k] = b < G ? part[b] : 0.0;
+  }
   double s = 0.0, c = 0.0;
-  for (int b = lane; b < G; b += 64) {
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    if (lane + k * kWave >= G) break;
     double e;
-    two_sum(s, part[b], s, e);
+    two_sum(s, pv[k], s, e);
     c += e;
   }
   for (int o = 32; o > 0; o >>= 1) {

@@ -32,22 +32,46 @@ __device__ void ds_store(DevScalars* ds, const DevScalars* sds) {
   for (int i = threadIdx.x; i < kDsWords; i += blockDim.x) dst[i] = src[i];
 }
 
-// Compensated reduction of value rows v0..v0+nv (each G partials long) into
-// out[0..nv): one wave per row, four independent accumulators per lane,
-// two-sum combination across accumulators and lanes.
+// The bulk passes leave at most kMaxBlocks partials per value.  The
+// finishing reductions load every partial of a thread before the first
+// operation on them, so a thread waits on its loads once instead of once per
+// loop trip, and then combine them in the order of the plain loop.
+constexpr int kPartPerLane = kMaxBlocks / kWave;     // one wave per row
+constexpr int kPartPerThread = kMaxBlocks / kBlock;  // the whole block
+
+// Compensated reduction of value rows v0..v0+nv (each G <= kMaxBlocks
+// partials long) into out[0..nv): one wave per row, four independent
+// accumulators per lane (partials lane, lane + 64, lane + 128, lane + 192,
+// then the next 256; the tail into the first), two-sum combination across
+// accumulators and lanes.
 __device__ void reduce_rows(const double* part, int G, int v0, int nv, double* out) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int v = w; v < nv; v += kWavesPerBlock) {
     const double* row = part + (size_t)(v0 + v) * G;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int b = lane;
-    for (; b + 3 * kWave < G; b += 4 * kWave) {
-      a0 += row[b];
-      a1 += row[b + kWave];
-      a2 += row[b + 2 * kWave];
-      a3 += row[b + 3 * kWave];
+    double pv[kPartPerLane];
+#pragma unroll
+    for (int k = 0; k < kPartPerLane; k++) {
+      const int b = lane + k * kWave;
+      pv[k] = b < G ? row[b] : 0.0;
     }
-    for (; b < G; b += kWave) a0 += row[b];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    bool tail = false;
+#pragma unroll
+    for (int q = 0; q < kPartPerLane / 4; q++) {
+      const int b = lane + q * 4 * kWave;
+      if (tail) continue;
+      if (b + 3 * kWave < G) {
+        a0 += pv[4 * q];
+        a1 += pv[4 * q + 1];
+        a2 += pv[4 * q + 2];
+        a3 += pv[4 * q + 3];
+      } else {
+        tail = true;  // b + 4 * kWave > G: nothing after this group
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+          if (b + r * kWave < G) a0 += pv[4 * q + r];
+      }
+    }
     double s, c = 0.0, e;
     two_sum(a0, a1, s, e);
     c += e;
@@ -65,9 +89,22 @@ __device__ void reduce_rows(const double* part, int G, int v0, int nv, double* o
   }
 }
 
+// The partials of this thread (b = threadIdx.x + k * kBlock < G), padded.
+__device__ __forceinline__ void load_partials(const double* p, int G, double pad,
+                                              double (&v)[kPartPerThread]) {
+#pragma unroll
+  for (int k = 0; k < kPartPerThread; k++) {
+    const int b = threadIdx.x + k * kBlock;
+    v[k] = b < G ? p[b] : pad;
+  }
+}
+
 __device__ double reduce_max(const double* pm, int G, double* sh) {
+  double pv[kPartPerThread];
+  load_partials(pm, G, -__builtin_inf(), pv);
   double m = -__builtin_inf();
-  for (int b = threadIdx.x; b < G; b += blockDim.x) m = fmax(m, pm[b]);
+#pragma unroll
+  for (int k = 0; k < kPartPerThread; k++) m = fmax(m, pv[k]);
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
@@ -77,16 +114,24 @@ __device__ double reduce_max(const double* pm, int G, double* sh) {
   return r;
 }
 
+// Smallest (value, index) pair, index breaking ties.
 __device__ void reduce_argmin(const double* pv, const long long* pi, int G, double* shv,
                               long long* shi, double& v, long long& idx) {
+  double xv[kPartPerThread];
+  long long xi[kPartPerThread];
+  load_partials(pv, G, __builtin_inf(), xv);
+#pragma unroll
+  for (int k = 0; k < kPartPerThread; k++) {
+    const int b = threadIdx.x + k * kBlock;
+    xi[k] = b < G ? pi[b] : 0x7fffffffffffffffLL;
+  }
   double bv = __builtin_inf();
   long long bi = 0x7fffffffffffffffLL;
-  for (int b = threadIdx.x; b < G; b += blockDim.x) {
-    const double x = pv[b];
-    const long long k = pi[b];
-    if (x < bv || (x == bv && k < bi)) {
-      bv = x;
-      bi = k;
+#pragma unroll
+  for (int k = 0; k < kPartPerThread; k++) {
+    if (xv[k] < bv || (xv[k] == bv && xi[k] < bi)) {
+      bv = xv[k];
+      bi = xi[k];
     }
   }
   wave_argmin(bv, bi);
@@ -106,8 +151,11 @@ __device__ void reduce_argmin(const double* pv, const long long* pi, int G, doub
 }
 
 __device__ void reduce_min(const double* pv, int G, double* sh, double& v) {
+  double xv[kPartPerThread];
+  load_partials(pv, G, __builtin_inf(), xv);
   double m = __builtin_inf();
-  for (int b = threadIdx.x; b < G; b += blockDim.x) m = fmin(m, pv[b]);
+#pragma unroll
+  for (int k = 0; k < kPartPerThread; k++) m = fmin(m, xv[k]);
   m = -wave_max(-m);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
@@ -135,13 +183,23 @@ k_rowsum(const double* part, int G, int v0, DevScalars* ds, int p4_rows) {
     }
   }
   const double* row = part + (size_t)(v0 + v) * G;
+  // partials t, t + 256 into a0, a1, then the next 512; the tail into a0
+  double pv[kPartPerThread];
+  load_partials(row, G, 0.0, pv);
   double a0 = 0.0, a1 = 0.0;
-  int b = threadIdx.x;
-  for (; b + kBlock < G; b += 2 * kBlock) {
-    a0 += row[b];
-    a1 += row[b + kBlock];
+  bool tail = false;
+#pragma unroll
+  for (int q = 0; q < kPartPerThread / 2; q++) {
+    const int b = threadIdx.x + q * 2 * kBlock;
+    if (tail) continue;
+    if (b + kBlock < G) {
+      a0 += pv[2 * q];
+      a1 += pv[2 * q + 1];
+    } else {
+      tail = true;  // b + 2 * kBlock > G: nothing after this pair
+      if (b < G) a0 += pv[2 * q];
+    }
   }
-  for (; b < G; b += kBlock) a0 += row[b];
   double s, c, e;
   two_sum(a0, a1, s, c);
 #pragma unroll
