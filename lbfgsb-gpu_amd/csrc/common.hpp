@@ -237,4 +237,10 @@ __device__ inline T ld_nt(const T* p) {
   return __builtin_nontemporal_load(p);
 }
 
+// Streaming store of a bulk pass's output (nontemporal hint).
+template <typename T>
+__device__ inline void st_nt(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 }  // namespace lbfgsb

@@ -108,8 +108,8 @@ struct PassAOp {
         ys = gi - e.go;
         ss = stp1 ? e.dd : e.dd * stp;
         acc_prod(acc[PA_RR], ys, ys);
-        W[widx(i, itail)] = ys;
-        W[widx(i, 8 + itail)] = ss;
+        st_nt(&W[widx(i, itail)], ys);
+        st_nt(&W[widx(i, 8 + itail)], ss);
       }
     }
     // Unused positions hold zeros (mask_ring), and with an update the
@@ -158,7 +158,7 @@ struct PassAOp {
       // class itself is not stored: the Cauchy-point pass reclassifies from
       // the stored one (classification is idempotent), so this pass writes
       // one stream less
-      tout[i] = isbrk ? tbrk : (T)__builtin_inf();
+      st_nt(&tout[i], isbrk ? tbrk : (T)__builtin_inf());
     } else {
       iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb));
     }
@@ -364,7 +364,7 @@ struct P4Op {
           flag = -1;
         }
       }
-      iwhere[i] = iw_make(iw, freenow, wasfree);
+      st_nt(&iwhere[i], iw_make(iw, freenow, wasfree));
     }
     // Positions j >= col hold zeros (mask_ring), so the sums below run over
     // all kMaxM positions without guards: unused accumulators collect exact
@@ -495,7 +495,7 @@ struct P5aOp {
     for (int j = 0; j < kMaxM; j++)
       if (j < col) di += (double)e.yv[j] * wy[j] + (double)e.sv[j] * wsv[j];
     const T dk = (T)(di / theta);
-    dsub[i] = dk;
+    st_nt(&dsub[i], dk);
     const int nb = e.nb;
     if (nb != 0) {
       double cand = 2.0;
@@ -609,7 +609,7 @@ struct P5bOp {
     const T big = (T)1.0e10;
     const T xi = e.x;
     const T zi = subspace_point(i, e);
-    z[i] = zi;
+    z[i] = zi;  // read back soon (objective, k_gd): normal stores
     const T di = zi - xi;
     d[i] = di;
     acc[P5_DTD] += (double)di * (double)di;
