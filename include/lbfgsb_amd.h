@@ -113,10 +113,10 @@ void* lbfgsb_engine_g(lbfgsb_engine* e, int which);
 /* Named scalar of the device state (see DevScalars), as double. */
 int lbfgsb_engine_get_scalar(lbfgsb_engine* e, const char* name, double* value);
 int lbfgsb_engine_set_scalar(lbfgsb_engine* e, const char* name, double value);
-/* Named array copied to / from host memory.  Names: x, g, xb, gb, d, dsub
- * (bulk type, n); xcp, z (bulk type, n; read only: the Cauchy point and
- * the subspace point computed from the current state, which the iteration
- * does not keep);
+/* Named array copied to / from host memory.  Names: x, g, xb, gb, dsub
+ * (bulk type, n); xcp, z, d (bulk type, n; read only: the Cauchy point,
+ * the subspace point and the line-search direction computed from the
+ * current state, which the iteration does not keep as such);
  * iwhere (int32, n); iwbits (uint8, n); ws, wy (bulk type, n x m
  * row-major); index, indx2 (int32, n, 1-based as the reference); sy, ss,
  * wt (m x m), wn, snd (2m x 2m), p, c, mc, wv, pa (double). */

@@ -422,7 +422,7 @@ template <typename T, bool GO>
 __device__ __forceinline__ void b_p5b(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt) return;
   P5bOp<T> op;
-  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.dsub, P.d);
+  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.dsub);
   for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     typename P5bOp<T>::In e;
     op.load(e, i);
@@ -439,25 +439,19 @@ __device__ __forceinline__ void b_p5b(const Prob<T>& P, BatchLds& L, Gang<GO>& G
 // Trial point x_b = stp d + x_a, or the subspace point z when stp == 1
 // (lbfgsbcpu.cpp:1329-1335).  The subspace pass leaves z in the trial
 // buffer, so a unit step only gets here (write_trial) when a trial since has
-// overwritten it; z is then formed again from the state it came from.
+// overwritten it; z is then formed again from the state it came from.  The
+// first trial off z stores d = z - x (trial_element).
 template <typename T>
 __device__ __forceinline__ void b_trial(const Prob<T>& P, BatchLds& L) {
   if (!L.s.write_trial) return;
   const T* xa = P.X[L.s.trial_src];
   T* xb = P.X[L.s.trial_dst];
   const double stp = L.s.stp;
-  if (stp == 1.0) {
-    P5bOp<T> op;
-    op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.dsub, nullptr);
-    for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
-      typename P5bOp<T>::In e;
-      op.load(e, i);
-      xb[i] = op.subspace_point(i, e);
-    }
-  } else {
-    for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
-      xb[i] = (T)(stp * (double)P.d[i] + (double)xa[i]);
-  }
+  const bool trial_d = L.s.trial_d;
+  P5bOp<T> sub;
+  sub.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.dsub);
+  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
+    trial_element(i, &L.s, sub, xa, xb, P.d, stp, trial_d);
   __syncthreads();
 }
 
@@ -465,9 +459,11 @@ __device__ __forceinline__ void b_trial(const Prob<T>& P, BatchLds& L) {
 template <typename T, bool GO>
 __device__ __forceinline__ void b_gd(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   const T* gb = P.G[(L.s.phase & 1) ^ 1];
+  LsDirection<T> dir;
+  dir.init(&L.s, P.X, P.d);
   double acc[1] = {0.0};
   for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
-    acc[0] += (double)gb[i] * (double)P.d[i];
+    acc[0] += (double)gb[i] * (double)dir.at(i);
   bsum<1>(acc, L.u.r.sh, L.red);
   gang_combine(G, L, 1, 0, 0, 0, 0, &L.gflag);
 }

@@ -316,18 +316,18 @@ k_p5a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __re
 }
 
 // k_p5b: apply the backtrack (pinning the limiting variable), z into the
-// trial buffer, d = z - x, and the line-search set-up sums d'd, g'd and the
+// trial buffer, and the line-search set-up sums d'd, g'd (d = z - x) and the
 // nbd-aware max step.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_p5b(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
       const T* __restrict__ u, const int* __restrict__ nbd,
-      const uint8_t* __restrict__ iwhere, const T* __restrict__ dsub, T* __restrict__ d,
-      DevScalars* ds, double* part, double* pmin) {
+      const uint8_t* __restrict__ iwhere, const T* __restrict__ dsub, DevScalars* ds,
+      double* part, double* pmin) {
   __shared__ double sh[kWavesPerBlock * P5_COUNT];
   if (ds->halt) return;
   P5bOp<T> op;
-  op.init(ds, X, Gv, l, u, nbd, iwhere, dsub, d);
+  op.init(ds, X, Gv, l, u, nbd, iwhere, dsub);
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
     typename P5bOp<T>::In e;
@@ -357,7 +357,7 @@ __device__ __forceinline__ void subspace_points(long long n, Pair<T> X, Pair<T> 
                                                 const T* __restrict__ dsub,
                                                 const DevScalars* ds, T* __restrict__ out) {
   P5bOp<T> op;
-  op.init(ds, X, Gv, l, u, nbd, iwhere, dsub, nullptr);
+  op.init(ds, X, Gv, l, u, nbd, iwhere, dsub);
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
     typename P5bOp<T>::In e;
@@ -369,23 +369,23 @@ __device__ __forceinline__ void subspace_points(long long n, Pair<T> X, Pair<T> 
 // Trial point x_b = stp*d + x_a, or the subspace point z when stp == 1
 // (lbfgsbcpu.cpp:1329-1335).  k_p5b leaves z in the trial buffer, so a unit
 // step only gets here (write_trial) when a trial since has overwritten it;
-// z is then formed again from the state it came from.
+// z is then formed again from the state it came from.  The first trial off
+// z stores d = z - x (trial_element).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_trial(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __restrict__ u,
         const int* __restrict__ nbd, const uint8_t* __restrict__ iwhere,
-        const T* __restrict__ dsub, const T* __restrict__ d, const DevScalars* ds) {
+        const T* __restrict__ dsub, T* __restrict__ d, const DevScalars* ds) {
   if (!ds->write_trial) return;
   const T* __restrict__ xa = X[ds->trial_src];
   T* __restrict__ xb = X[ds->trial_dst];
   const double stp = ds->stp;
-  if (stp == 1.0) {
-    subspace_points(n, X, Gv, l, u, nbd, iwhere, dsub, ds, xb);
-    return;
-  }
+  const bool trial_d = ds->trial_d;
+  P5bOp<T> sub;
+  sub.init(ds, X, Gv, l, u, nbd, iwhere, dsub);
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock)
-    xb[i] = (T)(stp * (double)d[i] + (double)xa[i]);
+    trial_element(i, ds, sub, xa, xb, d, stp, trial_d);
 }
 
 // z of every variable into out (an inspection aid: the iteration keeps z
@@ -402,15 +402,17 @@ k_subspace_point(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
 // g_b . d after an evaluation (lbfgsbcpu.cpp:1312).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_gd(long long n, Pair<T> Gv, const T* __restrict__ d, const DevScalars* ds,
+k_gd(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ d, const DevScalars* ds,
      double* part) {
   __shared__ double sh[kWavesPerBlock];
   const int cur = ds->phase & 1;
   const T* __restrict__ gb = Gv[cur ^ 1];
+  LsDirection<T> dir;
+  dir.init(ds, X, d);
   double acc[1] = {0.0};
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock)
-    acc[0] += (double)gb[i] * (double)d[i];
+    acc[0] += (double)gb[i] * (double)dir.at(i);
   block_sum_store<1>(acc, sh, part + blockIdx.x, gridDim.x);
 }
 
@@ -469,14 +471,14 @@ void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> G
 template <typename T>
 void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
                 const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
-                const T* dsub, T* d, DevScalars* ds, double* part, double* pmin) {
+                const T* dsub, DevScalars* ds, double* part, double* pmin) {
   hipLaunchKernelGGL(k_p5b<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd,
-                     iwhere, dsub, d, ds, part, pmin);
+                     iwhere, dsub, ds, part, pmin);
 }
 template <typename T>
 void launch_trial(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                   const T* u, const int* nbd, const uint8_t* iwhere, const T* dsub,
-                  const T* d, const DevScalars* ds) {
+                  T* d, const DevScalars* ds) {
   hipLaunchKernelGGL(k_trial<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd, iwhere,
                      dsub, d, ds);
 }
@@ -488,9 +490,25 @@ void launch_subspace_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T>
                      iwhere, dsub, ds, out);
 }
 template <typename T>
-void launch_gd(hipStream_t s, int G, long long n, Pair<T> Gv, const T* d,
+void launch_gd(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* d,
                const DevScalars* ds, double* part) {
-  hipLaunchKernelGGL(k_gd<T>, dim3(G), dim3(kBlock), 0, s, n, Gv, d, ds, part);
+  hipLaunchKernelGGL(k_gd<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, d, ds, part);
+}
+// d = z - x of every variable during the line search into out (inspection).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_direction(long long n, Pair<T> X, const T* __restrict__ d, const DevScalars* ds,
+            T* __restrict__ out) {
+  LsDirection<T> dir;
+  dir.init(ds, X, d);
+  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    out[i] = dir.at(i);
+}
+template <typename T>
+void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
+                      const DevScalars* ds, T* out) {
+  hipLaunchKernelGGL(k_direction<T>, dim3(G), dim3(kBlock), 0, s, n, X, d, ds, out);
 }
 
 #define LBFGSB_INST_BULK(T)                                                         \
@@ -515,16 +533,18 @@ void launch_gd(hipStream_t s, int G, long long n, Pair<T> Gv, const T* d,
                                        const uint8_t*, const DevScalars*, T*);      \
   template void launch_p5b<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
                               const T*, const T*, const int*, const uint8_t*,       \
-                              const T*, T*, DevScalars*, double*, double*);         \
+                              const T*, DevScalars*, double*, double*);             \
   template void launch_trial<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
                                 const T*, const T*, const int*, const uint8_t*,     \
-                                const T*, const T*, const DevScalars*);             \
+                                const T*, T*, const DevScalars*);                   \
   template void launch_subspace_point<T>(hipStream_t, int, long long, Pair<T>,      \
                                          Pair<T>, const T*, const T*, const int*,  \
                                          const uint8_t*, const T*,                  \
                                          const DevScalars*, T*);                    \
-  template void launch_gd<T>(hipStream_t, int, long long, Pair<T>, const T*,      \
-                             const DevScalars*, double*);
+  template void launch_gd<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,      \
+                             const T*, const DevScalars*, double*);                 \
+  template void launch_direction<T>(hipStream_t, int, long long, Pair<T>, const T*, \
+                                    const DevScalars*, T*);
 
 LBFGSB_INST_BULK(float)
 LBFGSB_INST_BULK(double)

@@ -54,7 +54,7 @@ const ScalarField kFields[] = {
     SF_I(gcp_nfreegcp), SF_I(gcp_unbnd), SF_L(gcp_kpassed), SF_L(gcp_last_idx),
     SF_D(gcp_last_t), SF_D(gcp_f1), SF_D(gcp_f2), SF_D(gcp_f2org), SF_D(gcp_dtm),
     SF_D(gcp_tsum), SF_D(gcp_bkmin), SF_L(gcp_ibkmin), SF_D(gcp_D0), SF_I(gcp_round),
-    SF_I(gcp_floor_hits), SF_D(alpha), SF_L(ibd), SF_I(write_trial), SF_I(trial_src),
+    SF_I(gcp_floor_hits), SF_D(alpha), SF_L(ibd), SF_I(write_trial), SF_I(trial_src), SF_I(trial_has_z), SF_I(d_stored), SF_I(trial_d),
     SF_I(trial_dst), SF_I(has_after_iteration), SF_I(has_custom_stop), SF_I(gcp_pending),
     SF_I(upd_done), SF_I(p4_flagged), SF_I(gcp_rounds_run), SF_L(gcp_passed_total),
     SF_L(cand_count), SF_I(gcp_need_select), SF_I(gcp_ranges)};

@@ -370,6 +370,9 @@ __device__ inline void request_trial(DevScalars* ds) {
   } else {
     ds->write_trial = 1;
     ds->trial_has_z = 0;  // k_trial overwrites it
+    // the first trial that leaves z stores d = z - x before it does
+    ds->trial_d = !ds->d_stored;
+    ds->d_stored = 1;
   }
 }
 
@@ -629,6 +632,7 @@ __device__ inline void logic_ls_init(DevScalars* s, double dtd, double gd, doubl
   s->iback = 0;
   s->csave = 0;
   s->trial_has_z = 1;  // the subspace pass wrote z into the trial buffer
+  s->d_stored = 0;     // and did not store d
   s->gd = gd;
   s->gdold = s->gd;
   if (s->gd >= 0) {

@@ -307,7 +307,7 @@ struct Engine final : EngineBase {
         launch_alpha(st, ds, pmin, pmini, G);
         break;
       case K_P5B:
-        launch_p5b<T>(st, G, n, dX, dG, l, u, nbd, iwhere, dsub, d, ds, part, pmin);
+        launch_p5b<T>(st, G, n, dX, dG, l, u, nbd, iwhere, dsub, ds, part, pmin);
         break;
       case K_LSINIT:
         launch_ls_init(st, ds, part, pmin, G);
@@ -316,7 +316,7 @@ struct Engine final : EngineBase {
         launch_trial<T>(st, G, n, dX, dG, l, u, nbd, iwhere, dsub, d, ds);
         break;
       case K_GD:
-        launch_gd<T>(st, G, n, dG, d, ds, part);
+        launch_gd<T>(st, G, n, dX, dG, d, ds, part);
         break;
       case K_LSSTEP:
         launch_ls_step(st, ds, part, G, f);
@@ -428,16 +428,17 @@ struct Engine final : EngineBase {
     if (name == "g") return dcopy(Gr[c], nn * sizeof(T));
     if (name == "xb") return dcopy(X[c ^ 1], nn * sizeof(T));
     if (name == "gb") return dcopy(Gr[c ^ 1], nn * sizeof(T));
-    if (name == "d") return dcopy(d, nn * sizeof(T));
-    if (name == "xcp" || name == "z") {
-      // the Cauchy point or the subspace point as the passes after the
-      // search compute them, from the current x, g, class, search outcome
-      // (and d_sub, alpha for z)
+    if (name == "xcp" || name == "z" || name == "d") {
+      // the Cauchy point, the subspace point or the line-search direction
+      // as the passes compute them from the current state (d is stored only
+      // once a trial moves off z)
       if (!insp && dmalloc(insp, nn)) return -1;
       if (name == "xcp")
         launch_cauchy_point<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, insp);
-      else
+      else if (name == "z")
         launch_subspace_point<T>(st, G, n, dX, dG, l, u, nbd, iwhere, dsub, ds, insp);
+      else
+        launch_direction<T>(st, G, n, dX, d, ds, insp);
       HIPCHK(hipGetLastError());
       return dcopy(insp, nn * sizeof(T));
     }
@@ -509,7 +510,6 @@ struct Engine final : EngineBase {
     if (name == "g") return hcopy(Gr[c], nn * sizeof(T));
     if (name == "xb") return hcopy(X[c ^ 1], nn * sizeof(T));
     if (name == "gb") return hcopy(Gr[c ^ 1], nn * sizeof(T));
-    if (name == "d") return hcopy(d, nn * sizeof(T));
     if (name == "iwbits") return hcopy(iwhere, nn);
     if (name == "ws" || name == "wy") {
       std::vector<T> w(wsize(nn));

@@ -49,12 +49,12 @@ void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> G
                          const DevScalars* ds, T* xcp);
 template <typename T>
 void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-                const T* u, const int* nbd, const uint8_t* iwhere, const T* dsub, T* d,
+                const T* u, const int* nbd, const uint8_t* iwhere, const T* dsub,
                 DevScalars* ds, double* part, double* pmin);
 template <typename T>
 void launch_trial(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-                  const T* u, const int* nbd, const uint8_t* iwhere, const T* dsub,
-                  const T* d, const DevScalars* ds);
+                  const T* u, const int* nbd, const uint8_t* iwhere, const T* dsub, T* d,
+                  const DevScalars* ds);
 // The subspace point z of every variable from the current state into out
 // (inspection only).
 template <typename T>
@@ -62,8 +62,12 @@ void launch_subspace_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T>
                            const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
                            const T* dsub, const DevScalars* ds, T* out);
 template <typename T>
-void launch_gd(hipStream_t s, int G, long long n, Pair<T> Gv, const T* d,
+void launch_gd(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* d,
                const DevScalars* ds, double* part);
+// The line-search direction d of every variable into out (inspection only).
+template <typename T>
+void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
+                      const DevScalars* ds, T* out);
 
 // small.hip
 void launch_rowsum(hipStream_t s, const double* part, int G, int v0, int nrows, DevScalars* ds,

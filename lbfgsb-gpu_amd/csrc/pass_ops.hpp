@@ -33,7 +33,7 @@ struct PassAOp {
   const T* __restrict__ d;
   T* __restrict__ tout;  // breakpoint t of every variable (+inf: none), or null
   int col, itail;
-  bool upd, stp1;
+  bool upd, stp1, dstored;
   T stp;
   RingReads rr;
   double acc[PA_COUNT];
@@ -69,6 +69,7 @@ struct PassAOp {
     upd = FUSED && ds->updatd;
     stp = (T)ds->stp;
     stp1 = ds->stp == 1.0;
+    dstored = ds->d_stored;
     // the slot being replaced by the new pair is not read
     rr = ring_reads(ds->head, ds->m, col, upd ? itail : -1);
 #pragma unroll
@@ -87,7 +88,7 @@ struct PassAOp {
     e.iwb = iwhere[i];
     e.xo = FUSED ? xold[i] : (T)0;
     e.go = upd ? gold[i] : (T)0;
-    e.dd = upd ? d[i] : (T)0;
+    e.dd = upd && dstored ? d[i] : (T)0;
     load_ring_raw(W, i, rr, e.yv, e.sv);
   }
   __device__ __forceinline__ void keep(const In& e) const {
@@ -106,7 +107,9 @@ struct PassAOp {
       acc_prod(acc[PA_TF], xd, xd);
       if (upd) {
         ys = gi - e.go;
-        ss = stp1 ? e.dd : e.dd * stp;
+        // d = z - x_old; unless a trial stored it, the accepted point is z
+        const T dd = dstored ? e.dd : xi - e.xo;
+        ss = stp1 ? dd : dd * stp;
         acc_prod(acc[PA_RR], ys, ys);
         st_nt(&W[widx(i, itail)], ys);
         st_nt(&W[widx(i, 8 + itail)], ss);
@@ -536,7 +539,6 @@ struct P5bOp {
   const uint8_t* __restrict__ iwhere;
   T* __restrict__ z;
   const T* __restrict__ dsub;
-  T* __restrict__ d;
   bool sub, shortcut, pin, want_stpmx;
   T alpha;
   long long ibd;
@@ -552,7 +554,7 @@ struct P5bOp {
 
   __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
                               const T* u_, const int* nbd_, const uint8_t* iwhere_,
-                              const T* dsub_, T* d_) {
+                              const T* dsub_) {
     const int cur = ds->phase & 1;
     x = X[cur];
     g = Gv[cur];
@@ -562,7 +564,6 @@ struct P5bOp {
     iwhere = iwhere_;
     z = X[cur ^ 1];
     dsub = dsub_;
-    d = d_;
     sub = !ds->skip_subsm;
     shortcut = ds->shortcut;
     alpha = (T)ds->alpha;
@@ -610,8 +611,7 @@ struct P5bOp {
     const T xi = e.x;
     const T zi = subspace_point(i, e);
     z[i] = zi;  // read back soon (objective, k_gd): normal stores
-    const T di = zi - xi;
-    d[i] = di;
+    const T di = zi - xi;  // d is not stored: z - x while z is the trial point
     acc[P5_DTD] += (double)di * (double)di;
     acc[P5_GD] += (double)e.g * (double)di;
     if (want_stpmx) {
@@ -634,5 +634,51 @@ struct P5bOp {
     }
   }
 };
+
+// ---------------------------------------------------------------------------
+// The line search on the trial buffer (lbfgsbcpu.cpp:1312-1335).  The
+// subspace pass leaves z in X[cur ^ 1] and stores no d: while z is the trial
+// point, d = z - x is the difference of the two x buffers; the first trial
+// that moves off z stores d first (trial_d).
+
+// d_i for g'd after an evaluation of the trial point.
+template <typename T>
+struct LsDirection {
+  bool stored;
+  const T* __restrict__ d;
+  const T* __restrict__ x;
+  const T* __restrict__ xt;
+  __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, const T* d_) {
+    const int cur = ds->phase & 1;
+    stored = ds->d_stored;
+    d = d_;
+    x = X[cur];
+    xt = X[cur ^ 1];
+  }
+  __device__ __forceinline__ T at(long long i) const {
+    return stored ? d[i] : (T)(xt[i] - x[i]);
+  }
+};
+
+// Trial point i: x_t = stp d + x, or z for a unit step (recomputed when a
+// trial since has overwritten it; then d is stored).
+template <typename T>
+__device__ __forceinline__ void trial_element(long long i, const DevScalars* ds,
+                                              const P5bOp<T>& sub, const T* __restrict__ xa,
+                                              T* __restrict__ xb, T* __restrict__ d, double stp,
+                                              bool trial_d) {
+  if (stp == 1.0) {
+    typename P5bOp<T>::In e;
+    sub.load(e, i);
+    xb[i] = sub.subspace_point(i, e);
+  } else if (trial_d) {
+    const T di = xb[i] - xa[i];  // xb still holds z
+    d[i] = di;
+    xb[i] = (T)(stp * (double)di + (double)xa[i]);
+  } else {
+    xb[i] = (T)(stp * (double)d[i] + (double)xa[i]);
+  }
+  (void)ds;
+}
 
 }  // namespace lbfgsb

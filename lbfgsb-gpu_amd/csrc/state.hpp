@@ -154,8 +154,10 @@ struct DevScalars {
   // ---- trial point bookkeeping: k_trial writes X[trial_dst] from
   // X[trial_src] when write_trial is set.  The subspace pass leaves the
   // subspace point z in the trial buffer (trial_has_z), which is the trial
-  // point of a unit step.
-  int write_trial, trial_src, trial_dst, trial_has_z;
+  // point of a unit step; the direction d = z - x is stored only by the
+  // first trial that overwrites z (d_stored; trial_d: this k_trial does it),
+  // and is otherwise the difference of the two x buffers.
+  int write_trial, trial_src, trial_dst, trial_has_z, d_stored, trial_d;
   int has_after_iteration, has_custom_stop;
 
   // ---- append counters (breakpoint list, round candidates)
