@@ -59,8 +59,8 @@ def kernel_bytes_per_elem(s: int) -> dict:
 # fp32), from two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE, KiB per
 # dispatch) scaled by what the same counters read for a 64 MiB copy kernel
 # (fetch x1.999, write x1.000); see profiles/plain_rosenbrock_16M_fp32_hbm_traffic.md.
-MEASURED_TRAFFIC_16M_FP32 = {"pass_a_fused": 1649638490, "p4": 1589936986, "p5a": 1495306003,
-                             "p5b": 620720494, "trial": 134188562, "gd": 134232095}
+MEASURED_TRAFFIC_16M_FP32 = {"pass_a_fused": 1629668715, "p4": 1457059587, "p5a": 1493465866,
+                             "p5b": 486596373, "gd": 201308465}
 
 
 def b_alg_per_iter(n: int, s: int) -> float:
