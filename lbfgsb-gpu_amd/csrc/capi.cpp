@@ -26,7 +26,8 @@ struct lbfgsb_problem {
   ObjDesc desc;
   double* part;
   double* fdev;
-  double* fhost;
+  HostF* fhost;  // fine-grained pinned: the objective publishes f here
+  unsigned fseq;
   int G;
 };
 
@@ -213,12 +214,11 @@ int minimize_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x, const in
 template <typename R>
 int problem_funcgrad(lbfgsb_problem* p, R* x, R* f, R* g, void* stream) {
   hipStream_t st = (hipStream_t)stream;
-  launch_objective<R>(st, p->desc, p->n, x, g, p->part, p->G, p->fdev);
-  if (hipMemcpyAsync(p->fhost, p->fdev, sizeof(double), hipMemcpyDeviceToHost, st) !=
-      hipSuccess)
-    return -1;
-  if (hipStreamSynchronize(st) != hipSuccess) return -1;
-  *f = (R)*p->fhost;
+  launch_objective<R>(st, p->desc, p->n, x, g, p->part, p->G, p->fdev, p->fhost, ++p->fseq);
+  if (hipGetLastError() != hipSuccess) return -1;
+  // f is final on return, as the reference's callback contract requires
+  if (mail_wait(st, &p->fhost->s, p->fseq) != hipSuccess) return -1;
+  *f = (R)p->fhost->f;
   return 0;
 }
 
@@ -398,10 +398,12 @@ lbfgsb_problem* lbfgsb_problem_create(int dtype, int kind, long long n, long lon
   p->G = (int)(blocks < kMaxBlocks ? (blocks < 1 ? 1 : blocks) : kMaxBlocks);
   if (hipMalloc((void**)&p->part, (size_t)p->G * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&p->fdev, sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void**)&p->fhost, sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&p->fhost, sizeof(HostF), hipHostMallocCoherent) != hipSuccess) {
     delete p;
     return nullptr;
   }
+  std::memset(p->fhost, 0, sizeof(HostF));
+  p->fseq = 0;
   return p;
 }
 

@@ -66,7 +66,9 @@ struct Engine final : EngineBase {
   T* insp = nullptr;
   uint8_t* iwhere = nullptr;
   DevScalars* ds = nullptr;
-  DevScalars* hs = nullptr;
+  DevScalars* hs = nullptr;  // fine-grained pinned: the end of a chain writes it directly
+  MailSeq* mseq = nullptr;
+  unsigned mail_seq = 0;
   double *part = nullptr, *pmax = nullptr, *pmin = nullptr;
   long long* pmini = nullptr;
   Cand *ca = nullptr, *cb = nullptr;
@@ -115,8 +117,10 @@ struct Engine final : EngineBase {
       return -1;
     HIPCHK(hipMemsetAsync(W, 0, wsize(nn) * sizeof(T), st));
     if (dmalloc(ds, 1)) return -1;
-    HIPCHK(hipHostMalloc((void**)&hs, sizeof(DevScalars), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&hs, sizeof(DevScalars), hipHostMallocCoherent));
     std::memset(hs, 0, sizeof(DevScalars));
+    HIPCHK(hipHostMalloc((void**)&mseq, sizeof(MailSeq), hipHostMallocCoherent));
+    std::memset(mseq, 0, sizeof(MailSeq));
     const size_t nv = std::max<int>(P4_COUNT, PA_COUNT);
     if (dmalloc(part, nv * G) || dmalloc(pmax, G) || dmalloc(pmin, G) || dmalloc(pmini, G))
       return -1;
@@ -142,11 +146,13 @@ struct Engine final : EngineBase {
     ev_pool.pop_back();
     return e;
   }
-  // fold finished event pairs into the per-step totals (stream is idle)
+  // fold finished event pairs into the per-step totals (each pair precedes
+  // a completed synchronisation point on the stream)
   void drain_events() {
     for (const Pending& p : pending) {
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      if (hipEventSynchronize(p.b) == hipSuccess &&
+          hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
         step_ms[p.step] += ms;
         step_n[p.step] += 1;
       }
@@ -186,6 +192,7 @@ struct Engine final : EngineBase {
     hipFree(iwhere);
     hipFree(ds);
     hipHostFree(hs);
+    hipHostFree(mseq);
     hipFree(part);
     hipFree(pmax);
     hipFree(pmin);
@@ -212,6 +219,15 @@ struct Engine final : EngineBase {
   int read_scalars() override {
     HIPCHK(hipMemcpyAsync(hs, ds, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (!pending.empty()) drain_events();
+    return 0;
+  }
+  // The same at the end of a chain: a kernel writes the state into hs and
+  // publishes a sequence number the host spins on (host_mailbox.hpp).
+  int read_status() {
+    launch_publish(st, ds, hs, mseq, ++mail_seq);
+    HIPCHK(hipGetLastError());
+    HIPCHK(mail_wait(st, mseq, mail_seq));
     if (!pending.empty()) drain_events();
     return 0;
   }
@@ -376,7 +392,7 @@ struct Engine final : EngineBase {
     while (j < steps.size() && steps[j] != stop_step) j++;
     for (size_t k = i; k < j; k++) enqueue(steps[k], f);
     HIPCHK(hipGetLastError());
-    if (read_scalars()) return -1;
+    if (read_status()) return -1;
     if (hs->status == ST_GCP_ROUNDS) {
       hipEvent_t ea = (profile_from >= 0 && hs->iter >= profile_from &&
                        ((profile_mask >> K_END) & 1u))
@@ -399,7 +415,7 @@ struct Engine final : EngineBase {
       if (p < j) {
         for (size_t k = p; k < j; k++) enqueue(steps[k], f);
         HIPCHK(hipGetLastError());
-        if (read_scalars()) return -1;
+        if (read_status()) return -1;
       }
     }
     return hs->status;

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "common.hpp"
+#include "host_mailbox.hpp"
 #include "state.hpp"
 
 namespace lbfgsb {
@@ -70,6 +71,9 @@ void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
                       const DevScalars* ds, T* out);
 
 // small.hip
+// The device state into `host` (fine-grained pinned memory), then seq into m.
+void launch_publish(hipStream_t s, const DevScalars* ds, DevScalars* host, MailSeq* m,
+                    unsigned seq);
 void launch_rowsum(hipStream_t s, const double* part, int G, int v0, int nrows, DevScalars* ds,
                    int p4_rows);
 void launch_resume(hipStream_t s, DevScalars* ds, int reset_bp);

@@ -6,6 +6,7 @@
 // (examples/dsscfg/dsscfg_cuda.cu:43-133): a device kernel for g and an
 // energy reduction whose value is handed back as the host f.
 #include "common.hpp"
+#include "host_mailbox.hpp"
 #include "launch.hpp"
 #include "obj_device.hpp"
 #include "objectives.hpp"
@@ -24,9 +25,11 @@ k_obj(ObjDesc o, long long n, const T* __restrict__ x, T* __restrict__ g, double
   block_sum_store<1>(acc, sh, part + blockIdx.x, gridDim.x);
 }
 
-__global__ void s_obj_finish(const double* part, int G, double* f_out) {
+// f into f_out on the device and, when hf is given, into the host's pinned
+// mailbox followed by its sequence number.  One wave.
+__global__ void __launch_bounds__(kWave)
+s_obj_finish(const double* part, int G, double* f_out, HostF* hf, unsigned seq) {
   const int lane = threadIdx.x & 63;
-  if (threadIdx.x >= 64) return;
   // every partial of the lane (b = lane + 64 k < G <= kMaxBlocks) loaded
   // before the first two-sum, then summed in order of b
   constexpr int kPer = kMaxBlocks / kWave;
@@ -49,19 +52,23 @@ __global__ void s_obj_finish(const double* part, int G, double* f_out) {
     two_sum(s, os, s, e);
     c += oc + e;
   }
-  if (lane == 0) *f_out = s + c;
+  if (lane == 0) {
+    *f_out = s + c;
+    if (hf) hf->f = s + c;
+  }
+  if (hf) mail_publish(&hf->s, seq);
 }
 
 template <typename T>
 void launch_objective(hipStream_t s, const ObjDesc& o, long long n, const T* x, T* g,
-                      double* part, int G, double* f_dev) {
+                      double* part, int G, double* f_dev, HostF* hf, unsigned seq) {
   hipLaunchKernelGGL(k_obj<T>, dim3(G), dim3(kBlock), 0, s, o, n, x, g, part);
-  hipLaunchKernelGGL(s_obj_finish, dim3(1), dim3(64), 0, s, part, G, f_dev);
+  hipLaunchKernelGGL(s_obj_finish, dim3(1), dim3(kWave), 0, s, part, G, f_dev, hf, seq);
 }
 
 template void launch_objective<float>(hipStream_t, const ObjDesc&, long long, const float*,
-                                      float*, double*, int, double*);
+                                      float*, double*, int, double*, HostF*, unsigned);
 template void launch_objective<double>(hipStream_t, const ObjDesc&, long long, const double*,
-                                       double*, double*, int, double*);
+                                       double*, double*, int, double*, HostF*, unsigned);
 
 }  // namespace lbfgsb

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "host_mailbox.hpp"
+
 namespace lbfgsb {
 
 enum ObjKind : int { OBJK_GRID_QUAD = 1, OBJK_GRID_COS = 2, OBJK_ROSEN = 3, OBJK_PENTA = 4 };
@@ -19,9 +21,12 @@ struct ObjDesc {
   const void* a2;
 };
 
+// g, and f into f_dev; with hf, f also into the host mailbox hf followed by
+// the sequence number seq.
 template <typename T>
 void launch_objective(hipStream_t s, const ObjDesc& o, long long n, const T* x, T* g,
-                      double* part, int G, double* f_dev);
+                      double* part, int G, double* f_dev, HostF* hf = nullptr,
+                      unsigned seq = 0);
 
 void launch_export_index(hipStream_t s, const uint8_t* iwhere, long long n, int track,
                          unsigned* counts, unsigned long long* totals, int* index, int* indx2);

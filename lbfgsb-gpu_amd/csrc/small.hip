@@ -9,6 +9,7 @@
 #include "common.hpp"
 #include "dense.hpp"
 #include "control.hpp"
+#include "host_mailbox.hpp"
 #include "launch.hpp"
 #include "state.hpp"
 
@@ -365,7 +366,21 @@ s_ls_step(DevScalars* ds, const double* part, int G, double f) {
   ds_store(ds, s);
 }
 
+// The device state into the host's pinned copy, then the sequence number
+// (the end of every chain the host waits on).
+__global__ void __launch_bounds__(kBlock)
+s_publish(const DevScalars* ds, DevScalars* host, MailSeq* m, unsigned seq) {
+  const unsigned long long* src = (const unsigned long long*)ds;
+  unsigned long long* dst = (unsigned long long*)host;
+  for (int i = threadIdx.x; i < kDsWords; i += blockDim.x) dst[i] = src[i];
+  mail_publish(m, seq);
+}
+
 // ---------------------------------------------------------------------------
+void launch_publish(hipStream_t s, const DevScalars* ds, DevScalars* host, MailSeq* m,
+                    unsigned seq) {
+  hipLaunchKernelGGL(s_publish, dim3(1), dim3(kBlock), 0, s, ds, host, m, seq);
+}
 void launch_rowsum(hipStream_t s, const double* part, int G, int v0, int nrows, DevScalars* ds,
                    int p4_rows) {
   hipLaunchKernelGGL(k_rowsum, dim3(nrows), dim3(kBlock), 0, s, part, G, v0, ds, p4_rows);
