@@ -36,14 +36,15 @@ STEPS = {"pass_a_fused": 4, "p4": 7, "formk_subsm": 8, "p5a": 9, "alpha": 10, "p
 
 def kernel_bytes_per_elem(s: int) -> dict:
     """Algorithmic HBM bytes per element of each bulk pass at steady state
-    (col = m = 8, one memory update per iteration; nbd int32, iwhere 1 byte).
-      pass_a_fused: read x, x_old, g, g_old, l, u (6s), nbd, iwhere, the 14
+    (col = m = 8, one memory update per iteration; the class byte iwhere also
+    carries nbd, so a pass reads one byte per variable for both).
+      pass_a_fused: read x, x_old, g, g_old, l, u (6s), iwhere, the 14
                     retained W entries (14s); write the new S/Y pair (2s) and
                     the breakpoint t (s) the Cauchy search selects from (the
                     step d of a unit step is x - x_old)
-      p4:           read x, g, l, u (4s), nbd, iwhere, W (16s); write iwhere
-      p5a:          read x, g, l, u (4s), nbd, iwhere, W (16s); write d_sub (s)
-      p5b:          read x, g, d_sub, l, u (5s), nbd, iwhere; write z into the
+      p4:           read x, g, l, u (4s), iwhere, W (16s); write iwhere
+      p5a:          read x, g, l, u (4s), iwhere, W (16s); write d_sub (s)
+      p5b:          read x, g, d_sub, l, u (5s), iwhere; write z into the
                     trial buffer (s)
       trial:        nothing: the first trial is a unit step, whose trial point
                     z is already in place (the first non-unit step moves 4s
@@ -51,8 +52,8 @@ def kernel_bytes_per_elem(s: int) -> dict:
       gd:           read g_trial, x_trial, x (3s): d = z - x
     x_cp and r are recomputed per element by the passes that need them (from
     x, g, l, u, the class and W) instead of being stored and read back."""
-    return {"pass_a_fused": 23 * s + 5, "p4": 20 * s + 6, "p5a": 21 * s + 5,
-            "p5b": 6 * s + 5, "trial": 0, "gd": 3 * s}
+    return {"pass_a_fused": 23 * s + 1, "p4": 20 * s + 2, "p5a": 21 * s + 1,
+            "p5b": 6 * s + 1, "trial": 0, "gd": 3 * s}
 
 
 # HBM bytes per launch at steady state for the default workload (n = 16,777,216,
@@ -65,7 +66,7 @@ MEASURED_TRAFFIC_16M_FP32 = {"pass_a_fused": 1629668715, "p4": 1457059587, "p5a"
 
 def b_alg_per_iter(n: int, s: int) -> float:
     """Algorithmic bytes of one steady-state iteration with one (unit-step)
-    trial point: the sum of kernel_bytes_per_elem, n (73 s + 21)."""
+    trial point: the sum of kernel_bytes_per_elem, n (73 s + 5)."""
     return float(n) * sum(kernel_bytes_per_elem(s).values())
 
 

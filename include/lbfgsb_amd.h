@@ -117,7 +117,8 @@ int lbfgsb_engine_set_scalar(lbfgsb_engine* e, const char* name, double value);
  * (bulk type, n); xcp, z, d (bulk type, n; read only: the Cauchy point,
  * the subspace point and the line-search direction computed from the
  * current state, which the iteration does not keep as such);
- * iwhere (int32, n); iwbits (uint8, n); ws, wy (bulk type, n x m
+ * iwhere (int32, n); iwbits (uint8, n: bits 0-3 iwhere + 4, bit 4 free in the
+ * last partition, bit 5 free in the one before, bits 6-7 nbd); ws, wy (bulk type, n x m
  * row-major); index, indx2 (int32, n, 1-based as the reference); sy, ss,
  * wt (m x m), wn, snd (2m x 2m), p, c, mc, wv, pa (double). */
 int lbfgsb_engine_get_array(lbfgsb_engine* e, const char* name, void* out, long long count);

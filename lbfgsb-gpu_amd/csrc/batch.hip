@@ -281,7 +281,7 @@ __device__ __forceinline__ void b_active(const Prob<T>& P, BatchLds& L, Gang<GO>
       v[1] += 1.0;
       iw = (nb == 2 && ui - li <= 0) ? 3 : 0;
     }
-    P.iwhere[i] = iw_make(iw, true, true);
+    P.iwhere[i] = iw_make(iw, true, true, nb & 3);
   }
   bsum<3>(v, L.u.r.sh, L.red);
   gang_combine(G, L, 3, 0, 0, 0, 0, &L.gflag);
@@ -342,7 +342,7 @@ __device__ __forceinline__ void b_pass_a(const Prob<T>& P, BatchLds& L, Gang<GO>
   if (L.s.halt) return;
   using Op = PassAOp<T, FUSED>;
   Op op;
-  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.W, P.d);
+  op.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.W, P.d);
   for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     typename Op::In e;
     op.load(e, i);
@@ -368,7 +368,7 @@ template <typename T, bool GO>
 __device__ __forceinline__ void b_p4(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt) return;
   P4Op<T> op;
-  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.W);
+  op.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.W);
   const long long hi = P.hi;
   for (long long i0 = P.lo; i0 < hi; i0 += blockDim.x) {
     const long long i = i0 + threadIdx.x;
@@ -400,7 +400,7 @@ template <typename T, bool GO>
 __device__ __forceinline__ void b_p5a(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt || L.s.skip_subsm) return;
   P5aOp<T> op;
-  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.W, P.dsub);
+  op.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.W, P.dsub);
   for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     typename P5aOp<T>::In e;
     op.load(e, i);
@@ -422,7 +422,7 @@ template <typename T, bool GO>
 __device__ __forceinline__ void b_p5b(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
   if (L.s.halt) return;
   P5bOp<T> op;
-  op.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.dsub);
+  op.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.dsub);
   for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     typename P5bOp<T>::In e;
     op.load(e, i);
@@ -449,7 +449,7 @@ __device__ __forceinline__ void b_trial(const Prob<T>& P, BatchLds& L) {
   const double stp = L.s.stp;
   const bool trial_d = L.s.trial_d;
   P5bOp<T> sub;
-  sub.init(&L.s, P.X, P.G, P.l, P.u, P.nbd, P.iwhere, P.dsub);
+  sub.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.dsub);
   for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
     trial_element(i, &L.s, sub, xa, xb, P.d, stp, trial_d);
   __syncthreads();
@@ -485,7 +485,8 @@ __device__ __forceinline__ void b_select(const Prob<T>& P, BatchLds& L, Gang<GO>
     const long long i = i0 + threadIdx.x;
     bool take = false;
     T tb = 0;
-    if (i < P.hi && breakpoint_of(x[i], g[i], P.l[i], P.u[i], P.nbd[i], iw_get(P.iwhere[i]), tb)) {
+    const uint8_t iwb = i < P.hi ? P.iwhere[i] : (uint8_t)0;
+    if (i < P.hi && breakpoint_of(x[i], g[i], P.l[i], P.u[i], iw_nbd(iwb), iw_get(iwb), tb)) {
       const double t = (double)tb;
       take = t >= lo && t < hi;
     }

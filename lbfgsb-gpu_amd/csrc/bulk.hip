@@ -86,8 +86,9 @@ k_active(long long n, T* __restrict__ x, const T* __restrict__ l,
       v[1] += 1.0;
       iw = (nb == 2 && ui - li <= 0) ? 3 : 0;
     }
-    // every variable starts in the free set of the (empty) previous partition
-    iwhere[i] = iw_make(iw, true, true);
+    // every variable starts in the free set of the (empty) previous
+    // partition; the byte carries nbd from here on (validated by k_errclb)
+    iwhere[i] = iw_make(iw, true, true, nb & 3);
   }
   block_sum_store<3>(v, sh, part + blockIdx.x, gridDim.x);
 }
@@ -128,15 +129,14 @@ k_projgr(long long n, const T* __restrict__ x, const T* __restrict__ g,
 template <typename T, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
 k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
-         const T* __restrict__ u, const int* __restrict__ nbd,
-         uint8_t* __restrict__ iwhere, T* __restrict__ W, const T* __restrict__ d,
+         const T* __restrict__ u, uint8_t* __restrict__ iwhere, T* __restrict__ W, const T* __restrict__ d,
          T* __restrict__ tbrk, DevScalars* ds, double* part, double* pmax, double* pmin,
          long long* pmini) {
   __shared__ double sh[kWavesPerBlock * PA_COUNT];
   if (ds->halt) return;
   using Op = PassAOp<T, FUSED>;
   Op op;
-  op.init(ds, X, Gv, l, u, nbd, iwhere, W, d, tbrk);
+  op.init(ds, X, Gv, l, u, iwhere, W, d, tbrk);
   if constexpr (sizeof(T) == 4) {
     // FP32: two consecutive runs of 256 elements per block step, software
     // pipelined: the loads of one run are in flight while the other run is
@@ -199,13 +199,12 @@ k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
-     const T* __restrict__ u, const int* __restrict__ nbd,
-     uint8_t* __restrict__ iwhere, const T* __restrict__ W, DevScalars* ds, double* part) {
+     const T* __restrict__ u, uint8_t* __restrict__ iwhere, const T* __restrict__ W, DevScalars* ds, double* part) {
   __shared__ double sh[kWavesPerBlock * P4_CORR];
   __shared__ double s_corr[kWavesPerBlock][136];
   if (ds->halt) return;
   P4Op<T> op;
-  op.init(ds, X, Gv, l, u, nbd, iwhere, W);
+  op.init(ds, X, Gv, l, u, iwhere, W);
   if constexpr (sizeof(T) == 4) {
     // FP32: each block takes two consecutive runs of 256 elements per step,
     // software pipelined: one run's loads are in flight while the other run
@@ -260,8 +259,7 @@ k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_cauchy_point(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
-               const T* __restrict__ u, const int* __restrict__ nbd,
-               const uint8_t* __restrict__ iwhere, const DevScalars* ds, T* __restrict__ xcp) {
+               const T* __restrict__ u, const uint8_t* __restrict__ iwhere, const DevScalars* ds, T* __restrict__ xcp) {
   CauchyOutcome<T> co;
   co.init(ds);
   const int cur = ds->phase & 1;
@@ -270,7 +268,7 @@ k_cauchy_point(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
     int iw = iw_get(iwhere[i]);
-    xcp[i] = co.point(i, x[i], g[i], l[i], u[i], nbd[i], iw);
+    xcp[i] = co.point(i, x[i], g[i], l[i], u[i], iw_nbd(iwhere[i]), iw);
   }
 }
 
@@ -279,12 +277,12 @@ k_cauchy_point(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_p5a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __restrict__ u,
-      const int* __restrict__ nbd, const uint8_t* __restrict__ iwhere,
+      const uint8_t* __restrict__ iwhere,
       const T* __restrict__ W, T* __restrict__ dsub, DevScalars* ds, double* pmin,
       long long* pmini) {
   if (ds->halt || ds->skip_subsm) return;
   P5aOp<T> op;
-  op.init(ds, X, Gv, l, u, nbd, iwhere, W, dsub);
+  op.init(ds, X, Gv, l, u, iwhere, W, dsub);
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
     typename P5aOp<T>::In e;
@@ -321,13 +319,12 @@ k_p5a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __re
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_p5b(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
-      const T* __restrict__ u, const int* __restrict__ nbd,
-      const uint8_t* __restrict__ iwhere, const T* __restrict__ dsub, DevScalars* ds,
+      const T* __restrict__ u, const uint8_t* __restrict__ iwhere, const T* __restrict__ dsub, DevScalars* ds,
       double* part, double* pmin) {
   __shared__ double sh[kWavesPerBlock * P5_COUNT];
   if (ds->halt) return;
   P5bOp<T> op;
-  op.init(ds, X, Gv, l, u, nbd, iwhere, dsub);
+  op.init(ds, X, Gv, l, u, iwhere, dsub);
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
     typename P5bOp<T>::In e;
@@ -352,12 +349,11 @@ template <typename T>
 __device__ __forceinline__ void subspace_points(long long n, Pair<T> X, Pair<T> Gv,
                                                 const T* __restrict__ l,
                                                 const T* __restrict__ u,
-                                                const int* __restrict__ nbd,
                                                 const uint8_t* __restrict__ iwhere,
                                                 const T* __restrict__ dsub,
                                                 const DevScalars* ds, T* __restrict__ out) {
   P5bOp<T> op;
-  op.init(ds, X, Gv, l, u, nbd, iwhere, dsub);
+  op.init(ds, X, Gv, l, u, iwhere, dsub);
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock) {
     typename P5bOp<T>::In e;
@@ -374,7 +370,7 @@ __device__ __forceinline__ void subspace_points(long long n, Pair<T> X, Pair<T> 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_trial(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __restrict__ u,
-        const int* __restrict__ nbd, const uint8_t* __restrict__ iwhere,
+        const uint8_t* __restrict__ iwhere,
         const T* __restrict__ dsub, T* __restrict__ d, const DevScalars* ds) {
   if (!ds->write_trial) return;
   const T* __restrict__ xa = X[ds->trial_src];
@@ -382,7 +378,7 @@ k_trial(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __
   const double stp = ds->stp;
   const bool trial_d = ds->trial_d;
   P5bOp<T> sub;
-  sub.init(ds, X, Gv, l, u, nbd, iwhere, dsub);
+  sub.init(ds, X, Gv, l, u, iwhere, dsub);
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock)
     trial_element(i, ds, sub, xa, xb, d, stp, trial_d);
@@ -393,10 +389,9 @@ k_trial(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_subspace_point(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
-                 const T* __restrict__ u, const int* __restrict__ nbd,
-                 const uint8_t* __restrict__ iwhere, const T* __restrict__ dsub,
+                 const T* __restrict__ u, const uint8_t* __restrict__ iwhere, const T* __restrict__ dsub,
                  const DevScalars* ds, T* __restrict__ out) {
-  subspace_points(n, X, Gv, l, u, nbd, iwhere, dsub, ds, out);
+  subspace_points(n, X, Gv, l, u, iwhere, dsub, ds, out);
 }
 
 // g_b . d after an evaluation (lbfgsbcpu.cpp:1312).
@@ -437,57 +432,53 @@ void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g,
 }
 template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X,
-                   Pair<T> Gv, const T* l, const T* u, const int* nbd,
-                   uint8_t* iwhere, T* W, const T* d, T* tbrk, DevScalars* ds,
+                   Pair<T> Gv, const T* l, const T* u, uint8_t* iwhere, T* W, const T* d, T* tbrk, DevScalars* ds,
                    double* part, double* pmax, double* pmin, long long* pmini) {
   if (fused)
     hipLaunchKernelGGL((k_pass_a<T, true>), dim3(G), dim3(kBlock), 0, s, n, X, Gv,
-                       l, u, nbd, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini);
+                       l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini);
   else
     hipLaunchKernelGGL((k_pass_a<T, false>), dim3(G), dim3(kBlock), 0, s, n, X,
-                       Gv, l, u, nbd, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini);
+                       Gv, l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini);
 }
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
-               const T* l, const T* u, const int* nbd, uint8_t* iwhere,
+               const T* l, const T* u, uint8_t* iwhere,
                const T* W, DevScalars* ds, double* part) {
-  hipLaunchKernelGGL(k_p4<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd,
-                     iwhere, W, ds, part);
+  hipLaunchKernelGGL(k_p4<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere, W, ds, part);
 }
 template <typename T>
 void launch_p5a(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-                const T* u, const int* nbd, const uint8_t* iwhere, const T* W, T* dsub,
+                const T* u, const uint8_t* iwhere, const T* W, T* dsub,
                 DevScalars* ds, double* pmin, long long* pmini) {
-  hipLaunchKernelGGL(k_p5a<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd, iwhere,
+  hipLaunchKernelGGL(k_p5a<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere,
                      W, dsub, ds, pmin, pmini);
 }
 template <typename T>
 void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-                         const T* u, const int* nbd, const uint8_t* iwhere,
+                         const T* u, const uint8_t* iwhere,
                          const DevScalars* ds, T* xcp) {
-  hipLaunchKernelGGL(k_cauchy_point<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd, iwhere,
+  hipLaunchKernelGGL(k_cauchy_point<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere,
                      ds, xcp);
 }
 template <typename T>
 void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
-                const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
+                const T* l, const T* u, const uint8_t* iwhere,
                 const T* dsub, DevScalars* ds, double* part, double* pmin) {
-  hipLaunchKernelGGL(k_p5b<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd,
-                     iwhere, dsub, ds, part, pmin);
+  hipLaunchKernelGGL(k_p5b<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere, dsub, ds, part, pmin);
 }
 template <typename T>
 void launch_trial(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-                  const T* u, const int* nbd, const uint8_t* iwhere, const T* dsub,
+                  const T* u, const uint8_t* iwhere, const T* dsub,
                   T* d, const DevScalars* ds) {
-  hipLaunchKernelGGL(k_trial<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd, iwhere,
+  hipLaunchKernelGGL(k_trial<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere,
                      dsub, d, ds);
 }
 template <typename T>
 void launch_subspace_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
-                           const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
+                           const T* l, const T* u, const uint8_t* iwhere,
                            const T* dsub, const DevScalars* ds, T* out) {
-  hipLaunchKernelGGL(k_subspace_point<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, nbd,
-                     iwhere, dsub, ds, out);
+  hipLaunchKernelGGL(k_subspace_point<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere, dsub, ds, out);
 }
 template <typename T>
 void launch_gd(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* d,
@@ -519,26 +510,26 @@ void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
   template void launch_projgr<T>(hipStream_t, int, long long, const T*, const T*,   \
                                  const T*, const T*, const int*, double*);          \
   template void launch_pass_a<T>(hipStream_t, int, bool, long long, Pair<T>,      \
-                                 Pair<T>, const T*, const T*, const int*,         \
+                                 Pair<T>, const T*, const T*,         \
                                  uint8_t*, T*, const T*, T*, DevScalars*, double*,  \
                                  double*, double*, long long*);                     \
   template void launch_p4<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,     \
-                             const T*, const T*, const int*, uint8_t*, const T*,    \
+                             const T*, const T*, uint8_t*, const T*,    \
                              DevScalars*, double*);                                 \
   template void launch_p5a<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,      \
-                              const T*, const T*, const int*, const uint8_t*,       \
+                              const T*, const T*, const uint8_t*,       \
                               const T*, T*, DevScalars*, double*, long long*);      \
   template void launch_cauchy_point<T>(hipStream_t, int, long long, Pair<T>,      \
-                                       Pair<T>, const T*, const T*, const int*,    \
+                                       Pair<T>, const T*, const T*,    \
                                        const uint8_t*, const DevScalars*, T*);      \
   template void launch_p5b<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
-                              const T*, const T*, const int*, const uint8_t*,       \
+                              const T*, const T*, const uint8_t*,       \
                               const T*, DevScalars*, double*, double*);             \
   template void launch_trial<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
-                                const T*, const T*, const int*, const uint8_t*,     \
+                                const T*, const T*, const uint8_t*,     \
                                 const T*, T*, const DevScalars*);                   \
   template void launch_subspace_point<T>(hipStream_t, int, long long, Pair<T>,      \
-                                         Pair<T>, const T*, const T*, const int*,  \
+                                         Pair<T>, const T*, const T*,  \
                                          const uint8_t*, const T*,                  \
                                          const DevScalars*, T*);                    \
   template void launch_gd<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,      \

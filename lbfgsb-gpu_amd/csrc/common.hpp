@@ -47,11 +47,15 @@ struct Pair {
 //   bits 0..3: iwhere + 4   (values -3, -1, 0, 1, 2, 3 -> 1, 3, 4, 5, 6, 7)
 //   bit 4    : free in the last free/active partition
 //   bit 5    : free in the partition before it (entering/leaving sets)
+//   bits 6..7: the variable's bound type nbd (0..3), copied in at set-up so
+//              that the passes read one byte per variable instead of the
+//              int32 nbd array as well
 __device__ __host__ inline int iw_get(uint8_t b) { return (int)(b & 15) - 4; }
 __device__ __host__ inline bool iw_wasfree(uint8_t b) { return (b >> 4) & 1; }
 __device__ __host__ inline bool iw_prevfree(uint8_t b) { return (b >> 5) & 1; }
-__device__ __host__ inline uint8_t iw_make(int iw, bool freelast, bool freeprev) {
-  return (uint8_t)((iw + 4) | (freelast ? 16 : 0) | (freeprev ? 32 : 0));
+__device__ __host__ inline int iw_nbd(uint8_t b) { return b >> 6; }
+__device__ __host__ inline uint8_t iw_make(int iw, bool freelast, bool freeprev, int nb) {
+  return (uint8_t)((iw + 4) | (freelast ? 16 : 0) | (freeprev ? 32 : 0) | (nb << 6));
 }
 
 // ---------------------------------------------------------------------------

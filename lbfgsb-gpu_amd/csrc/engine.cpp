@@ -287,7 +287,7 @@ struct Engine final : EngineBase {
         launch_resume(st, ds, 0);
         break;
       case K_PASSA_SA:
-        launch_pass_a<T>(st, G, false, n, dX, dG, l, u, nbd, iwhere, W, d, tbrk, ds, part, pmax,
+        launch_pass_a<T>(st, G, false, n, dX, dG, l, u, iwhere, W, d, tbrk, ds, part, pmax,
                          pmin, pmini);
         break;
       case K_UPD_REDUCE:
@@ -295,7 +295,7 @@ struct Engine final : EngineBase {
         launch_update_gcp(st, ds, part, pmin, pmini, G, 1);
         break;
       case K_PASSA_FUSED:
-        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, nbd, iwhere, W, d, tbrk, ds, part, pmax,
+        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, iwhere, W, d, tbrk, ds, part, pmax,
                          pmin, pmini);
         break;
       case K_GCP:
@@ -310,26 +310,26 @@ struct Engine final : EngineBase {
         launch_update_gcp(st, ds, part, pmin, pmini, G, 0);
         break;
       case K_P4:
-        launch_p4<T>(st, G, n, dX, dG, l, u, nbd, iwhere, W, ds, part);
+        launch_p4<T>(st, G, n, dX, dG, l, u, iwhere, W, ds, part);
         break;
       case K_FORMK:
         launch_rowsum(st, part, G, 0, P4_COUNT, ds, 1);
         launch_formk_subsm(st, ds, part, G);
         break;
       case K_P5A:
-        launch_p5a<T>(st, G, n, dX, dG, l, u, nbd, iwhere, W, dsub, ds, pmin, pmini);
+        launch_p5a<T>(st, G, n, dX, dG, l, u, iwhere, W, dsub, ds, pmin, pmini);
         break;
       case K_ALPHA:
         launch_alpha(st, ds, pmin, pmini, G);
         break;
       case K_P5B:
-        launch_p5b<T>(st, G, n, dX, dG, l, u, nbd, iwhere, dsub, ds, part, pmin);
+        launch_p5b<T>(st, G, n, dX, dG, l, u, iwhere, dsub, ds, part, pmin);
         break;
       case K_LSINIT:
         launch_ls_init(st, ds, part, pmin, G);
         break;
       case K_TRIAL:
-        launch_trial<T>(st, G, n, dX, dG, l, u, nbd, iwhere, dsub, d, ds);
+        launch_trial<T>(st, G, n, dX, dG, l, u, iwhere, dsub, d, ds);
         break;
       case K_GD:
         launch_gd<T>(st, G, n, dX, dG, d, ds, part);
@@ -450,9 +450,9 @@ struct Engine final : EngineBase {
       // once a trial moves off z)
       if (!insp && dmalloc(insp, nn)) return -1;
       if (name == "xcp")
-        launch_cauchy_point<T>(st, G, n, dX, dG, l, u, nbd, iwhere, ds, insp);
+        launch_cauchy_point<T>(st, G, n, dX, dG, l, u, iwhere, ds, insp);
       else if (name == "z")
-        launch_subspace_point<T>(st, G, n, dX, dG, l, u, nbd, iwhere, dsub, ds, insp);
+        launch_subspace_point<T>(st, G, n, dX, dG, l, u, iwhere, dsub, ds, insp);
       else
         launch_direction<T>(st, G, n, dX, d, ds, insp);
       HIPCHK(hipGetLastError());

@@ -32,35 +32,35 @@ void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g, co
 // Cauchy search's selections.
 template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X, Pair<T> Gv,
-                   const T* l, const T* u, const int* nbd, uint8_t* iwhere, T* W, const T* d,
+                   const T* l, const T* u, uint8_t* iwhere, T* W, const T* d,
                    T* tbrk, DevScalars* ds, double* part, double* pmax, double* pmin,
                    long long* pmini);
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-               const T* u, const int* nbd, uint8_t* iwhere, const T* W, DevScalars* ds,
+               const T* u, uint8_t* iwhere, const T* W, DevScalars* ds,
                double* part);
 template <typename T>
 void launch_p5a(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-                const T* u, const int* nbd, const uint8_t* iwhere, const T* W, T* dsub,
+                const T* u, const uint8_t* iwhere, const T* W, T* dsub,
                 DevScalars* ds, double* pmin, long long* pmini);
 // x_cp of every variable from the current state (inspection only).
 template <typename T>
 void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-                         const T* u, const int* nbd, const uint8_t* iwhere,
+                         const T* u, const uint8_t* iwhere,
                          const DevScalars* ds, T* xcp);
 template <typename T>
 void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-                const T* u, const int* nbd, const uint8_t* iwhere, const T* dsub,
+                const T* u, const uint8_t* iwhere, const T* dsub,
                 DevScalars* ds, double* part, double* pmin);
 template <typename T>
 void launch_trial(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
-                  const T* u, const int* nbd, const uint8_t* iwhere, const T* dsub, T* d,
+                  const T* u, const uint8_t* iwhere, const T* dsub, T* d,
                   const DevScalars* ds);
 // The subspace point z of every variable from the current state into out
 // (inspection only).
 template <typename T>
 void launch_subspace_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
-                           const T* l, const T* u, const int* nbd, const uint8_t* iwhere,
+                           const T* l, const T* u, const uint8_t* iwhere,
                            const T* dsub, const DevScalars* ds, T* out);
 template <typename T>
 void launch_gd(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* d,

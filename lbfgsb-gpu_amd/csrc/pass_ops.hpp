@@ -27,7 +27,6 @@ struct PassAOp {
   const T* __restrict__ gold;
   const T* __restrict__ l;
   const T* __restrict__ u;
-  const int* __restrict__ nbd;
   uint8_t* __restrict__ iwhere;
   T* __restrict__ W;
   const T* __restrict__ d;
@@ -44,13 +43,12 @@ struct PassAOp {
 
   struct In {
     T x, g, l, u, xo, go, dd;
-    int nb;
     uint8_t iwb;
     T yv[kMaxM], sv[kMaxM];
   };
 
   __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
-                              const T* u_, const int* nbd_, uint8_t* iwhere_, T* W_,
+                              const T* u_, uint8_t* iwhere_, T* W_,
                               const T* d_, T* tout_ = nullptr) {
     const int cur = ds->phase & 1;
     tout = tout_;
@@ -60,7 +58,6 @@ struct PassAOp {
     gold = Gv[cur ^ 1];
     l = l_;
     u = u_;
-    nbd = nbd_;
     iwhere = iwhere_;
     W = W_;
     d = d_;
@@ -84,7 +81,6 @@ struct PassAOp {
     e.g = g[i];
     e.l = l[i];
     e.u = u[i];
-    e.nb = nbd[i];
     e.iwb = iwhere[i];
     e.xo = FUSED ? xold[i] : (T)0;
     e.go = upd ? gold[i] : (T)0;
@@ -92,12 +88,12 @@ struct PassAOp {
     load_ring_raw(W, i, rr, e.yv, e.sv);
   }
   __device__ __forceinline__ void keep(const In& e) const {
-    keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.xo, e.go, e.dd, e.yv, e.sv);
+    keep_loads(e.x, e.g, e.l, e.u, e.iwb, e.xo, e.go, e.dd, e.yv, e.sv);
   }
   __device__ __forceinline__ void element(long long i, In& e) {
     mask_ring(rr, e.yv, e.sv);
     const T xi = e.x, gi = e.g, li = e.l, ui = e.u;
-    const int nb = e.nb;
+    const int nb = iw_nbd(e.iwb);
     const uint8_t iwb = e.iwb;
     T ys = 0, ss = 0;
     if (FUSED) {
@@ -163,7 +159,7 @@ struct PassAOp {
       // one stream less
       st_nt(&tout[i], isbrk ? tbrk : (T)__builtin_inf());
     } else {
-      iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb));
+      iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb), nb);
     }
   }
   __device__ __forceinline__ void finish() {
@@ -269,7 +265,6 @@ struct P4Op {
   const T* __restrict__ g;
   const T* __restrict__ l;
   const T* __restrict__ u;
-  const int* __restrict__ nbd;
   uint8_t* __restrict__ iwhere;
   const T* __restrict__ W;
   DevScalars* ds;
@@ -287,21 +282,19 @@ struct P4Op {
 
   struct In {
     T x, g, l, u;
-    int nb;
     uint8_t iwb;
     T yv[kMaxM], sv[kMaxM];
     T yn, sn;  // the newest pair again (a cache hit): no select over positions
   };
 
   __device__ __forceinline__ void init(DevScalars* ds_, Pair<T> X, Pair<T> Gv, const T* l_, const T* u_,
-                              const int* nbd_, uint8_t* iwhere_, const T* W_) {
+                              uint8_t* iwhere_, const T* W_) {
     ds = ds_;
     const int cur = ds->phase & 1;
     x = X[cur];
     g = Gv[cur];
     l = l_;
     u = u_;
-    nbd = nbd_;
     iwhere = iwhere_;
     W = W_;
     const int m = ds->m, head = ds->head;
@@ -334,7 +327,6 @@ struct P4Op {
     e.g = g[i];
     e.l = l[i];
     e.u = u[i];
-    e.nb = nbd[i];
     e.iwb = iwhere[i];
     load_ring_raw(W, i, rr, e.yv, e.sv);
     const T* __restrict__ row = W + widx((size_t)i, 0);
@@ -342,14 +334,14 @@ struct P4Op {
     e.sn = row[(8 + rr.newest) * kTile];
   }
   __device__ __forceinline__ void keep(const In& e) const {
-    keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.yv, e.sv, e.yn, e.sn);
+    keep_loads(e.x, e.g, e.l, e.u, e.iwb, e.yv, e.sv, e.yn, e.sn);
   }
   // Returns +1 / -1 for an entering / leaving variable, 0 otherwise.
   __device__ __forceinline__ int element(long long i, In& e) {
     mask_ring(rr, e.yv, e.sv);
     int flag = 0;
     const T xi = e.x, gi = e.g, li = e.l, ui = e.u;
-    const int nb = e.nb;
+    const int nb = iw_nbd(e.iwb);
     const uint8_t iwb = e.iwb;
     const bool wasfree = iw_wasfree(iwb);
     int iw = iw_get(iwb);
@@ -367,7 +359,7 @@ struct P4Op {
           flag = -1;
         }
       }
-      st_nt(&iwhere[i], iw_make(iw, freenow, wasfree));
+      st_nt(&iwhere[i], iw_make(iw, freenow, wasfree, nb));
     }
     // Positions j >= col hold zeros (mask_ring), so the sums below run over
     // all kMaxM positions without guards: unused accumulators collect exact
@@ -428,7 +420,6 @@ struct P5aOp {
   const T* __restrict__ g;
   const T* __restrict__ l;
   const T* __restrict__ u;
-  const int* __restrict__ nbd;
   const uint8_t* __restrict__ iwhere;
   const T* __restrict__ W;
   T* __restrict__ dsub;
@@ -443,20 +434,18 @@ struct P5aOp {
 
   struct In {
     T x, g, l, u;
-    int nb;
     uint8_t iwb;
     T yv[kMaxM], sv[kMaxM];
   };
 
   __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
-                                       const T* u_, const int* nbd_, const uint8_t* iwhere_,
+                                       const T* u_, const uint8_t* iwhere_,
                                        const T* W_, T* dsub_) {
     const int cur = ds->phase & 1;
     x = X[cur];
     g = Gv[cur];
     l = l_;
     u = u_;
-    nbd = nbd_;
     iwhere = iwhere_;
     W = W_;
     dsub = dsub_;
@@ -479,11 +468,10 @@ struct P5aOp {
     e.g = g[i];
     e.l = l[i];
     e.u = u[i];
-    e.nb = nbd[i];
     load_ring_raw(W, i, rr, e.yv, e.sv);
   }
   __device__ __forceinline__ void keep(const In& e) const {
-    keep_loads(e.x, e.g, e.l, e.u, e.nb, e.iwb, e.yv, e.sv);
+    keep_loads(e.x, e.g, e.l, e.u, e.iwb, e.yv, e.sv);
   }
   __device__ __forceinline__ void element(long long i, In& e) {
     int iw = iw_get(e.iwb);
@@ -491,7 +479,7 @@ struct P5aOp {
     if (!freenow) return;
     mask_ring(rr, e.yv, e.sv);
     // x_cp and r as the Cauchy-point pass formed them (r rounded to T)
-    const T zi = co.point(i, e.x, e.g, e.l, e.u, e.nb, iw);
+    const T zi = co.point(i, e.x, e.g, e.l, e.u, iw_nbd(e.iwb), iw);
     const T rt = (T)co.reduced(e.x, e.g, zi, e.yv, e.sv);
     double di = (double)rt;
 #pragma unroll
@@ -499,7 +487,7 @@ struct P5aOp {
       if (j < col) di += (double)e.yv[j] * wy[j] + (double)e.sv[j] * wsv[j];
     const T dk = (T)(di / theta);
     st_nt(&dsub[i], dk);
-    const int nb = e.nb;
+    const int nb = iw_nbd(e.iwb);
     if (nb != 0) {
       double cand = 2.0;
       if (dk < 0 && nb <= 2) {
@@ -535,7 +523,6 @@ struct P5bOp {
   const T* __restrict__ g;
   const T* __restrict__ l;
   const T* __restrict__ u;
-  const int* __restrict__ nbd;
   const uint8_t* __restrict__ iwhere;
   T* __restrict__ z;
   const T* __restrict__ dsub;
@@ -549,18 +536,16 @@ struct P5bOp {
   struct In {
     T x, g, l, u, dsub;
     uint8_t iwb;
-    int nb;
   };
 
   __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
-                              const T* u_, const int* nbd_, const uint8_t* iwhere_,
+                              const T* u_, const uint8_t* iwhere_,
                               const T* dsub_) {
     const int cur = ds->phase & 1;
     x = X[cur];
     g = Gv[cur];
     l = l_;
     u = u_;
-    nbd = nbd_;
     iwhere = iwhere_;
     z = X[cur ^ 1];
     dsub = dsub_;
@@ -581,16 +566,15 @@ struct P5bOp {
     e.u = u[i];
     e.iwb = iwhere[i];
     e.dsub = sub ? dsub[i] : (T)0;
-    e.nb = nbd[i];
   }
   __device__ __forceinline__ void keep(const In& e) const {
-    keep_loads(e.x, e.g, e.l, e.u, e.iwb, e.dsub, e.nb);
+    keep_loads(e.x, e.g, e.l, e.u, e.iwb, e.dsub);
   }
   // The subspace point of variable i (x_cp where there is no subspace step).
   __device__ __forceinline__ T subspace_point(long long i, const In& e) const {
     int iw = iw_get(e.iwb);
     const bool freenow = shortcut || iw <= 0;
-    T zi = co.point(i, e.x, e.g, e.l, e.u, e.nb, iw);  // x_cp
+    T zi = co.point(i, e.x, e.g, e.l, e.u, iw_nbd(e.iwb), iw);  // x_cp
     if (sub && freenow) {
       T dk = e.dsub;
       if (pin && i == ibd) {
@@ -615,7 +599,7 @@ struct P5bOp {
     acc[P5_DTD] += (double)di * (double)di;
     acc[P5_GD] += (double)e.g * (double)di;
     if (want_stpmx) {
-      const int nb = e.nb;
+      const int nb = iw_nbd(e.iwb);
       if (nb != 0) {
         if (di < 0 && nb <= 2) {
           const T a2 = e.l - xi;

@@ -56,7 +56,8 @@ def _load(engine, s, p):
     free_last = np.zeros(n, bool)
     idx = s.arr("index")
     free_last[idx[: st.nfree] - 1] = True
-    bits = ((iw + 4) | (free_last.astype(np.int64) << 4) | (free_last.astype(np.int64) << 5))
+    bits = ((iw + 4) | (free_last.astype(np.int64) << 4) | (free_last.astype(np.int64) << 5)
+            | (p.nbd.astype(np.int64) << 6))  # the class byte carries nbd
     engine.set_array("iwbits", bits.astype(np.uint8))
     vals = dict(col=st.col, head=st.head - 1, itail=max(st.itail - 1, 0), iupdat=st.iupdat,
                 iter=st.iter, updatd=st.updatd, theta=st.theta, f=st.f, cnstnd=st.cnstnd,
