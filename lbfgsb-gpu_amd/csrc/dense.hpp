@@ -125,47 +125,94 @@ __device__ __forceinline__ bool d_dpofa_r(double (&a)[NMAX][NMAX], int n) {
 }
 
 // p = M v with the 2col x 2col middle matrix through sy and the Cholesky
-// factor of T in wt (lbfgsbcpu.cpp:517-565).  sy, wt have ld 8.  The two
-// halves of v and p (Y part, S part) are kept in registers.
+// factor of T in wt (lbfgsbcpu.cpp:517-565).  sy, wt have ld 8.  Every entry
+// of sy's lower and wt's upper triangle is loaded before the first
+// operation, and the loops run over the fixed size 8 with the entries beyond
+// col masked out by selects, not branches: the loads go out together and the
+// independent divisions overlap.  Each used entry sees the reference's
+// operations in the reference's order; entries beyond col are never stored.
 __device__ inline int d_bmv(const double* __restrict__ sy, const double* __restrict__ wt,
                             int col, const double* v, double* p) {
   if (col == 0) return 0;
+  double S[8][8], Tu[8][8];  // S[i][k], k <= i; Tu[k][j], k <= j (0-based)
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (k <= i) S[i][k] = sy[i * 8 + k];
+      if (i <= k) Tu[i][k] = wt[i * 8 + k];
+    }
   double vy[8], vs[8], py[8], ps[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    vy[q] = q < col ? v[q] : 0.0;
-    vs[q] = q < col ? v[col + q] : 0.0;
-    py[q] = 0.0;
-    ps[q] = 0.0;
+    const double a = v[q], b = v[col + q < 16 ? col + q : 15];
+    vy[q] = q < col ? a : 0.0;
+    vs[q] = q < col ? b : 0.0;
   }
+  // dtrsl's zero-pivot test comes before any arithmetic: the first zero
+  // diagonal entry of T within col
+  int info = 0;
+#pragma unroll
+  for (int j = 7; j >= 0; j--)
+    if (j < col && Tu[j][j] == 0.0) info = j + 1;
+  if (info != 0) return info;
+  // ps_i = vs_i + sum_{k < i} sy(i,k) vy_k / sy(k,k)
   ps[0] = vs[0];
 #pragma unroll
-  for (int i = 2; i <= 8; i++) {
-    if (i > col) continue;
+  for (int i = 1; i < 8; i++) {
     double s = 0.0;
 #pragma unroll
-    for (int k = 1; k < 8; k++)
-      if (k <= i - 1) s = s + A1(sy, 8, i, k) * vy[k - 1] / A1(sy, 8, k, k);
-    ps[i - 1] = vs[i - 1] + s;
+    for (int k = 0; k < i; k++) s = s + S[i][k] * vy[k] / S[k][k];
+    ps[i] = vs[i] + s;
   }
-  int info = d_dtrsl_r<8>(wt, 8, col, ps, 11);
-  if (info != 0) return info;
+  // solve T' x = ps (dtrsl job 11): forward, ALGLIB blocked dots
+  ps[0] = ps[0] / Tu[0][0];
 #pragma unroll
-  for (int i = 1; i <= 8; i++)
-    if (i <= col) py[i - 1] = vy[i - 1] / sqrt(A1(sy, 8, i, i));
-  info = d_dtrsl_r<8>(wt, 8, col, ps, 1);
-  if (info != 0) return info;
+  for (int j = 1; j < 8; j++) {
+    if (j < col) {
+      double r = 0.0;
 #pragma unroll
-  for (int i = 1; i <= 8; i++)
-    if (i <= col) py[i - 1] = -py[i - 1] / sqrt(A1(sy, 8, i, i));
+      for (int blk = 0; blk < 2; blk++) {
+        if (4 * blk + 4 <= j) {
+          const int i = 4 * blk;
+          r += Tu[i][j] * ps[i] + Tu[i + 1][j] * ps[i + 1] + Tu[i + 2][j] * ps[i + 2] +
+               Tu[i + 3][j] * ps[i + 3];
+        }
+      }
 #pragma unroll
-  for (int i = 1; i <= 8; i++) {
-    if (i > col) continue;
+      for (int i = 0; i < 8; i++)
+        if (i >= (j / 4) * 4 && i < j) r += Tu[i][j] * ps[i];
+      ps[j] = ps[j] - r;
+      ps[j] = ps[j] / Tu[j][j];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) py[i] = vy[i] / sqrt(S[i][i]);
+  // solve T x = ps (dtrsl job 01): backward
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k == col - 1) ps[k] = ps[k] / Tu[k][k];
+#pragma unroll
+  for (int j = 7; j >= 1; j--) {
+    if (j <= col - 1) {
+      const double temp = -ps[j];
+#pragma unroll
+      for (int i = 0; i < j; i++) ps[i] += temp * Tu[i][j];
+      ps[j - 1] = ps[j - 1] / Tu[j - 1][j - 1];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) py[i] = -py[i] / sqrt(S[i][i]);
+  // py_i += sum_{i < k < col} sy(k,i) ps_k / sy(i,i)
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
     double s = 0.0;
 #pragma unroll
-    for (int k = 2; k <= 8; k++)
-      if (k >= i + 1 && k <= col) s = s + A1(sy, 8, k, i) * ps[k - 1] / A1(sy, 8, i, i);
-    py[i - 1] = py[i - 1] + s;
+    for (int k = i + 1; k < 8; k++) {
+      const double term = S[k][i] * ps[k] / S[i][i];
+      s = k < col ? s + term : s;
+    }
+    py[i] = py[i] + s;
   }
 #pragma unroll
   for (int q = 0; q < 8; q++) {
