@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(kBlock)
 k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
          const T* __restrict__ u, uint8_t* __restrict__ iwhere, T* __restrict__ W, const T* __restrict__ d,
          T* __restrict__ tbrk, DevScalars* ds, double* part, double* pmax, double* pmin,
-         long long* pmini) {
+         long long* pmini, int rev) {
   __shared__ double sh[kWavesPerBlock * PA_COUNT];
   if (ds->halt) return;
   using Op = PassAOp<T, FUSED>;
@@ -141,27 +141,32 @@ k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
     // FP32: two consecutive runs of 256 elements per block step, software
     // pipelined: the loads of one run are in flight while the other run is
     // processed
-    const long long step = (long long)gridDim.x * (2 * kBlock);
-    long long i0 = blockIdx.x * (long long)(2 * kBlock);
+    const long long nc = (n + 2 * kBlock - 1) / (2 * kBlock);
+    long long c = blockIdx.x;
     typename Op::In ea, eb;
-    if (i0 < n) {
-      const long long ia = i0 + threadIdx.x;
+    if (c < nc) {
+      const long long ia = chunk_start(c, nc, 2 * kBlock, rev) + threadIdx.x;
       op.load(ea, ia < n ? ia : n - 1);
     }
-    for (; i0 < n; i0 += step) {
-      const long long ia = i0 + threadIdx.x, ib = ia + kBlock, na = ia + step;
+    for (; c < nc; c += gridDim.x) {
+      const long long i0 = chunk_start(c, nc, 2 * kBlock, rev);
+      const long long ia = i0 + threadIdx.x, ib = ia + kBlock;
+      const long long cn = c + gridDim.x;
+      const long long na = (cn < nc ? chunk_start(cn, nc, 2 * kBlock, rev) : i0) + threadIdx.x;
       op.load(eb, ib < n ? ib : n - 1);
       op.keep(ea);
       if (ia < n) op.element(ia, ea);
-      // unconditional (clamped past the end, unused there): a branch around
-      // it would make the wait for eb's loads cover these too
+      // unconditional (the next chunk, or this one again after the last):
+      // a branch around it would make the wait for eb's loads cover these too
       op.load(ea, na < n ? na : n - 1);
       op.keep(eb);
       if (ib < n) op.element(ib, eb);
     }
   } else {
-    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
-         i += (long long)gridDim.x * kBlock) {
+    const long long nc = (n + kBlock - 1) / kBlock;
+    for (long long c = blockIdx.x; c < nc; c += gridDim.x) {
+      const long long i = chunk_start(c, nc, kBlock, rev) + threadIdx.x;
+      if (i >= n) continue;
       typename Op::In e;
       op.load(e, i);
       op.keep(e);
@@ -199,7 +204,8 @@ k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
-     const T* __restrict__ u, uint8_t* __restrict__ iwhere, const T* __restrict__ W, DevScalars* ds, double* part) {
+     const T* __restrict__ u, uint8_t* __restrict__ iwhere, const T* __restrict__ W, DevScalars* ds,
+     double* part, int rev) {
   __shared__ double sh[kWavesPerBlock * P4_CORR];
   __shared__ double s_corr[kWavesPerBlock][136];
   if (ds->halt) return;
@@ -210,15 +216,18 @@ k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
     // software pipelined: one run's loads are in flight while the other run
     // is processed (the register footprint leaves two waves per SIMD, so a
     // wave cannot rely on others to cover its arithmetic)
-    const long long step = (long long)gridDim.x * (2 * kBlock);
-    long long i0 = blockIdx.x * (long long)(2 * kBlock);
+    const long long nc = (n + 2 * kBlock - 1) / (2 * kBlock);
+    long long c = blockIdx.x;
     typename P4Op<T>::In ea, eb;
-    if (i0 < n) {
-      const long long ia = i0 + threadIdx.x;
+    if (c < nc) {
+      const long long ia = chunk_start(c, nc, 2 * kBlock, rev) + threadIdx.x;
       op.load(ea, ia < n ? ia : n - 1);
     }
-    for (; i0 < n; i0 += step) {
-      const long long ia = i0 + threadIdx.x, ib = ia + kBlock, na = ia + step;
+    for (; c < nc; c += gridDim.x) {
+      const long long i0 = chunk_start(c, nc, 2 * kBlock, rev);
+      const long long ia = i0 + threadIdx.x, ib = ia + kBlock;
+      const long long cn = c + gridDim.x;
+      const long long na = (cn < nc ? chunk_start(cn, nc, 2 * kBlock, rev) : i0) + threadIdx.x;
       op.load(eb, ib < n ? ib : n - 1);
       op.keep(ea);
       op.corr_step(ia, ia < n ? op.element(ia, ea) : 0);
@@ -228,9 +237,9 @@ k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
     }
   } else {
     // FP64: one element per step (two would not fit two waves per SIMD)
-    for (long long i0 = blockIdx.x * (long long)kBlock; i0 < n;
-         i0 += (long long)gridDim.x * kBlock) {
-      const long long ia = i0 + threadIdx.x;
+    const long long nc = (n + kBlock - 1) / kBlock;
+    for (long long c = blockIdx.x; c < nc; c += gridDim.x) {
+      const long long ia = chunk_start(c, nc, kBlock, rev) + threadIdx.x;
       typename P4Op<T>::In ea;
       op.load(ea, ia < n ? ia : n - 1);
       op.keep(ea);
@@ -279,12 +288,14 @@ __global__ void __launch_bounds__(kBlock)
 k_p5a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __restrict__ u,
       const uint8_t* __restrict__ iwhere,
       const T* __restrict__ W, T* __restrict__ dsub, DevScalars* ds, double* pmin,
-      long long* pmini) {
+      long long* pmini, int rev) {
   if (ds->halt || ds->skip_subsm) return;
   P5aOp<T> op;
   op.init(ds, X, Gv, l, u, iwhere, W, dsub);
-  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
-       i += (long long)gridDim.x * kBlock) {
+  const long long nc = (n + kBlock - 1) / kBlock;
+  for (long long c = blockIdx.x; c < nc; c += gridDim.x) {
+    const long long i = chunk_start(c, nc, kBlock, rev) + threadIdx.x;
+    if (i >= n) continue;
     typename P5aOp<T>::In e;
     op.load(e, i);
     op.keep(e);
@@ -320,13 +331,15 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_p5b(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
       const T* __restrict__ u, const uint8_t* __restrict__ iwhere, const T* __restrict__ dsub, DevScalars* ds,
-      double* part, double* pmin) {
+      double* part, double* pmin, int rev) {
   __shared__ double sh[kWavesPerBlock * P5_COUNT];
   if (ds->halt) return;
   P5bOp<T> op;
   op.init(ds, X, Gv, l, u, iwhere, dsub);
-  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
-       i += (long long)gridDim.x * kBlock) {
+  const long long nc = (n + kBlock - 1) / kBlock;
+  for (long long c = blockIdx.x; c < nc; c += gridDim.x) {
+    const long long i = chunk_start(c, nc, kBlock, rev) + threadIdx.x;
+    if (i >= n) continue;
     typename P5bOp<T>::In e;
     op.load(e, i);
     op.keep(e);
@@ -398,16 +411,18 @@ k_subspace_point(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_gd(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ d, const DevScalars* ds,
-     double* part) {
+     double* part, int rev) {
   __shared__ double sh[kWavesPerBlock];
   const int cur = ds->phase & 1;
   const T* __restrict__ gb = Gv[cur ^ 1];
   LsDirection<T> dir;
   dir.init(ds, X, d);
   double acc[1] = {0.0};
-  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
-       i += (long long)gridDim.x * kBlock)
-    acc[0] += (double)gb[i] * (double)dir.at(i);
+  const long long nc = (n + kBlock - 1) / kBlock;
+  for (long long c = blockIdx.x; c < nc; c += gridDim.x) {
+    const long long i = chunk_start(c, nc, kBlock, rev) + threadIdx.x;
+    if (i < n) acc[0] += (double)gb[i] * (double)dir.at(i);
+  }
   block_sum_store<1>(acc, sh, part + blockIdx.x, gridDim.x);
 }
 
@@ -433,26 +448,26 @@ void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g,
 template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X,
                    Pair<T> Gv, const T* l, const T* u, uint8_t* iwhere, T* W, const T* d, T* tbrk, DevScalars* ds,
-                   double* part, double* pmax, double* pmin, long long* pmini) {
+                   double* part, double* pmax, double* pmin, long long* pmini, int rev) {
   if (fused)
     hipLaunchKernelGGL((k_pass_a<T, true>), dim3(G), dim3(kBlock), 0, s, n, X, Gv,
-                       l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini);
+                       l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini, rev);
   else
     hipLaunchKernelGGL((k_pass_a<T, false>), dim3(G), dim3(kBlock), 0, s, n, X,
-                       Gv, l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini);
+                       Gv, l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini, rev);
 }
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
                const T* l, const T* u, uint8_t* iwhere,
-               const T* W, DevScalars* ds, double* part) {
-  hipLaunchKernelGGL(k_p4<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere, W, ds, part);
+               const T* W, DevScalars* ds, double* part, int rev) {
+  hipLaunchKernelGGL(k_p4<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere, W, ds, part, rev);
 }
 template <typename T>
 void launch_p5a(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                 const T* u, const uint8_t* iwhere, const T* W, T* dsub,
-                DevScalars* ds, double* pmin, long long* pmini) {
+                DevScalars* ds, double* pmin, long long* pmini, int rev) {
   hipLaunchKernelGGL(k_p5a<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere,
-                     W, dsub, ds, pmin, pmini);
+                     W, dsub, ds, pmin, pmini, rev);
 }
 template <typename T>
 void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
@@ -464,8 +479,8 @@ void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> G
 template <typename T>
 void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
                 const T* l, const T* u, const uint8_t* iwhere,
-                const T* dsub, DevScalars* ds, double* part, double* pmin) {
-  hipLaunchKernelGGL(k_p5b<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere, dsub, ds, part, pmin);
+                const T* dsub, DevScalars* ds, double* part, double* pmin, int rev) {
+  hipLaunchKernelGGL(k_p5b<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere, dsub, ds, part, pmin, rev);
 }
 template <typename T>
 void launch_trial(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
@@ -482,8 +497,8 @@ void launch_subspace_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T>
 }
 template <typename T>
 void launch_gd(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* d,
-               const DevScalars* ds, double* part) {
-  hipLaunchKernelGGL(k_gd<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, d, ds, part);
+               const DevScalars* ds, double* part, int rev) {
+  hipLaunchKernelGGL(k_gd<T>, dim3(G), dim3(kBlock), 0, s, n, X, Gv, d, ds, part, rev);
 }
 // d = z - x of every variable during the line search into out (inspection).
 template <typename T>
@@ -512,19 +527,19 @@ void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
   template void launch_pass_a<T>(hipStream_t, int, bool, long long, Pair<T>,      \
                                  Pair<T>, const T*, const T*,         \
                                  uint8_t*, T*, const T*, T*, DevScalars*, double*,  \
-                                 double*, double*, long long*);                     \
+                                 double*, double*, long long*, int);                     \
   template void launch_p4<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,     \
                              const T*, const T*, uint8_t*, const T*,    \
-                             DevScalars*, double*);                                 \
+                             DevScalars*, double*, int);                                 \
   template void launch_p5a<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,      \
                               const T*, const T*, const uint8_t*,       \
-                              const T*, T*, DevScalars*, double*, long long*);      \
+                              const T*, T*, DevScalars*, double*, long long*, int);      \
   template void launch_cauchy_point<T>(hipStream_t, int, long long, Pair<T>,      \
                                        Pair<T>, const T*, const T*,    \
                                        const uint8_t*, const DevScalars*, T*);      \
   template void launch_p5b<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
                               const T*, const T*, const uint8_t*,       \
-                              const T*, DevScalars*, double*, double*);             \
+                              const T*, DevScalars*, double*, double*, int);             \
   template void launch_trial<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,    \
                                 const T*, const T*, const uint8_t*,     \
                                 const T*, T*, const DevScalars*);                   \
@@ -533,7 +548,7 @@ void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
                                          const uint8_t*, const T*,                  \
                                          const DevScalars*, T*);                    \
   template void launch_gd<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,      \
-                             const T*, const DevScalars*, double*);                 \
+                             const T*, const DevScalars*, double*, int);                 \
   template void launch_direction<T>(hipStream_t, int, long long, Pair<T>, const T*, \
                                     const DevScalars*, T*);
 

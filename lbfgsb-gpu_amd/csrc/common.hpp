@@ -236,6 +236,16 @@ __device__ inline void load_ring(const T* __restrict__ W, long long i, const Rin
   mask_ring(rr, yv, sv);
 }
 
+// Start of chunk c (of nchunks, C elements each) of a grid-stride pass, the
+// chunks taken in ascending or (rev) descending order; the elements inside a
+// chunk stay ascending, so the accesses stay coalesced.  Consecutive passes
+// alternate directions: each starts where the previous one ended, on data
+// that is still in the device's last-level cache.
+__device__ __forceinline__ long long chunk_start(long long c, long long nchunks, int C,
+                                                 bool rev) {
+  return (rev ? nchunks - 1 - c : c) * (long long)C;
+}
+
 template <typename T>
 __device__ inline T ld_nt(const T* p) {
   return __builtin_nontemporal_load(p);

@@ -288,7 +288,7 @@ struct Engine final : EngineBase {
         break;
       case K_PASSA_SA:
         launch_pass_a<T>(st, G, false, n, dX, dG, l, u, iwhere, W, d, tbrk, ds, part, pmax,
-                         pmin, pmini);
+                         pmin, pmini, 0);
         break;
       case K_UPD_REDUCE:
         launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0);
@@ -296,7 +296,7 @@ struct Engine final : EngineBase {
         break;
       case K_PASSA_FUSED:
         launch_pass_a<T>(st, G, true, n, dX, dG, l, u, iwhere, W, d, tbrk, ds, part, pmax,
-                         pmin, pmini);
+                         pmin, pmini, 0);
         break;
       case K_GCP:
         launch_gcp_select<T>(st, G, n, tbrk, ds, ca, n);
@@ -310,20 +310,20 @@ struct Engine final : EngineBase {
         launch_update_gcp(st, ds, part, pmin, pmini, G, 0);
         break;
       case K_P4:
-        launch_p4<T>(st, G, n, dX, dG, l, u, iwhere, W, ds, part);
+        launch_p4<T>(st, G, n, dX, dG, l, u, iwhere, W, ds, part, 1);
         break;
       case K_FORMK:
         launch_rowsum(st, part, G, 0, P4_COUNT, ds, 1);
         launch_formk_subsm(st, ds, part, G);
         break;
       case K_P5A:
-        launch_p5a<T>(st, G, n, dX, dG, l, u, iwhere, W, dsub, ds, pmin, pmini);
+        launch_p5a<T>(st, G, n, dX, dG, l, u, iwhere, W, dsub, ds, pmin, pmini, 0);
         break;
       case K_ALPHA:
         launch_alpha(st, ds, pmin, pmini, G);
         break;
       case K_P5B:
-        launch_p5b<T>(st, G, n, dX, dG, l, u, iwhere, dsub, ds, part, pmin);
+        launch_p5b<T>(st, G, n, dX, dG, l, u, iwhere, dsub, ds, part, pmin, 1);
         break;
       case K_LSINIT:
         launch_ls_init(st, ds, part, pmin, G);
@@ -332,7 +332,7 @@ struct Engine final : EngineBase {
         launch_trial<T>(st, G, n, dX, dG, l, u, iwhere, dsub, d, ds);
         break;
       case K_GD:
-        launch_gd<T>(st, G, n, dX, dG, d, ds, part);
+        launch_gd<T>(st, G, n, dX, dG, d, ds, part, 1);
         break;
       case K_LSSTEP:
         launch_ls_step(st, ds, part, G, f);

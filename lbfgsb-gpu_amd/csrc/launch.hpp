@@ -18,7 +18,8 @@ struct Cand {
   int pad;
 };
 
-// bulk.hip
+// bulk.hip (rev: the pass takes its chunks in descending order; see
+// chunk_start in common.hpp)
 template <typename T>
 void launch_errclb(hipStream_t s, int G, long long n, const int* nbd, const T* l, const T* u,
                    double* part);
@@ -34,15 +35,15 @@ template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X, Pair<T> Gv,
                    const T* l, const T* u, uint8_t* iwhere, T* W, const T* d,
                    T* tbrk, DevScalars* ds, double* part, double* pmax, double* pmin,
-                   long long* pmini);
+                   long long* pmini, int rev);
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                const T* u, uint8_t* iwhere, const T* W, DevScalars* ds,
-               double* part);
+               double* part, int rev);
 template <typename T>
 void launch_p5a(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                 const T* u, const uint8_t* iwhere, const T* W, T* dsub,
-                DevScalars* ds, double* pmin, long long* pmini);
+                DevScalars* ds, double* pmin, long long* pmini, int rev);
 // x_cp of every variable from the current state (inspection only).
 template <typename T>
 void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
@@ -51,7 +52,7 @@ void launch_cauchy_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> G
 template <typename T>
 void launch_p5b(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                 const T* u, const uint8_t* iwhere, const T* dsub,
-                DevScalars* ds, double* part, double* pmin);
+                DevScalars* ds, double* part, double* pmin, int rev);
 template <typename T>
 void launch_trial(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                   const T* u, const uint8_t* iwhere, const T* dsub, T* d,
@@ -64,7 +65,7 @@ void launch_subspace_point(hipStream_t s, int G, long long n, Pair<T> X, Pair<T>
                            const T* dsub, const DevScalars* ds, T* out);
 template <typename T>
 void launch_gd(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* d,
-               const DevScalars* ds, double* part);
+               const DevScalars* ds, double* part, int rev);
 // The line-search direction d of every variable into out (inspection only).
 template <typename T>
 void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
