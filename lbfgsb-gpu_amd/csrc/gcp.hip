@@ -299,6 +299,7 @@ __global__ void __launch_bounds__(kBlock)
 s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__ W,
                   DevScalars* ds) {
   __shared__ RoundLds L;
+  __shared__ DevScalars sds;
   if (!ds->gcp_pending || ds->halt) return;
   const long long K = (long long)ds->cand_count;
   if (K > kSmallRound) {
@@ -308,9 +309,13 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
     }
     return;
   }
-  stage_gcp_const(ds, L.gc);
+  // the round's scalar logic (stop tests, conclusion) on an LDS copy
+  ds_load(&sds, ds);
+  DevScalars* s = &sds;
+  stage_gcp_const(s, L.gc);
   block_sort_small(cand, K, L.sc);
-  block_round_scan(ds, L, L.sc, K, (const T*)Gv[ds->phase & 1], W);
+  block_round_scan(s, L, L.sc, K, (const T*)Gv[s->phase & 1], W);
+  ds_store(ds, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -249,9 +249,45 @@ __device__ inline void block_bitonic(Cand* sc, int P) {
   }
 }
 
+// Sort K <= 64 candidates with one wave: lane-to-lane bitonic network in
+// registers, then into LDS.  The other waves only take part in the barrier.
+__device__ inline void wave_sort_small(const Cand* cand, long long K, Cand* sc) {
+  if (threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    Cand c;
+    if (lane < K) {
+      c = cand[lane];
+    } else {
+      c.t = __builtin_inf();
+      c.idx = 0x7fffffff;
+      c.pad = 0;
+    }
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        Cand o;
+        o.t = __shfl_xor(c.t, j, kWave);
+        o.idx = __shfl_xor(c.idx, j, kWave);
+        o.pad = 0;
+        // pair (i, i + j): ascending blocks keep the smaller at i
+        const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+        const bool take = keep_min ? cand_less(o, c) : cand_less(c, o);
+        if (take) c = o;
+      }
+    }
+    sc[lane] = c;
+  }
+  __syncthreads();
+}
+
 // Load K <= kSmallRound candidates into LDS, padded to a power of two, and
 // sort them.
 __device__ inline void block_sort_small(const Cand* cand, long long K, Cand* sc) {
+  if (K <= kWave) {
+    wave_sort_small(cand, K, sc);
+    return;
+  }
   int P = 64;
   while (P < K) P <<= 1;
   for (int j = threadIdx.x; j < P; j += blockDim.x) {

@@ -17,21 +17,6 @@ namespace lbfgsb {
 
 #define A1(a, ld, i, j) (a)[((i)-1) * (ld) + ((j)-1)]
 
-static_assert(sizeof(DevScalars) % 8 == 0, "DevScalars is copied as 8-byte words");
-constexpr int kDsWords = sizeof(DevScalars) / 8;
-
-__device__ void ds_load(DevScalars* sds, const DevScalars* ds) {
-  const unsigned long long* src = (const unsigned long long*)ds;
-  unsigned long long* dst = (unsigned long long*)sds;
-  for (int i = threadIdx.x; i < kDsWords; i += blockDim.x) dst[i] = src[i];
-  __syncthreads();
-}
-__device__ void ds_store(DevScalars* ds, const DevScalars* sds) {
-  __syncthreads();
-  const unsigned long long* src = (const unsigned long long*)sds;
-  unsigned long long* dst = (unsigned long long*)ds;
-  for (int i = threadIdx.x; i < kDsWords; i += blockDim.x) dst[i] = src[i];
-}
 
 // The bulk passes leave at most kMaxBlocks partials per value.  The
 // finishing reductions load every partial of a thread before the first

@@ -186,4 +186,22 @@ struct DevScalars {
   double red[P4_COUNT + 1];
 };
 
+// The device state staged into LDS by a single-workgroup kernel (all threads
+// call; both end with the workgroup synchronised on the copy).
+static_assert(sizeof(DevScalars) % 8 == 0, "DevScalars is copied as 8-byte words");
+constexpr int kDsWords = sizeof(DevScalars) / 8;
+
+__device__ inline void ds_load(DevScalars* sds, const DevScalars* ds) {
+  const unsigned long long* src = (const unsigned long long*)ds;
+  unsigned long long* dst = (unsigned long long*)sds;
+  for (int i = threadIdx.x; i < kDsWords; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+}
+__device__ inline void ds_store(DevScalars* ds, const DevScalars* sds) {
+  __syncthreads();
+  const unsigned long long* src = (const unsigned long long*)sds;
+  unsigned long long* dst = (unsigned long long*)ds;
+  for (int i = threadIdx.x; i < kDsWords; i += blockDim.x) dst[i] = src[i];
+}
+
 }  // namespace lbfgsb
