@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "objectives.hpp"
 
 namespace lbfgsb {
@@ -47,10 +49,16 @@ __device__ inline double obj_term(const ObjDesc& o, long long n, const T* __rest
     return fk;
   }
   if (o.kind == OBJK_ROSEN) {
-    const double a = (double)x[2 * k], bb = (double)x[2 * k + 1];
+    // the pair (x_{2k}, x_{2k+1}) as one 2-vector load and store: a wave
+    // moves one contiguous run instead of two strided ones
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    const V2 xv = reinterpret_cast<const V2*>(x)[k];
+    const double a = (double)xv.x, bb = (double)xv.y;
     const double t1 = bb - a * a, t2 = 1.0 - a;
-    g[2 * k] = (T)(-400.0 * t1 * a - 2.0 * t2);
-    g[2 * k + 1] = (T)(200.0 * t1);
+    V2 gv;
+    gv.x = (T)(-400.0 * t1 * a - 2.0 * t2);
+    gv.y = (T)(200.0 * t1);
+    reinterpret_cast<V2*>(g)[k] = gv;
     return 100.0 * t1 * t1 + t2 * t2;
   }
   const T* a0 = (const T*)o.a0;
