@@ -21,6 +21,8 @@
 // function of the problem and S only.  The breakpoint rounds of the Cauchy
 // search are sorted and scanned by the gang together (gang_round).  Only
 // member 0 writes the problem's results and control state back.
+// the compact bmv: the kernel's registers are shared with its bulk passes
+#define LBFGSB_COMPACT_BMV 1
 #include "batch.hpp"
 #include "common.hpp"
 #include "control.hpp"

@@ -125,14 +125,67 @@ __device__ __forceinline__ bool d_dpofa_r(double (&a)[NMAX][NMAX], int n) {
 }
 
 // p = M v with the 2col x 2col middle matrix through sy and the Cholesky
-// factor of T in wt (lbfgsbcpu.cpp:517-565).  sy, wt have ld 8.  Every entry
+// factor of T in wt (lbfgsbcpu.cpp:517-565), compact form: sy, wt (ld 8)
+// read where used, branches on col.  The two halves of v and p (Y part,
+// S part) are kept in registers.
+__device__ inline int d_bmv_compact(const double* __restrict__ sy, const double* __restrict__ wt,
+                            int col, const double* v, double* p) {
+  if (col == 0) return 0;
+  double vy[8], vs[8], py[8], ps[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    vy[q] = q < col ? v[q] : 0.0;
+    vs[q] = q < col ? v[col + q] : 0.0;
+    py[q] = 0.0;
+    ps[q] = 0.0;
+  }
+  ps[0] = vs[0];
+#pragma unroll
+  for (int i = 2; i <= 8; i++) {
+    if (i > col) continue;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 1; k < 8; k++)
+      if (k <= i - 1) s = s + A1(sy, 8, i, k) * vy[k - 1] / A1(sy, 8, k, k);
+    ps[i - 1] = vs[i - 1] + s;
+  }
+  int info = d_dtrsl_r<8>(wt, 8, col, ps, 11);
+  if (info != 0) return info;
+#pragma unroll
+  for (int i = 1; i <= 8; i++)
+    if (i <= col) py[i - 1] = vy[i - 1] / sqrt(A1(sy, 8, i, i));
+  info = d_dtrsl_r<8>(wt, 8, col, ps, 1);
+  if (info != 0) return info;
+#pragma unroll
+  for (int i = 1; i <= 8; i++)
+    if (i <= col) py[i - 1] = -py[i - 1] / sqrt(A1(sy, 8, i, i));
+#pragma unroll
+  for (int i = 1; i <= 8; i++) {
+    if (i > col) continue;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 2; k <= 8; k++)
+      if (k >= i + 1 && k <= col) s = s + A1(sy, 8, k, i) * ps[k - 1] / A1(sy, 8, i, i);
+    py[i - 1] = py[i - 1] + s;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    if (q < col) {
+      p[q] = py[q];
+      p[col + q] = ps[q];
+    }
+  }
+  return 0;
+}
+
+// The same product, hoisted form: every entry
 // of sy's lower and wt's upper triangle is loaded before the first
 // operation, and the loops run over the fixed size 8 with the entries beyond
 // col masked out by selects, not branches: the loads go out together and the
 // independent divisions overlap.  Each used entry sees the reference's
 // operations in the reference's order; entries beyond col are never stored.
-__device__ inline int d_bmv(const double* __restrict__ sy, const double* __restrict__ wt,
-                            int col, const double* v, double* p) {
+__device__ inline int d_bmv_hoisted(const double* __restrict__ sy, const double* __restrict__ wt,
+                                    int col, const double* v, double* p) {
   if (col == 0) return 0;
   double S[8][8], Tu[8][8];  // S[i][k], k <= i; Tu[k][j], k <= j (0-based)
 #pragma unroll
@@ -222,6 +275,21 @@ __device__ inline int d_bmv(const double* __restrict__ sy, const double* __restr
     }
   }
   return 0;
+}
+
+// The single-workgroup control kernels take the hoisted form (its loads
+// overlap; it needs ~200 registers).  The batched kernel, whose register
+// allocation its bulk passes share, is built with LBFGSB_COMPACT_BMV set and
+// takes the compact one.  Both give the same results bit for bit.
+#ifndef LBFGSB_COMPACT_BMV
+#define LBFGSB_COMPACT_BMV 0
+#endif
+__device__ inline int d_bmv(const double* __restrict__ sy, const double* __restrict__ wt,
+                            int col, const double* v, double* p) {
+  if constexpr (LBFGSB_COMPACT_BMV)
+    return d_bmv_compact(sy, wt, col, v, p);
+  else
+    return d_bmv_hoisted(sy, wt, col, v, p);
 }
 
 __device__ inline double dmax_(double a, double b) { return a > b ? a : b; }
