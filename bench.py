@@ -31,17 +31,22 @@ HBM_PEAK_GBS = 8000.0
 # step ids of lbfgsb_engine_run / lbfgsb_engine_step_time
 STEPS = {"pass_a_fused": 4, "p4": 7, "formk_subsm": 8, "p5a": 9, "alpha": 10, "p5b": 11,
          "ls_init": 12, "trial": 13, "gd": 14, "ls_step": 15, "gcp_rounds_host": 16,
-         "gcp_round": 17, "stop_tests": 5, "update_gcp": 6}
+         "gcp_round": 17, "stop_tests": 5, "update_gcp": 6, "pass_a_spec": 18,
+         "ls_stop": 19}
 
 
 def kernel_bytes_per_elem(s: int) -> dict:
     """Algorithmic HBM bytes per element of each bulk pass at steady state
     (col = m = 8, one memory update per iteration; the class byte iwhere also
     carries nbd, so a pass reads one byte per variable for both).
-      pass_a_fused: read x, x_old, g, g_old, l, u (6s), iwhere, the 14
+      pass_a_spec:  read x, x_old, g, g_old, l, u (6s), iwhere, the 14
                     retained W entries (14s); write the new S/Y pair (2s) and
                     the breakpoint t (s) the Cauchy search selects from (the
-                    step d of a unit step is x - x_old)
+                    step d of a unit step is x - x_old).  It runs on the
+                    evaluated trial point ahead of the line-search step and
+                    also sums the g'd that step needs
+      pass_a_fused: the same pass after an acceptance it could not run ahead
+                    of (no memory update, a step without evaluation)
       p4:           read x, g, l, u (4s), iwhere, W (16s); write iwhere
       p5a:          read x, g, l, u (4s), iwhere, W (16s); write d_sub (s)
       p5b:          read x, g, d_sub, l, u (5s), iwhere; write z into the
@@ -49,25 +54,32 @@ def kernel_bytes_per_elem(s: int) -> dict:
       trial:        nothing: the first trial is a unit step, whose trial point
                     z is already in place (the first non-unit step moves 4s
                     and stores d)
-      gd:           read g_trial, x_trial, x (3s): d = z - x
+      gd:           read g_trial, x_trial, x (3s): d = z - x; only for
+                    trial points after the first of an iteration, or in the
+                    regular (non-speculative) program
     x_cp and r are recomputed per element by the passes that need them (from
     x, g, l, u, the class and W) instead of being stored and read back."""
-    return {"pass_a_fused": 23 * s + 1, "p4": 20 * s + 2, "p5a": 21 * s + 1,
-            "p5b": 6 * s + 1, "trial": 0, "gd": 3 * s}
+    return {"pass_a_spec": 23 * s + 1, "pass_a_fused": 23 * s + 1, "p4": 20 * s + 2,
+            "p5a": 21 * s + 1, "p5b": 6 * s + 1, "trial": 0, "gd": 3 * s}
+
+
+# the bulk passes of one steady-state iteration with one (unit-step) trial point
+STEADY_STATE = ("pass_a_spec", "p4", "p5a", "p5b", "trial")
 
 
 # HBM bytes per launch at steady state for the default workload (n = 16,777,216,
 # fp32), from two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE, KiB per
 # dispatch) scaled by what the same counters read for a 64 MiB copy kernel
 # (fetch x1.999, write x1.000); see profiles/plain_rosenbrock_16M_fp32_hbm_traffic.md.
-MEASURED_TRAFFIC_16M_FP32 = {"pass_a_fused": 1629668715, "p4": 1457059587, "p5a": 1493465866,
-                             "p5b": 486596373, "gd": 201308465}
+MEASURED_TRAFFIC_16M_FP32 = {"pass_a_spec": 1629668715, "p4": 1457059587,
+                             "p5a": 1493465866, "p5b": 486596373}
 
 
 def b_alg_per_iter(n: int, s: int) -> float:
     """Algorithmic bytes of one steady-state iteration with one (unit-step)
-    trial point: the sum of kernel_bytes_per_elem, n (73 s + 5)."""
-    return float(n) * sum(kernel_bytes_per_elem(s).values())
+    trial point: the passes of STEADY_STATE, n (70 s + 5)."""
+    kb = kernel_bytes_per_elem(s)
+    return float(n) * sum(kb[k] for k in STEADY_STATE)
 
 
 ALL_STEPS = 0xFFFFFFFF
@@ -358,7 +370,7 @@ def main():
     }
     if not a.no_fp64:
         p64 = run_gpu(n, True, a.warmup, prof_steps, ALL_STEPS)
-        r64 = run_gpu(n, True, a.warmup, a.steps, 1 << STEPS["pass_a_fused"])
+        r64 = run_gpu(n, True, a.warmup, a.steps, 1 << STEPS["pass_a_spec"])
         t64 = r64["timed"] / a.steps
         out["fp64"] = {"value": a.steps / r64["timed"], "unit": "iters/s",
                        "ms_per_step": t64 * 1e3,

@@ -105,7 +105,11 @@ int lbfgsb_engine_setup_finish(lbfgsb_engine* e, double f);
 /* Enqueue program `program` from step `start_step` (-1 = first) up to, not
  * including, `stop_step` (16 = to the end), then read the status.
  * Programs: 0 iteration start, 1 after an evaluation (f = its value),
- * 2 after an accepted step, 3 after the stopping check, 4 Cauchy-point tail. */
+ * 2 after an accepted step, 3 after the stopping check, 4 Cauchy-point tail.
+ * Statuses: 0 running, 1 evaluate the trial point (x/g which = 1), 2 step
+ * accepted, 3 stopping tests passed, 4 breakpoint rounds, 5 restart (run
+ * program 0), 6 done, 7 step accepted but the tail pass the whole program 1
+ * ran ahead of the line search does not apply (run program 2). */
 int lbfgsb_engine_run(lbfgsb_engine* e, int program, int start_step, int stop_step, double f);
 /* Device pointers of the accepted (which = 0) or pending trial (1) x / g. */
 void* lbfgsb_engine_x(lbfgsb_engine* e, int which);

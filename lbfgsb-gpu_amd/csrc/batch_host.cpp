@@ -3,7 +3,7 @@
 //
 // All per-problem arrays live in slices of arena-wide arrays, allocated once
 // (first fit over 64-element-aligned ranges, so a problem's W block starts at
-// 16 x its offset).  A slot holds one problem from admission to removal;
+// kWCols x its offset).  A slot holds one problem from admission to removal;
 // `run` launches the kernel over every unfinished slot, largest first, either
 // to completion or for an iteration budget (the serving loop resumes the
 // suspended problems in its next launch).  With a segment length set, a
@@ -115,7 +115,7 @@ struct Arena final : BatchBase {
     T** vec[] = {&x, &xb, &g, &gb, &l, &u, &d, &dsub, &ob, &oc, &oa0, &oa1, &oa2};
     for (T** p : vec)
       if (dm(*p, c)) return -1;
-    if (dm(W, c * 16) || dm(nbd, c) || dm(iwhere, c) || dm(ca, c) || dm(cb, c)) return -1;
+    if (dm(W, c * kWCols) || dm(nbd, c) || dm(iwhere, c) || dm(ca, c) || dm(cb, c)) return -1;
     // queue entries: one per problem plus one per extra gang member (each
     // member takes at least kMinSegment elements)
     max_items = (long long)maxp + cap / kMinSegment + 1;
@@ -123,7 +123,7 @@ struct Arena final : BatchBase {
         dm(count, 2) || dm(gang, maxp) || dm(gpart, (size_t)max_items * 2 * kGangPart) ||
         dm(next, 2))
       return -1;
-    BCHK(hipMemsetAsync(W, 0, c * 16 * sizeof(T), st));
+    BCHK(hipMemsetAsync(W, 0, c * kWCols * sizeof(T), st));
     BCHK(hipStreamSynchronize(st));
     return 0;
   }
@@ -212,11 +212,11 @@ struct Arena final : BatchBase {
     const long long off = sl.off, n = sl.n;
     // the memory slots of a reused range may hold another problem's pairs:
     // they are multiplied by zero coefficients only, but keep them finite
-    BCHK(hipMemsetAsync(W + off * 16, 0, (size_t)sl.npad * 16 * sizeof(T), st));
+    BCHK(hipMemsetAsync(W + off * kWCols, 0, (size_t)sl.npad * kWCols * sizeof(T), st));
     BatchProb& p = hprobs[slot];
     p.off = off;
     p.n = n;
-    p.woff = off * 16;
+    p.woff = off * kWCols;
     p.obj.kind = kind;
     p.obj.nx = nx;
     p.obj.ny = ny;
@@ -455,7 +455,7 @@ long long arena_bytes(int dtype, long long capacity, int max_problems) {
   const long long s = dtype == 1 ? 8 : 4;
   const long long cap = ((capacity + 63) / 64) * 64;
   const long long items = (long long)max_problems + cap / kMinSegment + 1;
-  long long b = cap * (13 * s + 16 * s + (long long)sizeof(int) + 1 + 2 * (long long)sizeof(Cand));
+  long long b = cap * (13 * s + kWCols * s + (long long)sizeof(int) + 1 + 2 * (long long)sizeof(Cand));
   b += (long long)max_problems *
        (long long)(sizeof(DevScalars) + sizeof(BatchResult) + sizeof(BatchProb) + sizeof(GangCtl));
   b += items * ((long long)sizeof(BatchItem) + 2LL * kGangPart * (long long)sizeof(double));

@@ -126,17 +126,22 @@ k_projgr(long long n, const T* __restrict__ x, const T* __restrict__ g,
 //          gold = G[1-cur]; when ds->updatd the new slot s = stp*d,
 //          y = g - gold is written at ds->itail and the Gram row
 //          s'y_j, s_j's over the retained slots is formed.
+//   spec (FUSED): the same on the evaluated trial point before the line
+//          search has accepted it, on the ring an accepted step with an
+//          update leaves, with the new pair in the spare physical slot and
+//          g'd summed as well; it runs whatever ds->halt says (the chain is
+//          halted after an evaluation request).
 template <typename T, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
 k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
          const T* __restrict__ u, uint8_t* __restrict__ iwhere, T* __restrict__ W, const T* __restrict__ d,
          T* __restrict__ tbrk, DevScalars* ds, double* part, double* pmax, double* pmin,
-         long long* pmini, int rev) {
+         long long* pmini, int rev, int spec) {
   __shared__ double sh[kWavesPerBlock * PA_COUNT];
-  if (ds->halt) return;
+  if (ds->halt && !spec) return;
   using Op = PassAOp<T, FUSED>;
   Op op;
-  op.init(ds, X, Gv, l, u, iwhere, W, d, tbrk);
+  op.init(ds, X, Gv, l, u, iwhere, W, d, tbrk, spec != 0);
   if constexpr (sizeof(T) == 4) {
     // FP32: two consecutive runs of 256 elements per block step, software
     // pipelined: the loads of one run are in flight while the other run is
@@ -448,13 +453,14 @@ void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g,
 template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X,
                    Pair<T> Gv, const T* l, const T* u, uint8_t* iwhere, T* W, const T* d, T* tbrk, DevScalars* ds,
-                   double* part, double* pmax, double* pmin, long long* pmini, int rev) {
+                   double* part, double* pmax, double* pmin, long long* pmini, int rev,
+                   int spec) {
   if (fused)
     hipLaunchKernelGGL((k_pass_a<T, true>), dim3(G), dim3(kBlock), 0, s, n, X, Gv,
-                       l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini, rev);
+                       l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini, rev, spec);
   else
     hipLaunchKernelGGL((k_pass_a<T, false>), dim3(G), dim3(kBlock), 0, s, n, X,
-                       Gv, l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini, rev);
+                       Gv, l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini, rev, 0);
 }
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
@@ -527,7 +533,7 @@ void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
   template void launch_pass_a<T>(hipStream_t, int, bool, long long, Pair<T>,      \
                                  Pair<T>, const T*, const T*,         \
                                  uint8_t*, T*, const T*, T*, DevScalars*, double*,  \
-                                 double*, double*, long long*, int);                     \
+                                 double*, double*, long long*, int, int);                \
   template void launch_p4<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,     \
                              const T*, const T*, uint8_t*, const T*,    \
                              DevScalars*, double*, int);                                 \

@@ -154,6 +154,10 @@ int drive(EngineBase* e, const Cb* cb, const Opt* opt, Summary* summary, void* s
       prog = PROG_AFTER_CHECK;
     } else if (status == ST_RESTART) {
       prog = PROG_ITER_START;
+    } else if (status == ST_REDO) {
+      // the step was accepted other than pass A assumed (no memory update,
+      // or a step the line search took without evaluating it): pass A again
+      prog = PROG_AFTER_ACCEPT;
     } else if (status != ST_DONE) {
       status = -1;
     }

@@ -10,19 +10,25 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;           // threads per block for bulk passes
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxM = 8;              // L-BFGS memory (the reference fixes m <= 8)
-constexpr int kTile = 64;             // W is stored as [n/64][16][64] tiles
+constexpr int kTile = 64;             // W is stored as [n/64][kWCols][64] tiles
+// W holds kWPhys = kMaxM + 1 physical pair slots: the ring's kMaxM pairs
+// and a spare that a new pair is written into before its step is accepted.
+constexpr int kWPhys = kMaxM + 1;
+constexpr int kWCols = 2 * kWPhys;
 constexpr int kMaxBlocks = 2048;      // grid cap for the bulk passes
 
 // ---------------------------------------------------------------------------
-// W = [Y | S] in HBM: element i, slot j (0..7 = Y, 8..15 = S) lives at
-//   ((i / 64) * 16 + j) * 64 + (i % 64)
-// so one wave reading one slot of 64 consecutive elements touches one
-// contiguous 256 B (fp32) / 512 B (fp64) segment, and a new slot is written
-// with fully coalesced stores.
+// W = [Y | S] in HBM: element i, column j (j < kWPhys: Y of physical slot
+// j; j >= kWPhys: S of physical slot j - kWPhys) lives at
+//   ((i / 64) * kWCols + j) * 64 + (i % 64)
+// so one wave reading one column of 64 consecutive elements touches one
+// contiguous 256 B (fp32) / 512 B (fp64) segment, and a new pair is written
+// with fully coalesced stores.  Ring slot s of the memory is physical slot
+// s ^ pslot_x[s] (DevScalars; all zero = the identity).
 __host__ __device__ inline size_t widx(size_t i, int j) {
-  return ((i >> 6) * 16 + (size_t)j) * 64 + (i & 63);
+  return ((i >> 6) * kWCols + (size_t)j) * 64 + (i & 63);
 }
-__host__ __device__ inline size_t wsize(size_t n) { return ((n + 63) / 64) * 16 * 64; }
+__host__ __device__ inline size_t wsize(size_t n) { return ((n + 63) / 64) * kWCols * 64; }
 
 // The two buffers of a double-buffered n-vector (accepted point and trial);
 // kernels pick one with the device-side phase bit.  Passed by value, so the
@@ -177,16 +183,19 @@ __device__ inline void ring_slots(int head, int m, int (&slot)[kMaxM]) {
 // being overwritten, skip_slot) read the slot of the first used position
 // instead: an address the same wave is reading anyway, so the load hits in
 // cache and the loop needs no branch around it.
+// rs[j] and newest are physical slots (pslot_x maps ring slots, above).
 struct RingReads {
   int rs[kMaxM];
   bool use[kMaxM];
-  int newest;  // slot of position col - 1 (0 when the memory is empty)
+  int newest;  // physical slot of position col - 1 (0 when the memory is empty)
 };
-__device__ inline RingReads ring_reads(int head, int m, int col, int skip_slot) {
+__device__ inline RingReads ring_reads(int head, int m, int col, int skip_slot,
+                                       const int* pslot_x) {
   RingReads r;
   int slot[kMaxM];
   ring_slots(head, m, slot);
-  r.newest = col > 0 ? (head + col - 1) % m : 0;
+  const int nl = col > 0 ? (head + col - 1) % m : 0;
+  r.newest = nl ^ pslot_x[nl];
   int fb = -1;
 #pragma unroll
   for (int j = 0; j < kMaxM; j++) {
@@ -195,7 +204,10 @@ __device__ inline RingReads ring_reads(int head, int m, int col, int skip_slot) 
   }
   if (fb < 0) fb = 0;
 #pragma unroll
-  for (int j = 0; j < kMaxM; j++) r.rs[j] = r.use[j] ? slot[j] : fb;
+  for (int j = 0; j < kMaxM; j++) {
+    const int s = r.use[j] ? slot[j] : fb;
+    r.rs[j] = s ^ pslot_x[s];
+  }
   return r;
 }
 
@@ -211,7 +223,7 @@ __device__ inline void load_ring_raw(const T* __restrict__ W, long long i, const
 #pragma unroll
   for (int j = 0; j < kMaxM; j++) {
     yv[j] = row[rr.rs[j] * kTile];
-    sv[j] = row[(8 + rr.rs[j]) * kTile];
+    sv[j] = row[(kWPhys + rr.rs[j]) * kTile];
   }
 }
 

@@ -24,14 +24,22 @@ namespace lbfgsb {
 
 static const int kTail[] = {K_P4, K_FORMK, K_P5A, K_ALPHA, K_P5B, K_LSINIT, K_TRIAL};
 
-static std::vector<int> program_steps(int prog) {
+// spec: after an evaluation, pass A runs on the trial point before the
+// line-search step (which then takes g'd from it) instead of after it; the
+// step is usually accepted, and otherwise its work is discarded.  Not with
+// an after-iteration callback, which wants the accepted point first, and
+// only for whole chains (partial runs name the steps of the regular form).
+static std::vector<int> program_steps(int prog, bool spec) {
   std::vector<int> s;
   switch (prog) {
     case PROG_ITER_START:
       s = {K_RESUME_RESET, K_PASSA_SA, K_UPD_REDUCE, K_GCP};
       break;
     case PROG_AFTER_EVAL:
-      s = {K_GD, K_LSSTEP, K_TRIAL, K_PASSA_FUSED, K_STOP, K_UPD, K_GCP};
+      if (spec)
+        s = {K_PASSA_SPEC, K_LSSTOP, K_TRIAL, K_GCP};
+      else
+        s = {K_GD, K_LSSTEP, K_TRIAL, K_PASSA_FUSED, K_STOP, K_UPD, K_GCP};
       break;
     case PROG_AFTER_ACCEPT:
       s = {K_RESUME_RESET, K_PASSA_FUSED, K_STOP, K_UPD, K_GCP};
@@ -306,6 +314,14 @@ struct Engine final : EngineBase {
         launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0);
         launch_stop_tests(st, ds, part, pmax, pmin, pmini, G);
         break;
+      case K_PASSA_SPEC:
+        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, iwhere, W, d, tbrk, ds, part, pmax,
+                         pmin, pmini, 0, 1);
+        break;
+      case K_LSSTOP:
+        launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0, 1);
+        launch_ls_stop(st, ds, pmax, pmin, pmini, G, f);
+        break;
       case K_UPD:
         launch_update_gcp(st, ds, part, pmin, pmini, G, 0);
         break;
@@ -371,7 +387,8 @@ struct Engine final : EngineBase {
   }
 
   int run(int program, int start_step, int stop_step, double f) override {
-    std::vector<int> steps = program_steps(program);
+    std::vector<int> steps = program_steps(
+        program, !hs->has_after_iteration && start_step < 0 && stop_step == K_END);
     if (!hs->has_custom_stop && program != PROG_AFTER_CHECK && start_step != K_UPD &&
         stop_step != K_UPD) {
       // without a customised stopping test, the stopping tests already did
@@ -472,10 +489,10 @@ struct Engine final : EngineBase {
       std::vector<T> w(wsize(nn));
       HIPCHK(hipMemcpyAsync(w.data(), W, w.size() * sizeof(T), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
-      const int off = name == "ws" ? 8 : 0;
+      const int off = name == "ws" ? kWPhys : 0;
       T* o = (T*)out;
       for (size_t i = 0; i < nn; i++)
-        for (int j = 0; j < m; j++) o[i * m + j] = w[widx(i, off + j)];
+        for (int j = 0; j < m; j++) o[i * m + j] = w[widx(i, off + ring_phys(hs, j))];
       return 0;
     }
     if (name == "index" || name == "indx2") {
@@ -531,10 +548,10 @@ struct Engine final : EngineBase {
       std::vector<T> w(wsize(nn));
       HIPCHK(hipMemcpyAsync(w.data(), W, w.size() * sizeof(T), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
-      const int off = name == "ws" ? 8 : 0;
+      const int off = name == "ws" ? kWPhys : 0;
       const T* src = (const T*)in;
       for (size_t i = 0; i < nn; i++)
-        for (int j = 0; j < m; j++) w[widx(i, off + j)] = src[i * m + j];
+        for (int j = 0; j < m; j++) w[widx(i, off + ring_phys(hs, j))] = src[i * m + j];
       HIPCHK(hipMemcpyAsync(W, w.data(), w.size() * sizeof(T), hipMemcpyHostToDevice, st));
       HIPCHK(hipStreamSynchronize(st));
       return 0;

@@ -145,7 +145,7 @@ k_gcp_tile_reduce(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
   if (!ds->gcp_pending) return;
   const long long K = round_size(ds, cap);
   if ((long long)blockIdx.x * kBlock >= K) return;
-  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1);
+  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1, ds->pslot_x);
   const T* g = Gv[ds->phase & 1];
   double v[kNV];
 #pragma unroll
@@ -208,7 +208,7 @@ k_gcp_tile_eval(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__
   const long long K = round_size(ds, cap);
   if ((long long)blockIdx.x * kBlock >= K) return;
   stage_gcp_const(ds, gc);
-  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1);
+  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1, ds->pslot_x);
   const T* g = Gv[ds->phase & 1];
   const long long pos = (long long)blockIdx.x * kBlock + threadIdx.x;
   double v[kNV], ex[kNV];
@@ -300,7 +300,13 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
                   DevScalars* ds) {
   __shared__ RoundLds L;
   __shared__ DevScalars sds;
-  if (!ds->gcp_pending || ds->halt) return;
+  if (ds->halt) {
+    // a halted chain has no trial point left to write (a line-search step
+    // before this one may have requested one, and k_trial has written it)
+    if (threadIdx.x == 0) ds->write_trial = 0;
+    return;
+  }
+  if (!ds->gcp_pending) return;
   const long long K = (long long)ds->cand_count;
   if (K > kSmallRound) {
     if (threadIdx.x == 0) {

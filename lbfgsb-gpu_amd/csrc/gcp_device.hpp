@@ -367,7 +367,7 @@ template <typename T>
 __device__ __forceinline__ void block_scan_range(const DevScalars* ds, RoundLds& L, const Cand* cands,
                                         long long c_lo, long long c_hi, long long K, bool test,
                                         const T* g, const T* W) {
-  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1);
+  const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1, ds->pslot_x);
   const double theta = ds->theta;
   const long long kpassed = ds->gcp_kpassed;
   const double t_prev0 = ds->gcp_t_prev;

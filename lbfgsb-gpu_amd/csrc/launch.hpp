@@ -35,7 +35,7 @@ template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X, Pair<T> Gv,
                    const T* l, const T* u, uint8_t* iwhere, T* W, const T* d,
                    T* tbrk, DevScalars* ds, double* part, double* pmax, double* pmin,
-                   long long* pmini, int rev);
+                   long long* pmini, int rev, int spec = 0);
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                const T* u, uint8_t* iwhere, const T* W, DevScalars* ds,
@@ -75,8 +75,10 @@ void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
 // The device state into `host` (fine-grained pinned memory), then seq into m.
 void launch_publish(hipStream_t s, const DevScalars* ds, DevScalars* host, MailSeq* m,
                     unsigned seq);
+// spec: sum even when the chain is halted (after the speculative pass A,
+// which runs on a chain halted by an evaluation request).
 void launch_rowsum(hipStream_t s, const double* part, int G, int v0, int nrows, DevScalars* ds,
-                   int p4_rows);
+                   int p4_rows, int spec = 0);
 void launch_resume(hipStream_t s, DevScalars* ds, int reset_bp);
 void launch_setup_errclb(hipStream_t s, DevScalars* ds, const double* part, int G);
 void launch_setup_active(hipStream_t s, DevScalars* ds, const double* part, int G);
@@ -91,6 +93,10 @@ void launch_alpha(hipStream_t s, DevScalars* ds, const double* pmin, const long 
 void launch_ls_init(hipStream_t s, DevScalars* ds, const double* part, const double* pmin,
                     int G);
 void launch_ls_step(hipStream_t s, DevScalars* ds, const double* part, int G, double f);
+// Line-search step on the g'd of a speculative pass A (row PA_GD of the
+// row-summed pass-A partials), then on acceptance the stopping tests.
+void launch_ls_stop(hipStream_t s, DevScalars* ds, const double* pmax, const double* pmin,
+                    const long long* pmini, int G, double f);
 
 // gcp.hip
 struct GcpScratch {

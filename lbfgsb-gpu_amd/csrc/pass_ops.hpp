@@ -31,8 +31,8 @@ struct PassAOp {
   T* __restrict__ W;
   const T* __restrict__ d;
   T* __restrict__ tout;  // breakpoint t of every variable (+inf: none), or null
-  int col, itail;
-  bool upd, stp1, dstored;
+  int col, itail, wnew;
+  bool upd, stp1, dstored, spec;
   T stp;
   RingReads rr;
   double acc[PA_COUNT];
@@ -47,10 +47,15 @@ struct PassAOp {
     T yv[kMaxM], sv[kMaxM];
   };
 
+  // spec: run on the evaluated trial point before the line search has
+  // accepted it (FUSED only): the trial buffers are the new iterate, the
+  // ring is the one an accepted step with a memory update leaves, the new
+  // pair goes to the spare physical slot, and g'd is summed as well.
   __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
                               const T* u_, uint8_t* iwhere_, T* W_,
-                              const T* d_, T* tout_ = nullptr) {
-    const int cur = ds->phase & 1;
+                              const T* d_, T* tout_ = nullptr, bool spec_ = false) {
+    spec = FUSED && spec_;
+    const int cur = (ds->phase & 1) ^ (spec ? 1 : 0);
     tout = tout_;
     x = X[cur];
     g = Gv[cur];
@@ -61,14 +66,32 @@ struct PassAOp {
     iwhere = iwhere_;
     W = W_;
     d = d_;
-    col = ds->col;
-    itail = ds->itail;
-    upd = FUSED && ds->updatd;
     stp = (T)ds->stp;
     stp1 = ds->stp == 1.0;
     dstored = ds->d_stored;
-    // the slot being replaced by the new pair is not read
-    rr = ring_reads(ds->head, ds->m, col, upd ? itail : -1);
+    if (spec) {
+      // the ring after accept_step with updatd = 1 (lbfgsbcpu.cpp:437-449)
+      const int m = ds->m, iu = ds->iupdat + 1;
+      int head = ds->head;
+      if (iu <= m) {
+        col = iu;
+        itail = (head + iu - 1) % m;
+      } else {
+        col = ds->col;
+        itail = (ds->itail + 1) % m;
+        head = (head + 1) % m;
+      }
+      upd = true;
+      rr = ring_reads(head, m, col, itail, ds->pslot_x);
+      wnew = ring_spare(ds);
+    } else {
+      col = ds->col;
+      itail = ds->itail;
+      upd = FUSED && ds->updatd;
+      // the slot being replaced by the new pair is not read
+      rr = ring_reads(ds->head, ds->m, col, upd ? itail : -1, ds->pslot_x);
+      wnew = ring_phys(ds, itail);  // physical slot the new pair goes to
+    }
 #pragma unroll
     for (int v = 0; v < PA_COUNT; v++) acc[v] = 0.0;
     sbg = 0.0;
@@ -107,8 +130,9 @@ struct PassAOp {
         const T dd = dstored ? e.dd : xi - e.xo;
         ss = stp1 ? dd : dd * stp;
         acc_prod(acc[PA_RR], ys, ys);
-        st_nt(&W[widx(i, itail)], ys);
-        st_nt(&W[widx(i, 8 + itail)], ss);
+        if (spec) acc_prod(acc[PA_GD], gi, dd);  // g'd of the line search
+        st_nt(&W[widx(i, wnew)], ys);
+        st_nt(&W[widx(i, kWPhys + wnew)], ss);
       }
     }
     // Unused positions hold zeros (mask_ring), and with an update the
@@ -304,7 +328,7 @@ struct P4Op {
     track = ds->iter > 0 && ds->cnstnd && !shortcut;
     newcol = ds->updatd && col > 0;
     const int upcl = ds->updatd ? col - 1 : col;
-    rr = ring_reads(head, m, col, -1);
+    rr = ring_reads(head, m, col, -1, ds->pslot_x);
 #pragma unroll
     for (int v = 0; v < kAcc; v++) acc[v] = 0.0;
     const int lane = threadIdx.x & 63;
@@ -318,8 +342,11 @@ struct P4Op {
       int sa = head + a, sb = head + b;
       if (sa >= m) sa -= m;
       if (sb >= m) sb -= m;
-      wa[q] = cat == 0 ? sa : 8 + sa;
-      wb[q] = cat == 1 ? 8 + sb : sb;
+      // columns of the pair's two factors: Y of a physical slot p is column
+      // p, S is column kWPhys + p
+      const int pa = ring_phys(ds, sa), pb = ring_phys(ds, sb);
+      wa[q] = cat == 0 ? pa : kWPhys + pa;
+      wb[q] = cat == 1 ? kWPhys + pb : pb;
     }
   }
   __device__ __forceinline__ void load(In& e, long long i) const {
@@ -331,7 +358,7 @@ struct P4Op {
     load_ring_raw(W, i, rr, e.yv, e.sv);
     const T* __restrict__ row = W + widx((size_t)i, 0);
     e.yn = row[rr.newest * kTile];
-    e.sn = row[(8 + rr.newest) * kTile];
+    e.sn = row[(kWPhys + rr.newest) * kTile];
   }
   __device__ __forceinline__ void keep(const In& e) const {
     keep_loads(e.x, e.g, e.l, e.u, e.iwb, e.yv, e.sv, e.yn, e.sn);
@@ -400,7 +427,7 @@ struct P4Op {
       fm &= fm - 1ull;
       const long long ei = __shfl(i, src, kWave);
       const double sg = (double)__shfl(flag, src, kWave);
-      const double wrow = lane < 16 ? (double)W[widx(ei, lane)] : 0.0;
+      const double wrow = lane < kWCols ? (double)W[widx(ei, lane)] : 0.0;
 #pragma unroll
       for (int q = 0; q < 3; q++) {
         const double va = __shfl(wrow, wa[q], kWave);
@@ -458,7 +485,7 @@ struct P5aOp {
       wy[j] = j < col ? ds->wv[j] / theta : 0.0;
       wsv[j] = j < col ? ds->wv[col + j] : 0.0;
     }
-    rr = ring_reads(ds->head, ds->m, col, -1);
+    rr = ring_reads(ds->head, ds->m, col, -1, ds->pslot_x);
     amin = 1.0;
     ai = 0x7fffffffffffffffLL;
   }

@@ -155,10 +155,11 @@ __device__ void reduce_min(const double* pv, int G, double* sh, double& v) {
 // entries) into ds->red[b] with a compensated block tree.  p4_rows skips the
 // rows of k_p4 that the iteration does not use (entering/leaving Gram when
 // nothing entered or left, the new formk column when no update happened).
+// spec: the rows of a speculative pass A, summed on a halted chain.
 __global__ void __launch_bounds__(kBlock)
-k_rowsum(const double* part, int G, int v0, DevScalars* ds, int p4_rows) {
+k_rowsum(const double* part, int G, int v0, DevScalars* ds, int p4_rows, int spec) {
   __shared__ double ss[kWavesPerBlock], sc[kWavesPerBlock];
-  if (ds->halt) return;
+  if (ds->halt && !spec) return;
   const int v = blockIdx.x;
   if (p4_rows) {
     const bool skip = (v >= P4_CORR && !ds->p4_flagged) ||
@@ -351,6 +352,45 @@ s_ls_step(DevScalars* ds, const double* part, int G, double f) {
   ds_store(ds, s);
 }
 
+// Line-search step after a speculative pass A on the evaluated trial point
+// (k_pass_a with spec), with g'd from that pass (ds->red[PA_GD], summed by
+// k_rowsum).  On acceptance with the memory update pass A assumed (updatd,
+// the evaluated point itself, no custom trial write), the spare ring slot
+// holding the new pair takes the newest logical slot and the stopping tests
+// follow as in s_stop_tests; on any other acceptance the status is ST_REDO
+// and the host runs the regular tail pass.
+__global__ void __launch_bounds__(kBlock)
+s_ls_stop(DevScalars* ds, const double* pmax, const double* pmin, const long long* pmini,
+          int G, double f) {
+  __shared__ DevScalars sds;
+  __shared__ double sh[kWavesPerBlock];
+  __shared__ long long shi[kWavesPerBlock];
+  const double sbg = reduce_max(pmax, G, sh);
+  double bk;
+  long long ibk;
+  reduce_argmin(pmin, pmini, G, sh, shi, bk, ibk);
+  ds_load(&sds, ds);
+  DevScalars* s = &sds;
+  if (threadIdx.x == 0) {
+    const int iter0 = s->iter;
+    const double stp0 = s->stp;
+    logic_ls_step(s, f, s->red[PA_GD]);
+    if (s->iter != iter0 && s->status == ST_RUNNING) {
+      if (s->updatd && !s->write_trial && s->stp == stp0) {
+        ring_take_spare(s, s->itail);
+        logic_stop_tests(s, s->red, sbg, bk, ibk);
+      } else {
+        s->status = ST_REDO;
+        s->halt = 1;
+      }
+    }
+  }
+  __syncthreads();
+  post_update_block(s);
+  build_middle(s);
+  ds_store(ds, s);
+}
+
 // The device state into the host's pinned copy, then the sequence number
 // (the end of every chain the host waits on).
 __global__ void __launch_bounds__(kBlock)
@@ -367,8 +407,9 @@ void launch_publish(hipStream_t s, const DevScalars* ds, DevScalars* host, MailS
   hipLaunchKernelGGL(s_publish, dim3(1), dim3(kBlock), 0, s, ds, host, m, seq);
 }
 void launch_rowsum(hipStream_t s, const double* part, int G, int v0, int nrows, DevScalars* ds,
-                   int p4_rows) {
-  hipLaunchKernelGGL(k_rowsum, dim3(nrows), dim3(kBlock), 0, s, part, G, v0, ds, p4_rows);
+                   int p4_rows, int spec) {
+  hipLaunchKernelGGL(k_rowsum, dim3(nrows), dim3(kBlock), 0, s, part, G, v0, ds, p4_rows,
+                     spec);
 }
 void launch_resume(hipStream_t s, DevScalars* ds, int reset_bp) {
   hipLaunchKernelGGL(s_resume, dim3(1), dim3(64), 0, s, ds, reset_bp);
@@ -404,6 +445,10 @@ void launch_ls_init(hipStream_t s, DevScalars* ds, const double* part, const dou
 }
 void launch_ls_step(hipStream_t s, DevScalars* ds, const double* part, int G, double f) {
   hipLaunchKernelGGL(s_ls_step, dim3(1), dim3(kBlock), 0, s, ds, part, G, f);
+}
+void launch_ls_stop(hipStream_t s, DevScalars* ds, const double* pmax, const double* pmin,
+                    const long long* pmini, int G, double f) {
+  hipLaunchKernelGGL(s_ls_stop, dim3(1), dim3(kBlock), 0, s, ds, pmax, pmin, pmini, G, f);
 }
 
 #undef A1

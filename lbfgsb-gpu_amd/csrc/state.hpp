@@ -8,7 +8,10 @@
 // bulk type.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "common.hpp"
 
 namespace lbfgsb {
 
@@ -32,9 +35,11 @@ enum Step : int {
   K_GD,
   K_LSSTEP,
   K_END,        // also the timing slot of host-sized breakpoint rounds
-  K_GCP = 17    // device-sized breakpoint round (select + one-workgroup round)
+  K_GCP = 17,   // device-sized breakpoint round (select + one-workgroup round)
+  K_PASSA_SPEC = 18,  // pass A on the evaluated trial point before its acceptance
+  K_LSSTOP = 19       // line-search step, then on acceptance the stopping tests
 };
-constexpr int kNumSteps = 18;
+constexpr int kNumSteps = 20;
 
 // Programs (sequences of steps).
 enum Program : int {
@@ -53,7 +58,8 @@ enum Status : int {
   ST_CHECKED = 3,     // stopping tests passed, custom stop may be asked
   ST_GCP_ROUNDS = 4,  // breakpoint search needs sorted rounds
   ST_RESTART = 5,     // memory reset: the iteration restarts at the Cauchy step
-  ST_DONE = 6         // solve ended, info holds the reason
+  ST_DONE = 6,        // solve ended, info holds the reason
+  ST_REDO = 7         // step accepted, but not as pass A assumed: run it again
 };
 
 // Slot layout of per-block partial sums (value-major: part[v * G + block]).
@@ -67,7 +73,8 @@ enum PartA : int {
   PA_SY = 6,        // 8: s . y_j for the retained slots
   PA_SS = 14,       // 8: s_j . s for the retained slots
   PA_P = 22,        // 16: p = [Y'd ; S'd] over GCP-free variables
-  PA_COUNT = 38
+  PA_GD = 38,       // g'd at the evaluated trial point (pass A ahead of acceptance)
+  PA_COUNT = 39
 };
 enum Part4 : int {
   P4_NFREE = 0,
@@ -158,6 +165,11 @@ struct DevScalars {
   // first trial that overwrites z (d_stored; trial_d: this k_trial does it),
   // and is otherwise the difference of the two x buffers.
   int write_trial, trial_src, trial_dst, trial_has_z, d_stored, trial_d;
+  // ---- physical W slots: ring slot s is physical slot s ^ pslot_x[s], the
+  // spare is kMaxM ^ pspare_x (zero-initialised state = the identity)
+  int pslot_x[kMaxM];
+  int pspare_x;
+  int pad_ring;
   int has_after_iteration, has_custom_stop;
 
   // ---- append counters (breakpoint list, round candidates)
@@ -202,6 +214,18 @@ __device__ inline void ds_store(DevScalars* ds, const DevScalars* sds) {
   const unsigned long long* src = (const unsigned long long*)sds;
   unsigned long long* dst = (unsigned long long*)ds;
   for (int i = threadIdx.x; i < kDsWords; i += blockDim.x) dst[i] = src[i];
+}
+
+// Physical W slot of ring slot s, the spare, and the exchange that makes the
+// spare (holding a new pair) ring slot s.
+__host__ __device__ inline int ring_phys(const DevScalars* ds, int s) {
+  return s ^ ds->pslot_x[s];
+}
+__host__ __device__ inline int ring_spare(const DevScalars* ds) { return kMaxM ^ ds->pspare_x; }
+__host__ __device__ inline void ring_take_spare(DevScalars* ds, int s) {
+  const int old = ring_phys(ds, s);
+  ds->pslot_x[s] = ring_spare(ds) ^ s;
+  ds->pspare_x = old ^ kMaxM;
 }
 
 }  // namespace lbfgsb

@@ -11,7 +11,8 @@ from . import _lib
 from .problems import Problem
 
 # engine status codes (DevScalars::status)
-ST_RUNNING, ST_NEED_EVAL, ST_ACCEPTED, ST_CHECKED, ST_GCP_ROUNDS, ST_RESTART, ST_DONE = range(7)
+(ST_RUNNING, ST_NEED_EVAL, ST_ACCEPTED, ST_CHECKED, ST_GCP_ROUNDS, ST_RESTART, ST_DONE,
+ ST_REDO) = range(8)
 # programs and steps of lbfgsb_engine_run
 PROG_ITER_START, PROG_AFTER_EVAL, PROG_AFTER_ACCEPT, PROG_AFTER_CHECK, PROG_TAIL = range(5)
 (K_RESUME_RESET, K_RESUME, K_PASSA_SA, K_UPD_REDUCE, K_PASSA_FUSED, K_STOP, K_UPD, K_P4,
@@ -197,7 +198,7 @@ class Engine:
         elif name in ("p", "c", "mc", "wv"):
             out = np.zeros(16 if count is None else count)
         elif name == "pa":
-            out = np.zeros(38)
+            out = np.zeros(39)
         else:
             out = np.zeros(n, dtype=self.np_dtype)
         rc = self.L.lbfgsb_engine_get_array(self.h, name.encode(),
@@ -225,7 +226,7 @@ class Engine:
                 prog = PROG_AFTER_EVAL
             elif st == ST_RESTART:
                 prog = PROG_ITER_START
-            elif st == ST_ACCEPTED:
+            elif st in (ST_ACCEPTED, ST_REDO):
                 prog = PROG_AFTER_ACCEPT
             elif st == ST_CHECKED:
                 prog = PROG_AFTER_CHECK
