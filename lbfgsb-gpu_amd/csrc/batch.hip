@@ -23,6 +23,9 @@
 // member 0 writes the problem's results and control state back.
 // the compact bmv: the kernel's registers are shared with its bulk passes
 #define LBFGSB_COMPACT_BMV 1
+// the ring is never exchanged with the spare slot here (pass A runs after the
+// acceptance): physical W slots are the ring slots
+#define LBFGSB_RING_IDENTITY 1
 #include "batch.hpp"
 #include "common.hpp"
 #include "control.hpp"
@@ -74,22 +77,30 @@ struct Prob {
 
 __device__ __forceinline__ int nwaves() { return blockDim.x >> 6; }
 
-// Sums of NV per-thread values over the workgroup into out[0 .. NV).
-template <int NV>
-__device__ __forceinline__ void bsum(const double (&v)[NV], double* sh, double* out) {
+// Sums of the first NR of NV per-thread values over the workgroup into
+// out[0 .. NR).
+template <int NR, int NV>
+__device__ __forceinline__ void bsum_first(const double (&v)[NV], double* sh, double* out) {
+  static_assert(NR <= NV, "more sums than values");
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = nwaves();
 #pragma unroll
-  for (int q = 0; q < NV; q++) {
+  for (int q = 0; q < NR; q++) {
     const double s = wave_sum(v[q]);
-    if (lane == 0) sh[wid * NV + q] = s;
+    if (lane == 0) sh[wid * NR + q] = s;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < NV; q += blockDim.x) {
+  for (int q = threadIdx.x; q < NR; q += blockDim.x) {
     double t = 0.0;
-    for (int w = 0; w < nw; w++) t += sh[w * NV + q];
+    for (int w = 0; w < nw; w++) t += sh[w * NR + q];
     out[q] = t;
   }
   __syncthreads();
+}
+
+// Sums of NV per-thread values over the workgroup into out[0 .. NV).
+template <int NV>
+__device__ __forceinline__ void bsum(const double (&v)[NV], double* sh, double* out) {
+  bsum_first<NV>(v, sh, out);
 }
 
 __device__ __forceinline__ double bmax(double v, double* sh) {
@@ -352,7 +363,9 @@ __device__ __forceinline__ void b_pass_a(const Prob<T>& P, BatchLds& L, Gang<GO>
     op.element(i, e);
   }
   op.finish();
-  bsum<PA_COUNT>(op.acc, L.u.r.sh, L.red);
+  // rows 0 .. PA_GD - 1: the g'd row of the plain engine's speculative pass
+  // is not formed here (the line search has its own g'd pass)
+  bsum_first<PA_GD>(op.acc, L.u.r.sh, L.red);
   const double sbg = bmax(op.sbg, L.u.r.sh);
   double bk = op.bkmin;
   long long ibk = op.ibk;
@@ -361,9 +374,10 @@ __device__ __forceinline__ void b_pass_a(const Prob<T>& P, BatchLds& L, Gang<GO>
     L.rv[0] = sbg;
     L.rv[1] = bk;
     L.ri[0] = ibk;
+    L.red[PA_GD] = 0.0;
   }
   __syncthreads();
-  gang_combine(G, L, PA_COUNT, 1, 3, 0, 0, &L.gflag);
+  gang_combine(G, L, PA_GD, 1, 3, 0, 0, &L.gflag);
 }
 
 template <typename T, bool GO>

@@ -184,6 +184,21 @@ __device__ inline void ring_slots(int head, int m, int (&slot)[kMaxM]) {
 // instead: an address the same wave is reading anyway, so the load hits in
 // cache and the loop needs no branch around it.
 // rs[j] and newest are physical slots (pslot_x maps ring slots, above).
+// Physical W slot of ring slot s under the exchange table pslot_x.  A
+// translation unit whose kernels never move the spare slot in (the batched
+// kernel) defines LBFGSB_RING_IDENTITY before its includes: its device code
+// then uses the identity, which the zero table encodes anyway, without the
+// table lookups (they cost the batched kernel registers).  Host code is the
+// same in every translation unit.
+__host__ __device__ inline int ring_map(int s, const int* pslot_x) {
+#if defined(LBFGSB_RING_IDENTITY) && defined(__HIP_DEVICE_COMPILE__)
+  (void)pslot_x;
+  return s;
+#else
+  return s ^ pslot_x[s];
+#endif
+}
+
 struct RingReads {
   int rs[kMaxM];
   bool use[kMaxM];
@@ -195,7 +210,7 @@ __device__ inline RingReads ring_reads(int head, int m, int col, int skip_slot,
   int slot[kMaxM];
   ring_slots(head, m, slot);
   const int nl = col > 0 ? (head + col - 1) % m : 0;
-  r.newest = nl ^ pslot_x[nl];
+  r.newest = ring_map(nl, pslot_x);
   int fb = -1;
 #pragma unroll
   for (int j = 0; j < kMaxM; j++) {
@@ -206,7 +221,7 @@ __device__ inline RingReads ring_reads(int head, int m, int col, int skip_slot,
 #pragma unroll
   for (int j = 0; j < kMaxM; j++) {
     const int s = r.use[j] ? slot[j] : fb;
-    r.rs[j] = s ^ pslot_x[s];
+    r.rs[j] = ring_map(s, pslot_x);
   }
   return r;
 }

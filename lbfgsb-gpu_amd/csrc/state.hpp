@@ -219,13 +219,17 @@ __device__ inline void ds_store(DevScalars* ds, const DevScalars* sds) {
 // Physical W slot of ring slot s, the spare, and the exchange that makes the
 // spare (holding a new pair) ring slot s.
 __host__ __device__ inline int ring_phys(const DevScalars* ds, int s) {
-  return s ^ ds->pslot_x[s];
+  return ring_map(s, ds->pslot_x);
 }
 __host__ __device__ inline int ring_spare(const DevScalars* ds) { return kMaxM ^ ds->pspare_x; }
+#if defined(LBFGSB_RING_IDENTITY) && defined(__HIP_DEVICE_COMPILE__)
+__device__ void ring_take_spare(DevScalars* ds, int s) = delete;  // the identity stays
+#else
 __host__ __device__ inline void ring_take_spare(DevScalars* ds, int s) {
   const int old = ring_phys(ds, s);
   ds->pslot_x[s] = ring_spare(ds) ^ s;
   ds->pspare_x = old ^ kMaxM;
 }
+#endif
 
 }  // namespace lbfgsb
