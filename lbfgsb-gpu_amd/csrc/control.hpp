@@ -542,6 +542,51 @@ __device__ inline void post_update_block(DevScalars* s) {
   __syncthreads();
 }
 
+// post_update_block followed by build_middle, with the columns M e_k formed
+// by the second wave while thread 0 runs the Cauchy controller (both only
+// read sy and wt; the controller changes col only when it discards the
+// memory, and then the columns are formed again for the new col).  The same
+// results as the two calls in sequence; needs at least 64 + 16 threads.
+__device__ inline void post_update_middle(DevScalars* s) {
+  const int mode = s->post_update;
+  if (!mode) {
+    build_middle(s);
+    return;
+  }
+  __shared__ double s_e[16][17], s_out[16][17];
+  __syncthreads();
+  if (mode == 1) {
+    if (threadIdx.x == 0) s->blk_flag = memory_update_pre(s) ? 1 : 0;
+    __syncthreads();
+    if (s->blk_flag) formt_block(s);
+  }
+  const int col = s->col;
+  const int k = (int)threadIdx.x - kWave;
+  if (threadIdx.x == 0) {
+    gcp_controller(s);
+    s->upd_done = 1;
+    s->post_update = 0;
+  } else if (k >= 0 && k < 2 * col) {
+    double* e = s_e[k];
+    for (int i = 0; i < 16; i++) e[i] = 0.0;
+    e[k] = 1.0;
+    d_bmv(s->sy, s->wt, col, e, s_out[k]);
+  }
+  __syncthreads();
+  if (!s->gcp_pending) return;
+  if (s->col != col) {
+    build_middle(s);
+    return;
+  }
+  for (int q = threadIdx.x; q < 256; q += blockDim.x) s->Mx[q] = 0.0;
+  __syncthreads();
+  if (k >= 0 && k < 2 * col) {
+    // the padded layout of build_middle
+    const int kp = k < col ? k : 8 + k - col;
+    for (int r = 0; r < 2 * col; r++) s->Mx[(r < col ? r : 8 + r - col) * 16 + kp] = s_out[k][r];
+  }
+}
+
 // After the Cauchy-point pass, by the whole workgroup: free/active
 // bookkeeping (lbfgsbcpu.cpp:1143-1161) on thread 0, formk, and the two
 // triangular solves of subsm (lbfgsbcpu.cpp:1459-1471) on thread 0 with the

@@ -37,7 +37,10 @@ static std::vector<int> program_steps(int prog, bool spec) {
       break;
     case PROG_AFTER_EVAL:
       if (spec)
-        s = {K_PASSA_SPEC, K_LSSTOP, K_TRIAL, K_GCP};
+        // a trial point the line-search step requests is written by the
+        // tail's k_trial (the kernels between no-op on the halted chain), so
+        // an accepted step launches no k_trial of its own
+        s = {K_PASSA_SPEC, K_LSSTOP, K_GCP};
       else
         s = {K_GD, K_LSSTEP, K_TRIAL, K_PASSA_FUSED, K_STOP, K_UPD, K_GCP};
       break;

@@ -300,13 +300,7 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
                   DevScalars* ds) {
   __shared__ RoundLds L;
   __shared__ DevScalars sds;
-  if (ds->halt) {
-    // a halted chain has no trial point left to write (a line-search step
-    // before this one may have requested one, and k_trial has written it)
-    if (threadIdx.x == 0) ds->write_trial = 0;
-    return;
-  }
-  if (!ds->gcp_pending) return;
+  if (!ds->gcp_pending || ds->halt) return;
   const long long K = (long long)ds->cand_count;
   if (K > kSmallRound) {
     if (threadIdx.x == 0) {

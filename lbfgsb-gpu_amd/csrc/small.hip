@@ -262,8 +262,7 @@ s_stop_tests(DevScalars* ds, const double* part, const double* pmax,
   // s->red holds the row sums of the pass-A partials (k_rowsum)
   if (threadIdx.x == 0) logic_stop_tests(s, s->red, sbg, bk, ibk);
   __syncthreads();
-  post_update_block(s);
-  build_middle(s);
+  post_update_middle(s);
   ds_store(ds, s);
 }
 
@@ -288,8 +287,7 @@ s_update_gcp(DevScalars* ds, const double* part, const double* pmin,
   (void)part;
   if (threadIdx.x == 0) logic_update_gcp(s, reduce, s->red, bk, ibk);
   __syncthreads();
-  post_update_block(s);
-  build_middle(s);
+  post_update_middle(s);
   ds_store(ds, s);
 }
 
@@ -386,8 +384,7 @@ s_ls_stop(DevScalars* ds, const double* pmax, const double* pmin, const long lon
     }
   }
   __syncthreads();
-  post_update_block(s);
-  build_middle(s);
+  post_update_middle(s);
   ds_store(ds, s);
 }
 
