@@ -7,7 +7,9 @@
 //   k_pass_a    [iteration tail: projgr, ||dx||, y=g-g_old, y'y, ring write,
 //                new Gram row]  + Cauchy classification, f1, p = W'd and the
 //                breakpoint list                        (lbfgsbcpu.cpp:394-449,
-//                                                        567-701, 1341-1387)
+//                                                        567-701, 1341-1387);
+//                speculative form on the evaluated trial point, with g'd
+//                                                      (lbfgsbcpu.cpp:1312)
 //   k_p4        Cauchy point, free/active counts, new formk column, signed
 //               entering/leaving Gram, reduced gradient r and W'r
 //                                                      (lbfgsbcpu.cpp:705-817,
@@ -17,7 +19,8 @@
 //   k_p5b       backtrack, d = z - x, d'd, g'd, stpmx     (lbfgsbcpu.cpp:1517-1533,
 //                                                        343-345, 1263-1299)
 //   k_trial     line-search trial point                  (lbfgsbcpu.cpp:1329-1335)
-//   k_gd        g'd after an evaluation                   (lbfgsbcpu.cpp:1312)
+//   k_gd        g'd after an evaluation, in the regular program and for
+//               trial points that pass A did not run ahead of (lbfgsbcpu.cpp:1312)
 //
 // Every kernel is a grid-stride loop over 256-thread blocks; per-block
 // partial sums go to part[v * G + block] and are finished by the

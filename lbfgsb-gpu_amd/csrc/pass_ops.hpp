@@ -621,7 +621,7 @@ struct P5bOp {
     const T big = (T)1.0e10;
     const T xi = e.x;
     const T zi = subspace_point(i, e);
-    z[i] = zi;  // read back soon (objective, k_gd): normal stores
+    z[i] = zi;  // read back soon (objective, pass A on the trial point): normal stores
     const T di = zi - xi;  // d is not stored: z - x while z is the trial point
     acc[P5_DTD] += (double)di * (double)di;
     acc[P5_GD] += (double)e.g * (double)di;
