@@ -54,9 +54,9 @@ def kernel_bytes_per_elem(s: int) -> dict:
       trial:        nothing: the first trial is a unit step, whose trial point
                     z is already in place (the first non-unit step moves 4s
                     and stores d)
-      gd:           read g_trial, x_trial, x (3s): d = z - x; only for
-                    trial points after the first of an iteration, or in the
-                    regular (non-speculative) program
+      gd:           read g_trial, x_trial, x (3s): d = z - x; only in the
+                    regular program (with an after-iteration callback), since
+                    the speculative pass A sums g'd of every evaluated point
     x_cp and r are recomputed per element by the passes that need them (from
     x, g, l, u, the class and W) instead of being stored and read back."""
     return {"pass_a_spec": 23 * s + 1, "pass_a_fused": 23 * s + 1, "p4": 20 * s + 2,

@@ -19,8 +19,8 @@
 //   k_p5b       backtrack, d = z - x, d'd, g'd, stpmx     (lbfgsbcpu.cpp:1517-1533,
 //                                                        343-345, 1263-1299)
 //   k_trial     line-search trial point                  (lbfgsbcpu.cpp:1329-1335)
-//   k_gd        g'd after an evaluation, in the regular program and for
-//               trial points that pass A did not run ahead of (lbfgsbcpu.cpp:1312)
+//   k_gd        g'd after an evaluation in the regular program (chains with
+//               an after-iteration callback, partial runs) (lbfgsbcpu.cpp:1312)
 //
 // Every kernel is a grid-stride loop over 256-thread blocks; per-block
 // partial sums go to part[v * G + block] and are finished by the
