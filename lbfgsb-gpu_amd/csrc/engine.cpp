@@ -314,7 +314,7 @@ struct Engine final : EngineBase {
         launch_gcp_round_small<T>(st, ca, dG, W, ds);
         break;
       case K_STOP:
-        launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0);
+        launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0, 0, pmax, pmin, pmini);
         launch_stop_tests(st, ds, part, pmax, pmin, pmini, G);
         break;
       case K_PASSA_SPEC:
@@ -322,7 +322,7 @@ struct Engine final : EngineBase {
                          pmin, pmini, 0, 1);
         break;
       case K_LSSTOP:
-        launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0, 1);
+        launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0, 1, pmax, pmin, pmini);
         launch_ls_stop(st, ds, pmax, pmin, pmini, G, f);
         break;
       case K_UPD:

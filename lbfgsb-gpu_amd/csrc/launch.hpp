@@ -76,9 +76,12 @@ void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
 void launch_publish(hipStream_t s, const DevScalars* ds, DevScalars* host, MailSeq* m,
                     unsigned seq);
 // spec: sum even when the chain is halted (after the speculative pass A,
-// which runs on a chain halted by an evaluation request).
+// which runs on a chain halted by an evaluation request).  With pmax (pass
+// A's partials), one more block finishes the projected-gradient norm and
+// the smallest breakpoint into ds->red_sbg, red_bk, red_ibk.
 void launch_rowsum(hipStream_t s, const double* part, int G, int v0, int nrows, DevScalars* ds,
-                   int p4_rows, int spec = 0);
+                   int p4_rows, int spec = 0, const double* pmax = nullptr,
+                   const double* pmin = nullptr, const long long* pmini = nullptr);
 void launch_resume(hipStream_t s, DevScalars* ds, int reset_bp);
 void launch_setup_errclb(hipStream_t s, DevScalars* ds, const double* part, int G);
 void launch_setup_active(hipStream_t s, DevScalars* ds, const double* part, int G);

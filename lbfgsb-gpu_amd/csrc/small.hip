@@ -155,11 +155,27 @@ __device__ void reduce_min(const double* pv, int G, double* sh, double& v) {
 // entries) into ds->red[b] with a compensated block tree.  p4_rows skips the
 // rows of k_p4 that the iteration does not use (entering/leaving Gram when
 // nothing entered or left, the new formk column when no update happened).
-// spec: the rows of a speculative pass A, summed on a halted chain.
+// spec: the rows of a speculative pass A, summed on a halted chain.  With
+// pmax, one block more finishes pass A's projected-gradient norm and smallest
+// breakpoint into ds->red_sbg, red_bk, red_ibk.
 __global__ void __launch_bounds__(kBlock)
-k_rowsum(const double* part, int G, int v0, DevScalars* ds, int p4_rows, int spec) {
+k_rowsum(const double* part, int G, int v0, DevScalars* ds, int p4_rows, int spec,
+         const double* pmax, const double* pmin, const long long* pmini) {
   __shared__ double ss[kWavesPerBlock], sc[kWavesPerBlock];
+  __shared__ long long si[kWavesPerBlock];
   if (ds->halt && !spec) return;
+  if (pmax && blockIdx.x == gridDim.x - 1) {
+    const double sbg = reduce_max(pmax, G, ss);
+    double bk;
+    long long ibk;
+    reduce_argmin(pmin, pmini, G, ss, si, bk, ibk);
+    if (threadIdx.x == 0) {
+      ds->red_sbg = sbg;
+      ds->red_bk = bk;
+      ds->red_ibk = ibk;
+    }
+    return;
+  }
   const int v = blockIdx.x;
   if (p4_rows) {
     const bool skip = (v >= P4_CORR && !ds->p4_flagged) ||
@@ -246,21 +262,20 @@ __global__ void __launch_bounds__(kBlock)
 s_stop_tests(DevScalars* ds, const double* part, const double* pmax,
                              const double* pmin, const long long* pmini, int G) {
   __shared__ DevScalars sds;
-  __shared__ double sh[kWavesPerBlock];
-  __shared__ long long shi[kWavesPerBlock];
   if (ds->halt) {
     if (threadIdx.x == 0) ds->write_trial = 0;
     return;
   }
-  const double sbg = reduce_max(pmax, G, sh);
-  double bk;
-  long long ibk;
-  reduce_argmin(pmin, pmini, G, sh, shi, bk, ibk);
+  (void)part;
+  (void)pmax;
+  (void)pmin;
+  (void)pmini;
+  (void)G;
   ds_load(&sds, ds);
   DevScalars* s = &sds;
-  (void)part;
-  // s->red holds the row sums of the pass-A partials (k_rowsum)
-  if (threadIdx.x == 0) logic_stop_tests(s, s->red, sbg, bk, ibk);
+  // s->red holds the row sums of the pass-A partials, red_sbg / red_bk /
+  // red_ibk the norm and smallest breakpoint (k_rowsum)
+  if (threadIdx.x == 0) logic_stop_tests(s, s->red, s->red_sbg, s->red_bk, s->red_ibk);
   __syncthreads();
   post_update_middle(s);
   ds_store(ds, s);
@@ -361,12 +376,10 @@ __global__ void __launch_bounds__(kBlock)
 s_ls_stop(DevScalars* ds, const double* pmax, const double* pmin, const long long* pmini,
           int G, double f) {
   __shared__ DevScalars sds;
-  __shared__ double sh[kWavesPerBlock];
-  __shared__ long long shi[kWavesPerBlock];
-  const double sbg = reduce_max(pmax, G, sh);
-  double bk;
-  long long ibk;
-  reduce_argmin(pmin, pmini, G, sh, shi, bk, ibk);
+  (void)pmax;
+  (void)pmin;
+  (void)pmini;
+  (void)G;
   ds_load(&sds, ds);
   DevScalars* s = &sds;
   if (threadIdx.x == 0) {
@@ -376,7 +389,7 @@ s_ls_stop(DevScalars* ds, const double* pmax, const double* pmin, const long lon
     if (s->iter != iter0 && s->status == ST_RUNNING) {
       if (s->updatd && !s->write_trial && s->stp == stp0) {
         ring_take_spare(s, s->itail);
-        logic_stop_tests(s, s->red, sbg, bk, ibk);
+        logic_stop_tests(s, s->red, s->red_sbg, s->red_bk, s->red_ibk);
       } else {
         s->status = ST_REDO;
         s->halt = 1;
@@ -404,9 +417,10 @@ void launch_publish(hipStream_t s, const DevScalars* ds, DevScalars* host, MailS
   hipLaunchKernelGGL(s_publish, dim3(1), dim3(kBlock), 0, s, ds, host, m, seq);
 }
 void launch_rowsum(hipStream_t s, const double* part, int G, int v0, int nrows, DevScalars* ds,
-                   int p4_rows, int spec) {
-  hipLaunchKernelGGL(k_rowsum, dim3(nrows), dim3(kBlock), 0, s, part, G, v0, ds, p4_rows,
-                     spec);
+                   int p4_rows, int spec, const double* pmax, const double* pmin,
+                   const long long* pmini) {
+  hipLaunchKernelGGL(k_rowsum, dim3(nrows + (pmax ? 1 : 0)), dim3(kBlock), 0, s, part, G, v0,
+                     ds, p4_rows, spec, pmax, pmin, pmini);
 }
 void launch_resume(hipStream_t s, DevScalars* ds, int reset_bp) {
   hipLaunchKernelGGL(s_resume, dim3(1), dim3(64), 0, s, ds, reset_bp);

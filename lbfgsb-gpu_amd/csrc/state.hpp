@@ -196,6 +196,10 @@ struct DevScalars {
 
   // ---- finished row sums of the last multi-block reduction (k_rowsum)
   double red[P4_COUNT + 1];
+  // pass A's projected-gradient norm and smallest breakpoint (value, index),
+  // finished by the extra block of k_rowsum
+  double red_sbg, red_bk;
+  long long red_ibk;
 };
 
 // The device state staged into LDS by a single-workgroup kernel (all threads
