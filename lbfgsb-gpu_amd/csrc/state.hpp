@@ -207,10 +207,26 @@ struct DevScalars {
 static_assert(sizeof(DevScalars) % 8 == 0, "DevScalars is copied as 8-byte words");
 constexpr int kDsWords = sizeof(DevScalars) / 8;
 
+// Every word a thread copies is loaded before its first LDS store, so the
+// loads are in flight together (sized for kBlock threads; other block sizes
+// finish in the remainder loop).
 __device__ inline void ds_load(DevScalars* sds, const DevScalars* ds) {
   const unsigned long long* src = (const unsigned long long*)ds;
   unsigned long long* dst = (unsigned long long*)sds;
-  for (int i = threadIdx.x; i < kDsWords; i += blockDim.x) dst[i] = src[i];
+  constexpr int kPer = (kDsWords + kBlock - 1) / kBlock;
+  const int nt = blockDim.x;
+  unsigned long long v[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int i = threadIdx.x + k * nt;
+    v[k] = i < kDsWords ? src[i] : 0ull;
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int i = threadIdx.x + k * nt;
+    if (i < kDsWords) dst[i] = v[k];
+  }
+  for (int i = threadIdx.x + kPer * nt; i < kDsWords; i += nt) dst[i] = src[i];
   __syncthreads();
 }
 __device__ inline void ds_store(DevScalars* ds, const DevScalars* sds) {
