@@ -167,6 +167,43 @@ __device__ inline bool memory_update_pre(DevScalars* ds) {
   return true;
 }
 
+// memory_update_pre by the whole workgroup (at least 64 threads; the caller
+// synchronises before, and it returns synchronised).  Each entry (i, j) of
+// the shift takes (i + 1, j + 1), which the serial loops read before they
+// overwrite it, so every thread reads its source before any writes.
+__device__ inline bool memory_update_block(DevScalars* ds) {
+  if (!ds->updatd) return false;
+  const int m = ds->m, col = ds->col;
+  const int t = threadIdx.x;
+  if (ds->iupdat > m) {
+    const int i = t / 8 + 1, j = t % 8 + 1;
+    const bool in = t < 64 && i <= col - 1 && j <= col - 1;
+    double vs = 0.0, vy = 0.0;
+    if (in) {
+      if (i <= j) vs = A1(ds->ss, 8, i + 1, j + 1);
+      if (i >= j) vy = A1(ds->sy, 8, i + 1, j + 1);
+    }
+    __syncthreads();
+    if (in) {
+      if (i <= j) A1(ds->ss, 8, i, j) = vs;
+      if (i >= j) A1(ds->sy, 8, i, j) = vy;
+    }
+    __syncthreads();
+  }
+  if (t < col - 1) {
+    A1(ds->sy, 8, col, t + 1) = ds->pa[PA_SY + t];
+    A1(ds->ss, 8, t + 1, col) = ds->pa[PA_SS + t];
+  }
+  if (t == 0) {
+    ds->rr = ds->pa[PA_RR];
+    ds->theta = ds->rr / ds->dr;
+    A1(ds->ss, 8, col, col) = ds->stp == 1.0 ? ds->dtd : ds->stp * ds->stp * ds->dtd;
+    A1(ds->sy, 8, col, col) = ds->dr;
+  }
+  __syncthreads();
+  return true;
+}
+
 // formt (lbfgsbcpu.cpp:1105-1133) by the whole workgroup: the entries of
 // T = theta S'S + L D^{-1} L' one per thread (each summed in the reference
 // order), then the Cholesky factor by thread 0 in registers.  A failed
@@ -555,11 +592,7 @@ __device__ inline void post_update_middle(DevScalars* s) {
   }
   __shared__ double s_e[16][17], s_out[16][17];
   __syncthreads();
-  if (mode == 1) {
-    if (threadIdx.x == 0) s->blk_flag = memory_update_pre(s) ? 1 : 0;
-    __syncthreads();
-    if (s->blk_flag) formt_block(s);
-  }
+  if (mode == 1 && memory_update_block(s)) formt_block(s);
   const int col = s->col;
   const int k = (int)threadIdx.x - kWave;
   if (threadIdx.x == 0) {
