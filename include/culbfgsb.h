@@ -27,6 +27,9 @@
 
 enum LBFGSB_CUDA_MODE { LCM_NO_ACCELERATION, LCM_CUDA };
 
+// machine_epsilon, machine_maximum and step_scaling are kept for source
+// compatibility and ignored: the update-skip test uses the CPU path's fixed
+// 5e-16 in both precisions (culbfgsb/lbfgsbcpu.cpp:443, culbfgsb/ap.cpp:38).
 template <typename real>
 struct LBFGSB_CUDA_OPTION {
   real machine_epsilon;

@@ -24,7 +24,12 @@ extern "C" {
  * is provided by this library; the CPU mode returns info = -1. */
 enum { LBFGSB_AMD_MODE_CPU = 0, LBFGSB_AMD_MODE_GPU = 1 };
 
-/* LBFGSB_CUDA_OPTION<real> (culbfgsb/culbfgsb.h:33-44), same field order. */
+/* LBFGSB_CUDA_OPTION<real> (culbfgsb/culbfgsb.h:33-44), same field order.
+ * machine_epsilon, machine_maximum and step_scaling are accepted and
+ * ignored: the solver follows the reference's CPU path, whose update-skip
+ * test, formt and formk use the fixed machine epsilon 5e-16 for both
+ * precisions (culbfgsb/lbfgsbcpu.cpp:443, culbfgsb/ap.cpp:38), not the
+ * option the reference's CUDA path reads (culbfgsb/lbfgsbcuda.cpp:352). */
 typedef struct {
   double machine_epsilon, machine_maximum, step_scaling, eps_g, eps_f, eps_x;
   int hessian_approximate_dimension, max_iteration, mode;
@@ -114,7 +119,9 @@ int lbfgsb_engine_run(lbfgsb_engine* e, int program, int start_step, int stop_st
 /* Device pointers of the accepted (which = 0) or pending trial (1) x / g. */
 void* lbfgsb_engine_x(lbfgsb_engine* e, int which);
 void* lbfgsb_engine_g(lbfgsb_engine* e, int which);
-/* Named scalar of the device state (see DevScalars), as double. */
+/* Named scalar of the device state (see DevScalars), as double; also
+ * "max_sorted_round", the most breakpoint candidates one host-sized round
+ * of the Cauchy search sorted so far (a statistic of the engine). */
 int lbfgsb_engine_get_scalar(lbfgsb_engine* e, const char* name, double* value);
 int lbfgsb_engine_set_scalar(lbfgsb_engine* e, const char* name, double value);
 /* Named array copied to / from host memory.  Names: x, g, xb, gb, dsub
@@ -155,7 +162,10 @@ int lbfgsb_engine_step_time(lbfgsb_engine* e, int step, double* total_ms, long l
  * reference library -- they restate its example callback shape,
  * examples/dsscfg/dsscfg_cuda.cu:43-154, for the benchmark workloads).
  * kind: 1 grid quadratic, 2 grid cos-energy, 3 extended Rosenbrock,
- * 4 pentadiagonal QP.  Arrays are device arrays of the bulk type. */
+ * 4 pentadiagonal QP, 6 the reference example's MINPACK-2 dsscfg energy
+ * (examples/dsscfg/dsscfg_cpu.cpp:17-150) on an nx x ny grid with h = lambda
+ * and no arrays.  Arrays are device arrays of the bulk type.  Returns NULL
+ * on a bad kind or shape (grids need nx*ny = n, Rosenbrock an even n). */
 typedef struct lbfgsb_problem lbfgsb_problem;
 lbfgsb_problem* lbfgsb_problem_create(int dtype, int kind, long long n, long long nx,
                                       long long ny, double h, const void* b, const void* c,

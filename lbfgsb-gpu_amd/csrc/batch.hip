@@ -465,7 +465,7 @@ __device__ __forceinline__ void b_trial(const Prob<T>& P, BatchLds& L) {
   const double stp = L.s.stp;
   const bool trial_d = L.s.trial_d;
   P5bOp<T> sub;
-  sub.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.dsub);
+  sub.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.dsub, L.s.trial_src);
   for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
     trial_element(i, &L.s, sub, xa, xb, P.d, stp, trial_d);
   __syncthreads();

@@ -399,7 +399,7 @@ k_trial(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l, const T* __
   const double stp = ds->stp;
   const bool trial_d = ds->trial_d;
   P5bOp<T> sub;
-  sub.init(ds, X, Gv, l, u, iwhere, dsub);
+  sub.init(ds, X, Gv, l, u, iwhere, dsub, ds->trial_src);
   for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
        i += (long long)gridDim.x * kBlock)
     trial_element(i, ds, sub, xa, xb, d, stp, trial_d);

@@ -315,6 +315,10 @@ void* lbfgsb_engine_g(lbfgsb_engine* e, int which) {
 }
 
 int lbfgsb_engine_get_scalar(lbfgsb_engine* e, const char* name, double* value) {
+  if (std::strcmp(name, "max_sorted_round") == 0) {
+    *value = (double)e->e->max_sorted_round;
+    return 0;
+  }
   const ScalarField* f = find_field(name);
   if (!f) return -2;
   if (e->e->read_scalars()) return -1;
@@ -382,8 +386,11 @@ int lbfgsb_engine_step_time(lbfgsb_engine* e, int step, double* total_ms, long l
 lbfgsb_problem* lbfgsb_problem_create(int dtype, int kind, long long n, long long nx,
                                       long long ny, double h, const void* b, const void* c,
                                       const void* a0, const void* a1, const void* a2) {
-  if (kind < OBJK_GRID_QUAD || kind > OBJK_PENTA || n <= 0) return nullptr;
-  if ((kind == OBJK_GRID_QUAD || kind == OBJK_GRID_COS) && nx * ny != n) return nullptr;
+  if (n <= 0 || !((kind >= OBJK_GRID_QUAD && kind <= OBJK_PENTA) || kind == OBJK_DSSCFG))
+    return nullptr;
+  if ((kind == OBJK_GRID_QUAD || kind == OBJK_GRID_COS || kind == OBJK_DSSCFG) &&
+      (nx <= 0 || ny <= 0 || nx * ny != n))
+    return nullptr;
   if (kind == OBJK_ROSEN && n % 2) return nullptr;
   lbfgsb_problem* p = new lbfgsb_problem();
   p->dtype = dtype;

@@ -380,6 +380,7 @@ struct Engine final : EngineBase {
         continue;  // size the round with the count just selected
       }
       const long long K = (long long)hs->cand_count;
+      max_sorted_round = std::max(max_sorted_round, K);
       launch_gcp_sortscan<T>(st, dG, W, ds, ca, cb, K, w);
       HIPCHK(hipGetLastError());
       if (read_scalars()) return -1;

@@ -35,6 +35,9 @@ struct EngineBase {
   virtual void set_profiling(int from_iter, unsigned step_mask) = 0;  // -1 = off
   virtual void step_time(int step, double* ms, long long* count) = 0;
   hipError_t last_error = hipSuccess;
+  // statistics of the host-sized breakpoint rounds: the most candidates one
+  // round sorted (tiles of kSortTile, then one merge-path pass per doubling)
+  long long max_sorted_round = 0;
 };
 
 EngineBase* make_engine(int dtype, long long n, int m, void* x, const int* nbd, const void* l,

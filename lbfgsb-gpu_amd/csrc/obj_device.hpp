@@ -4,6 +4,9 @@
 //   GRID_COS   f = 0.5 sum_edges (dv)^2 + h^2 sum (c cos v - b v)
 //   ROSEN      f = sum_i 100 (x_{2i} - x_{2i-1}^2)^2 + (1 - x_{2i-1})^2
 //   PENTA      f = 0.5 x'Ax - b'x, A symmetric pentadiagonal
+//   DSSCFG     MINPACK-2 steady-state combustion on nx x ny points, two
+//              linear triangles per point, zero outside the grid
+//              (examples/dsscfg/dsscfg_cpu.cpp:17-150), lambda in o.h
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -47,6 +50,36 @@ __device__ inline double obj_term(const ObjDesc& o, long long n, const T* __rest
     }
     g[k] = (T)gk;
     return fk;
+  }
+  if (o.kind == OBJK_DSSCFG) {
+    // Point k owns its lower triangle (v, right, top) and its upper one
+    // (v, left, bottom).  Its gradient gathers the five contributions the
+    // reference example scatters (its own, then from the bottom, left,
+    // right and top neighbours' triangles), in that order.
+    const long long nx = o.nx, ny = o.ny;
+    const long long ix = k % nx, iy = k / nx;
+    const double lam = o.h;
+    const double hx = 1.0 / (double)(nx + 1), hy = 1.0 / (double)(ny + 1);
+    const double area = 0.5 * hx * hy;
+    const double v = (double)x[k];
+    const double vr = ix != nx - 1 ? (double)x[k + 1] : 0.0;
+    const double vt = iy != ny - 1 ? (double)x[k + nx] : 0.0;
+    const double vl = ix != 0 ? (double)x[k - 1] : 0.0;
+    const double vb = iy != 0 ? (double)x[k - nx] : 0.0;
+    const double dxl = (vr - v) / hx, dyl = (vt - v) / hy;
+    const double dxu = (v - vl) / hx, dyu = (v - vb) / hy;
+    const double ev = exp(v), ler = lam * ev / 3.0;
+    double own = 0.0 + (-dxl / hx - dyl / hy - ler);
+    own = own + (dxu / hx + dyu / hy - ler);
+    double gk = own;
+    gk = gk + (iy != ny - 1 ? -dyl / hy - ler : 0.0);
+    gk = gk + (ix != nx - 1 ? -dxl / hx - ler : 0.0);
+    gk = gk + (ix != 0 ? dxu / hx - ler : 0.0);
+    gk = gk + (iy != 0 ? dyu / hy - ler : 0.0);
+    g[k] = (T)(gk * area);
+    const double fq = dxl * dxl + dyl * dyl + (dxu * dxu + dyu * dyu);
+    const double fe = lam * (ev + exp(vr) + exp(vt)) / 3.0 + lam * (exp(vb) + exp(vl) + ev) / 3.0;
+    return area * (0.5 * fq - fe);
   }
   if (o.kind == OBJK_ROSEN) {
     // the pair (x_{2k}, x_{2k+1}) as one 2-vector load and store: a wave

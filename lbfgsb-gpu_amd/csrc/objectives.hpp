@@ -7,7 +7,13 @@
 
 namespace lbfgsb {
 
-enum ObjKind : int { OBJK_GRID_QUAD = 1, OBJK_GRID_COS = 2, OBJK_ROSEN = 3, OBJK_PENTA = 4 };
+enum ObjKind : int {
+  OBJK_GRID_QUAD = 1,
+  OBJK_GRID_COS = 2,
+  OBJK_ROSEN = 3,
+  OBJK_PENTA = 4,
+  OBJK_DSSCFG = 6  // the reference example's problem; h holds lambda
+};
 
 struct ObjDesc {
   int kind;

@@ -565,10 +565,14 @@ struct P5bOp {
     uint8_t iwb;
   };
 
+  // cur_buf: the buffer pair the subspace step was formed from; by default
+  // the accepted one (phase & 1).  The trial pass names it (trial_src),
+  // because a step that the line search accepts without evaluating it
+  // (iback >= 20) flips the phase before that pass runs.
   __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
                               const T* u_, const uint8_t* iwhere_,
-                              const T* dsub_) {
-    const int cur = ds->phase & 1;
+                              const T* dsub_, int cur_buf = -1) {
+    const int cur = cur_buf >= 0 ? cur_buf : (ds->phase & 1);
     x = X[cur];
     g = Gv[cur];
     l = l_;

@@ -21,6 +21,11 @@ Workloads (n, bounds and objectives):
 * ``penta_qp`` -- f = 0.5 x'Ax - b'x, A symmetric pentadiagonal with
   diagonal U(4,8) and both off-diagonals U(-1,1)*0.9 (strictly diagonally
   dominant, hence SPD), b ~ N(0,1), l ~ U(-1,-0.1), u ~ U(0.1,1), x0 = 0.
+* ``dsscfg`` -- the reference's example problem (MINPACK-2 steady-state
+  combustion, examples/dsscfg/dsscfg_cpu.cpp): piecewise-linear elements on
+  two triangles per grid point, hx = 1/(nx+1), hy = 1/(ny+1),
+  f = area*(0.5*sum|grad v|^2 - lambda*sum trapezoid(exp v)), unconstrained
+  (nbd = 0), the reference's standard start ('XS').
 """
 from __future__ import annotations
 
@@ -40,6 +45,7 @@ STREAM_A, STREAM_N, STREAM_ARRIVAL, STREAM_MIX = 6, 7, 8, 9
 
 # objective kinds shared with the C-ABI (include/lbfgsb_amd.h)
 OBJ_GRID_QUAD, OBJ_GRID_COS, OBJ_ROSEN, OBJ_PENTA = 1, 2, 3, 4
+OBJ_DSSCFG = 6
 
 
 def splitmix64(z: np.ndarray) -> np.ndarray:
@@ -144,6 +150,24 @@ def penta_qp(n: int, seed: int = SEED_BASE + 3) -> Problem:
         b=normal(seed, STREAM_B, n), a0=a0, a1=a1, a2=a2, name=f"penta_qp_{n}")
 
 
+def dsscfg(nx: int, ny: int | None = None, lam: float = 1.0) -> Problem:
+    """The reference example's problem: nx*ny unknowns, no bounds, starting
+    point x_k = lambda/(lambda+1) * sqrt(min(min(i, nx-i)*hx,
+    min(j, ny-j)*hy)) for k = nx*j + i (examples/dsscfg/dsscfg_cpu.cpp:25-37).
+    The lambda of the objective travels in h."""
+    ny = nx if ny is None else ny
+    n = nx * ny
+    hx, hy = 1.0 / (nx + 1), 1.0 / (ny + 1)
+    i = np.arange(nx)
+    j = np.arange(ny)
+    tx = np.minimum(i, nx - i) * hx
+    ty = np.minimum(j, ny - j) * hy
+    x0 = (lam / (lam + 1.0)) * np.sqrt(np.minimum(tx[None, :], ty[:, None]))
+    return Problem(kind=OBJ_DSSCFG, n=n, nx=nx, ny=ny, h=lam, x0=x0.reshape(-1),
+                   nbd=np.zeros(n, dtype=np.int32), l=np.zeros(n), u=np.zeros(n),
+                   name=f"dsscfg_{nx}x{ny}")
+
+
 def batch_sizes(count: int, lo: float, hi: float, seed: int) -> np.ndarray:
     """n drawn log-uniform in [lo, hi] (floor), one per problem."""
     u = uniform(seed, STREAM_N, count)
@@ -180,7 +204,37 @@ def objective_numpy(p: Problem, x: np.ndarray):
     if p.kind == OBJ_PENTA:
         ax = penta_matvec(p, x)
         return float(0.5 * x @ ax - p.b @ x), ax - p.b
+    if p.kind == OBJ_DSSCFG:
+        return dsscfg_numpy(p, x)
     raise ValueError(p.kind)
+
+
+def dsscfg_numpy(p: Problem, x: np.ndarray):
+    """f and g of the dsscfg energy by automatic summation over the two
+    triangles of every grid point (the element form, independent of the
+    gather order the kernels use)."""
+    nx, ny, lam = p.nx, p.ny, p.h
+    hx, hy = 1.0 / (nx + 1), 1.0 / (ny + 1)
+    area = 0.5 * hx * hy
+    pad = np.zeros((ny + 2, nx + 2))
+    pad[1:-1, 1:-1] = np.asarray(x, dtype=np.float64).reshape(ny, nx)
+    v = pad[1:-1, 1:-1]
+    vr, vt = pad[1:-1, 2:], pad[2:, 1:-1]
+    vl, vb = pad[1:-1, :-2], pad[:-2, 1:-1]
+    dxl, dyl = (vr - v) / hx, (vt - v) / hy
+    dxu, dyu = (v - vl) / hx, (v - vb) / hy
+    f = area * (0.5 * np.sum(dxl ** 2 + dyl ** 2 + dxu ** 2 + dyu ** 2)
+                - lam * np.sum(np.exp(v) + np.exp(vr) + np.exp(vt)
+                               + np.exp(vb) + np.exp(vl) + np.exp(v)) / 3.0)
+    # gradient of each triangle's terms, scattered onto the padded grid
+    gp = np.zeros_like(pad)
+    c = slice(1, -1)
+    gp[c, c] += -dxl / hx - dyl / hy + dxu / hx + dyu / hy - 2 * lam * np.exp(v) / 3.0
+    gp[c, 2:] += dxl / hx - lam * np.exp(vr) / 3.0
+    gp[2:, c] += dyl / hy - lam * np.exp(vt) / 3.0
+    gp[c, :-2] += -dxu / hx - lam * np.exp(vl) / 3.0
+    gp[:-2, c] += -dyu / hy - lam * np.exp(vb) / 3.0
+    return float(f), (area * gp[c, c]).reshape(-1)
 
 
 def penta_matvec(p: Problem, x: np.ndarray) -> np.ndarray:

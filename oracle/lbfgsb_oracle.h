@@ -37,7 +37,16 @@ enum orc_objective_kind {
   ORC_OBJ_GRID_COS = 2,  /* 0.5*sum_edges (dv)^2 + h^2 sum(c cos v - b v) */
   ORC_OBJ_ROSEN = 3,     /* extended Rosenbrock over pairs */
   ORC_OBJ_PENTA = 4,     /* 0.5 x'Ax - b'x, A symmetric pentadiagonal */
-  ORC_OBJ_USER = 5       /* caller-supplied function pointer */
+  ORC_OBJ_USER = 5,      /* caller-supplied function pointer */
+  ORC_OBJ_DSSCFG = 6     /* MINPACK-2 steady-state combustion (the
+                            reference's example problem); h holds lambda */
+};
+
+/* Outcome of the part of an iteration after the line search. */
+enum orc_tail_result {
+  ORC_TAIL_NEXT = 0,    /* iteration done, the next one follows */
+  ORC_TAIL_RESTART = 1, /* memory reset: the iteration starts again */
+  ORC_TAIL_END = 2      /* solve ended, info holds the reason */
 };
 
 #define ORC_DEFINE_TYPES(REAL, SFX)                                            \
@@ -54,10 +63,14 @@ enum orc_objective_kind {
     orc_fg_##SFX user;                                                         \
     void* user_ctx;                                                            \
   } orc_objective_##SFX;                                                       \
+  /* stop_at_iter > 0 stands for a customised stopping callback that       \
+   * returns non-zero once iter >= stop_at_iter (culbfgsb.h:64-66). */         \
   typedef struct {                                                             \
     double eps_g, eps_f, eps_x;                                                \
     int m;                                                                     \
     int max_iteration;                                                         \
+    int stop_at_iter;                                                          \
+    int pad_;                                                                  \
   } orc_options_##SFX;                                                         \
   /* Per-iteration record written by orc_iterate (one per outer iteration). */ \
   typedef struct {                                                             \
@@ -101,6 +114,10 @@ ORC_DEFINE_TYPES(float, f32)
                       const orc_options_##SFX* opt,                            \
                       const orc_objective_##SFX* obj);                         \
   int orc_iterate_##SFX(orc_state_##SFX* s, orc_record_##SFX* rec);           \
+  /* the three parts of orc_iterate, for stepping tests */                     \
+  void orc_iter_head_##SFX(orc_state_##SFX* s);                                \
+  void orc_iter_search_##SFX(orc_state_##SFX* s);                              \
+  int orc_iter_tail_##SFX(orc_state_##SFX* s);                                 \
   int orc_minimize_##SFX(int n, const orc_options_##SFX* opt,                  \
                          const orc_objective_##SFX* obj, REAL* x,              \
                          const int* nbd, const REAL* l, const REAL* u,         \

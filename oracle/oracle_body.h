@@ -490,6 +490,48 @@ int S(orc_evaluate)(const S(orc_objective)* o, int n, const R* x, R* f,
       }
       break;
     }
+    case ORC_OBJ_DSSCFG: {
+      /* MINPACK-2 dsscfg as the reference's example evaluates it
+       * (examples/dsscfg/dsscfg_cpu.cpp:17-150): per grid point a lower
+       * triangle (v, right, top) and an upper one (v, left, bottom), zero
+       * outside the grid, trapezoidal rule for the exponential.  The
+       * gradient is gathered per point from the five contributions the
+       * reference scatters (its own, then bottom, left, right, top arrays),
+       * in the reference's order of operations; in double. */
+      int nx = o->nx, ny = o->ny, ix, iy;
+      double lam = o->h;
+      double hx = 1.0 / (double)(nx + 1), hy = 1.0 / (double)(ny + 1);
+      double area = 0.5 * hx * hy;
+      double fquad = 0.0, fexp = 0.0;
+      for (iy = 0; iy < ny; iy++) {
+        for (ix = 0; ix < nx; ix++) {
+          int k = iy * nx + ix;
+          double v = (double)x[k];
+          double vr = ix != nx - 1 ? (double)x[k + 1] : 0.0;
+          double vt = iy != ny - 1 ? (double)x[k + nx] : 0.0;
+          double vl = ix != 0 ? (double)x[k - 1] : 0.0;
+          double vb = iy != 0 ? (double)x[k - nx] : 0.0;
+          double dxl = (vr - v) / hx, dyl = (vt - v) / hy;
+          double dxu = (v - vl) / hx, dyu = (v - vb) / hy;
+          double ev = exp(v), ler = lam * ev / 3.0;
+          double own, gk;
+          fquad += dxl * dxl + dyl * dyl;
+          fexp = fexp - lam * (ev + exp(vr) + exp(vt)) / 3.0;
+          fquad += dxu * dxu + dyu * dyu;
+          fexp = fexp - lam * (exp(vb) + exp(vl) + ev) / 3.0;
+          own = 0.0 + (-dxl / hx - dyl / hy - ler);
+          own = own + (dxu / hx + dyu / hy - ler);
+          gk = own;
+          gk = gk + (iy != ny - 1 ? -dyl / hy - ler : 0.0); /* bottom */
+          gk = gk + (ix != nx - 1 ? -dxl / hx - ler : 0.0); /* left */
+          gk = gk + (ix != 0 ? dxu / hx - ler : 0.0);       /* right */
+          gk = gk + (iy != 0 ? dyu / hy - ler : 0.0);       /* top */
+          g[k] = (R)(gk * area);
+        }
+      }
+      acc = area * (0.5 * fquad + fexp);
+      break;
+    }
     case ORC_OBJ_USER:
       return o->user(o->user_ctx, x, f, g);
     default:
@@ -1329,12 +1371,12 @@ int S(orc_setup)(S(orc_state)* s, const R* x0, const int* nbd, const R* l,
   return 1;
 }
 
-/* One outer iteration (lbfgsbcpu.cpp:279-462), including the memory resets
- * that restart it.  Returns 1 to continue, 0 when the solve has ended. */
-int S(orc_iterate)(S(orc_state)* s, S(orc_record)* rec) {
+/* Head of an outer iteration (lbfgsbcpu.cpp:280-345): Cauchy point, free
+ * set, formk, cmprlb and subsm, each failure resetting the memory and
+ * starting the head again (:297-305, 317-325, 333-341); ends with
+ * d = z - x and task = 0. */
+void S(orc_iter_head)(S(orc_state)* s) {
   int n = s->n, i;
-  const S(orc_options)* opt = &s->opt;
-  int restarts0 = s->restarts;
   for (;;) {
     s->iword = -1;
     if (!s->cnstnd && s->col > 0) {
@@ -1366,98 +1408,118 @@ int S(orc_iterate)(S(orc_state)* s, S(orc_record)* rec) {
     }
     for (i = 0; i < n; i++) s->d[i] = s->z[i] - s->x[i];
     s->task = 0;
-    for (;;) {
-      S(orc_lnsrlb)(s);
-      if (s->internalinfo != 0 || s->iback >= 20 || s->task != 1) break;
-      S(orc_eval_state)(s);
-    }
-    if (s->internalinfo != 0) {
-      memcpy(s->x, s->t, (size_t)n * sizeof(R));
-      memcpy(s->g, s->r, (size_t)n * sizeof(R));
-      s->f = s->fold;
-      if (s->col == 0) {
-        s->task = 2;
-        s->iter = s->iter + 1;
-        s->info = -2;
-        goto record_end;
-      }
-      if (s->internalinfo == 0) s->nfgv = s->nfgv - 1;
-      S(orc_reset_memory)(s);
-      continue;
-    }
-    s->iter = s->iter + 1;
-    S(orc_projgr)(s);
-    s->residual_g = s->sbgnrm;
-    if (s->sbgnrm <= (R)opt->eps_g) {
-      s->info = 4;
-      goto record_end;
-    }
-    for (i = 0; i < n; i++) s->xdiff[i] = s->xold[i] - s->x[i];
-    s->tf = S(orc_dot)(s->xdiff, 1, s->xdiff, 1, n);
-    s->tf = RSQRT(s->tf);
-    s->residual_x = s->tf;
-    if (s->tf <= (R)opt->eps_x) {
-      s->info = 2;
-      goto record_end;
-    }
-    s->ddum = S(rmax)(RFABS(s->fold), S(rmax)(RFABS(s->f), (R)1));
-    s->residual_f = (s->fold - s->f) / s->ddum;
-    if (s->fold - s->f <= (R)opt->eps_f * s->ddum) {
-      s->info = 1;
-      goto record_end;
-    }
-    if (s->iter > opt->max_iteration && opt->max_iteration > 0) {
-      s->info = 5;
-      goto record_end;
-    }
-    memcpy(s->xold, s->x, (size_t)n * sizeof(R));
-    for (i = 0; i < n; i++) s->r[i] = s->g[i] - s->r[i];
-    s->rr = S(orc_dot)(s->r, 1, s->r, 1, n);
-    if (s->stp == 1) {
-      s->dr = s->gd - s->gdold;
-      s->ddum = -s->gdold;
-    } else {
-      s->dr = (s->gd - s->gdold) * s->stp;
-      for (i = 0; i < n; i++) s->d[i] *= s->stp;
-      s->ddum = -s->gdold * s->stp;
-    }
-    if ((double)s->dr <= ORC_MACHEPS * (double)s->ddum) {
-      s->nskip = s->nskip + 1;
-      s->updatd = 0;
-    } else {
-      s->updatd = 1;
-      s->iupdat = s->iupdat + 1;
-      S(orc_matupd)(s);
-      S(orc_formt)(s);
-      if (s->internalinfo != 0) {
-        S(orc_reset_memory)(s);
-        /* the reference restarts the loop here; the next call continues */
-      }
-    }
-    if (rec) {
-      rec->f = s->f; rec->sbgnrm = s->sbgnrm; rec->stp = s->stp;
-      rec->theta = s->theta; rec->dtd = s->dtd; rec->gd = s->gd;
-      rec->residual_f = s->residual_f; rec->residual_x = s->residual_x;
-      rec->iter = s->iter; rec->nint = s->nint; rec->nfree = s->nfree;
-      rec->nenter = s->nenter; rec->ileave = s->ileave; rec->wrk = s->wrk;
-      rec->iword = s->iword; rec->iback = s->iback; rec->nfgv = s->nfgv;
-      rec->col = s->col; rec->updatd = s->updatd;
-      rec->restarts = s->restarts - restarts0; rec->info = 0;
-    }
-    return 1;
+    return;
   }
-record_end:
-  if (rec) {
-    rec->f = s->f; rec->sbgnrm = s->sbgnrm; rec->stp = s->stp;
-    rec->theta = s->theta; rec->dtd = s->dtd; rec->gd = s->gd;
-    rec->residual_f = s->residual_f; rec->residual_x = s->residual_x;
-    rec->iter = s->iter; rec->nint = s->nint; rec->nfree = s->nfree;
-    rec->nenter = s->nenter; rec->ileave = s->ileave; rec->wrk = s->wrk;
-    rec->iword = s->iword; rec->iback = s->iback; rec->nfgv = s->nfgv;
-    rec->col = s->col; rec->updatd = s->updatd;
-    rec->restarts = s->restarts - restarts0; rec->info = s->info;
+}
+
+/* Line-search loop (lbfgsbcpu.cpp:346-358): lnsrlb and an evaluation per
+ * trial, left after an error, 20 backtracks or a finished search. */
+void S(orc_iter_search)(S(orc_state)* s) {
+  for (;;) {
+    S(orc_lnsrlb)(s);
+    if (s->internalinfo != 0 || s->iback >= 20 || s->task != 1) break;
+    S(orc_eval_state)(s);
   }
-  return 0;
+}
+
+/* The rest of an iteration (lbfgsbcpu.cpp:359-460): line-search failure,
+ * stopping tests in the reference's order (projected gradient, step,
+ * decrease, iteration count, customised test) and the memory update. */
+int S(orc_iter_tail)(S(orc_state)* s) {
+  int n = s->n, i;
+  const S(orc_options)* opt = &s->opt;
+  if (s->internalinfo != 0) {
+    memcpy(s->x, s->t, (size_t)n * sizeof(R));
+    memcpy(s->g, s->r, (size_t)n * sizeof(R));
+    s->f = s->fold;
+    if (s->col == 0) {
+      s->task = 2;
+      s->iter = s->iter + 1;
+      s->info = -2;
+      return ORC_TAIL_END;
+    }
+    S(orc_reset_memory)(s);
+    return ORC_TAIL_RESTART;
+  }
+  s->iter = s->iter + 1;
+  S(orc_projgr)(s);
+  s->residual_g = s->sbgnrm;
+  if (s->sbgnrm <= (R)opt->eps_g) {
+    s->info = 4;
+    return ORC_TAIL_END;
+  }
+  for (i = 0; i < n; i++) s->xdiff[i] = s->xold[i] - s->x[i];
+  s->tf = S(orc_dot)(s->xdiff, 1, s->xdiff, 1, n);
+  s->tf = RSQRT(s->tf);
+  s->residual_x = s->tf;
+  if (s->tf <= (R)opt->eps_x) {
+    s->info = 2;
+    return ORC_TAIL_END;
+  }
+  s->ddum = S(rmax)(RFABS(s->fold), S(rmax)(RFABS(s->f), (R)1));
+  s->residual_f = (s->fold - s->f) / s->ddum;
+  if (s->fold - s->f <= (R)opt->eps_f * s->ddum) {
+    s->info = 1;
+    return ORC_TAIL_END;
+  }
+  if (s->iter > opt->max_iteration && opt->max_iteration > 0) {
+    s->info = 5;
+    return ORC_TAIL_END;
+  }
+  if (opt->stop_at_iter > 0 && s->iter >= opt->stop_at_iter) {
+    s->info = 0;
+    return ORC_TAIL_END;
+  }
+  memcpy(s->xold, s->x, (size_t)n * sizeof(R));
+  for (i = 0; i < n; i++) s->r[i] = s->g[i] - s->r[i];
+  s->rr = S(orc_dot)(s->r, 1, s->r, 1, n);
+  if (s->stp == 1) {
+    s->dr = s->gd - s->gdold;
+    s->ddum = -s->gdold;
+  } else {
+    s->dr = (s->gd - s->gdold) * s->stp;
+    for (i = 0; i < n; i++) s->d[i] *= s->stp;
+    s->ddum = -s->gdold * s->stp;
+  }
+  if ((double)s->dr <= ORC_MACHEPS * (double)s->ddum) {
+    s->nskip = s->nskip + 1;
+    s->updatd = 0;
+  } else {
+    s->updatd = 1;
+    s->iupdat = s->iupdat + 1;
+    S(orc_matupd)(s);
+    S(orc_formt)(s);
+    /* a failed factorisation discards the memory; the reference's loop
+     * then goes on with the next iteration (:450-458) */
+    if (s->internalinfo != 0) S(orc_reset_memory)(s);
+  }
+  return ORC_TAIL_NEXT;
+}
+
+static void S(orc_fill_record)(const S(orc_state)* s, S(orc_record)* rec,
+                               int restarts0, int info) {
+  rec->f = s->f; rec->sbgnrm = s->sbgnrm; rec->stp = s->stp;
+  rec->theta = s->theta; rec->dtd = s->dtd; rec->gd = s->gd;
+  rec->residual_f = s->residual_f; rec->residual_x = s->residual_x;
+  rec->iter = s->iter; rec->nint = s->nint; rec->nfree = s->nfree;
+  rec->nenter = s->nenter; rec->ileave = s->ileave; rec->wrk = s->wrk;
+  rec->iword = s->iword; rec->iback = s->iback; rec->nfgv = s->nfgv;
+  rec->col = s->col; rec->updatd = s->updatd;
+  rec->restarts = s->restarts - restarts0; rec->info = info;
+}
+
+/* One outer iteration (lbfgsbcpu.cpp:279-462), including the memory resets
+ * that restart it.  Returns 1 to continue, 0 when the solve has ended. */
+int S(orc_iterate)(S(orc_state)* s, S(orc_record)* rec) {
+  int restarts0 = s->restarts, r;
+  for (;;) {
+    S(orc_iter_head)(s);
+    S(orc_iter_search)(s);
+    r = S(orc_iter_tail)(s);
+    if (r != ORC_TAIL_RESTART) break;
+  }
+  if (rec) S(orc_fill_record)(s, rec, restarts0, r == ORC_TAIL_END ? s->info : 0);
+  return r == ORC_TAIL_NEXT;
 }
 
 /* Whole solve: lbfgsbminimize in LCM_NO_ACCELERATION mode

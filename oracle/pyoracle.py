@@ -16,7 +16,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblbfgsb_oracle.so")
 
-OBJ_NONE, OBJ_GRID_QUAD, OBJ_GRID_COS, OBJ_ROSEN, OBJ_PENTA, OBJ_USER = range(6)
+OBJ_NONE, OBJ_GRID_QUAD, OBJ_GRID_COS, OBJ_ROSEN, OBJ_PENTA, OBJ_USER, OBJ_DSSCFG = range(7)
+TAIL_NEXT, TAIL_RESTART, TAIL_END = range(3)
 
 
 def build(force: bool = False) -> str:
@@ -38,7 +39,8 @@ def _types(real):
     class Options(C.Structure):
         _fields_ = [("eps_g", C.c_double), ("eps_f", C.c_double),
                     ("eps_x", C.c_double), ("m", C.c_int),
-                    ("max_iteration", C.c_int)]
+                    ("max_iteration", C.c_int), ("stop_at_iter", C.c_int),
+                    ("pad_", C.c_int)]
 
     class Record(C.Structure):
         _fields_ = [(k, C.c_double) for k in
@@ -96,6 +98,10 @@ class _Flavor:
         self.setup = fn("orc_setup", C.c_int, PS, P, PI, P, P,
                         C.POINTER(self.Options), C.POINTER(self.Objective))
         self.iterate = fn("orc_iterate", C.c_int, PS, C.POINTER(self.Record))
+        self.iter_head = fn("orc_iter_head", None, PS)
+        self.iter_search = fn("orc_iter_search", None, PS)
+        self.iter_tail = fn("orc_iter_tail", C.c_int, PS)
+        self.lnsrlb_step = fn("orc_lnsrlb_step", C.c_int, PS)
         self.minimize = fn("orc_minimize", C.c_int, C.c_int,
                            C.POINTER(self.Options), C.POINTER(self.Objective),
                            P, PI, P, P, C.POINTER(self.Record), C.c_int,
@@ -168,14 +174,15 @@ class Solver:
     """Stepping view of one oracle solve."""
 
     def __init__(self, problem, dtype=np.float64, m=8, eps=1e-8, max_iter=1000,
-                 eps_g=None, eps_f=None, eps_x=None):
+                 eps_g=None, eps_f=None, eps_x=None, stop_at_iter=0):
         self.fl = fl = flavor(dtype)
         self.p = problem.cast(fl.np)
         self.n = n = problem.n
         self.obj = Objective(fl, self.p)
         self.opt = fl.Options(eps if eps_g is None else eps_g,
                               eps if eps_f is None else eps_f,
-                              eps if eps_x is None else eps_x, m, max_iter)
+                              eps if eps_x is None else eps_x, m, max_iter,
+                              stop_at_iter, 0)
         self.s = fl.create(n, m)
         self.more = bool(fl.setup(self.s, fl.ptr(self.p.x0), _iptr(self.p.nbd),
                                   fl.ptr(self.p.l), fl.ptr(self.p.u),
@@ -222,11 +229,12 @@ class Solver:
 
 
 def minimize(problem, dtype=np.float64, m=8, eps=1e-8, max_iter=1000,
-             max_records=100000):
+             max_records=100000, eps_g=None, eps_f=None, eps_x=None, stop_at_iter=0):
     fl = flavor(dtype)
     p = problem.cast(fl.np)
     obj = Objective(fl, p)
-    opt = fl.Options(eps, eps, eps, m, max_iter)
+    opt = fl.Options(eps if eps_g is None else eps_g, eps if eps_f is None else eps_f,
+                     eps if eps_x is None else eps_x, m, max_iter, stop_at_iter, 0)
     x = p.x0.copy()
     recs = (fl.Record * max_records)()
     iters = C.c_int(0)
