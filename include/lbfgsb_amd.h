@@ -131,7 +131,8 @@ int lbfgsb_engine_set_scalar(lbfgsb_engine* e, const char* name, double value);
  * iwhere (int32, n); iwbits (uint8, n: bits 0-3 iwhere + 4, bit 4 free in the
  * last partition, bit 5 free in the one before, bits 6-7 nbd); ws, wy (bulk type, n x m
  * row-major); index, indx2 (int32, n, 1-based as the reference); sy, ss,
- * wt (m x m), wn, snd (2m x 2m), p, c, mc, wv, pa (double). */
+ * wt (m x m), wn, snd (2m x 2m), p, c, mc, wv, pa (double); dsave (13) and
+ * isave (2, as double): the line search's saved state. */
 int lbfgsb_engine_get_array(lbfgsb_engine* e, const char* name, void* out, long long count);
 int lbfgsb_engine_set_array(lbfgsb_engine* e, const char* name, const void* in,
                             long long count);

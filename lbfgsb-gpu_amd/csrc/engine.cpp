@@ -531,6 +531,11 @@ struct Engine final : EngineBase {
     if (name == "mc") return vec(hs->mc, std::min<long long>(count, 16));
     if (name == "wv") return vec(hs->wv, std::min<long long>(count, 16));
     if (name == "pa") return vec(hs->pa, std::min<long long>(count, PA_COUNT));
+    if (name == "dsave") return vec(hs->dsave, std::min<long long>(count, 13));
+    if (name == "isave") {
+      for (long long i = 0; i < std::min<long long>(count, 2); i++) o[i] = hs->isave[i];
+      return 0;
+    }
     return -2;
   }
 
@@ -576,6 +581,11 @@ struct Engine final : EngineBase {
       return write_scalars();
     };
     if (name == "c") return vec(hs->c, std::min<long long>(count, 16));
+    if (name == "dsave") return vec(hs->dsave, std::min<long long>(count, 13));
+    if (name == "isave") {
+      for (long long i = 0; i < std::min<long long>(count, 2); i++) hs->isave[i] = (int)s[i];
+      return write_scalars();
+    }
     return -2;
   }
 };

@@ -622,7 +622,6 @@ struct P5bOp {
     return zi;
   }
   __device__ __forceinline__ void element(long long i, const In& e) {
-    const T big = (T)1.0e10;
     const T xi = e.x;
     const T zi = subspace_point(i, e);
     z[i] = zi;  // read back soon (objective, pass A on the trial point): normal stores
@@ -630,20 +629,26 @@ struct P5bOp {
     acc[P5_DTD] += (double)di * (double)di;
     acc[P5_GD] += (double)e.g * (double)di;
     if (want_stpmx) {
+      // the line search's largest feasible step is an FP64 scalar: each
+      // ratio is formed in FP64 (the bound distance of two bulk values is
+      // exact there), so the fp32 form takes the decisions the FP64 search
+      // takes on the same data
+      constexpr double big = 1.0e10;
       const int nb = iw_nbd(e.iwb);
+      const double dd = (double)di;
       if (nb != 0) {
         if (di < 0 && nb <= 2) {
-          const T a2 = e.l - xi;
+          const double a2 = (double)e.l - (double)xi;
           if (a2 >= 0)
             smin = 0.0;
-          else if (di * big < a2)
-            smin = fmin(smin, (double)(a2 / di));
+          else if (dd * big < a2)
+            smin = fmin(smin, a2 / dd);
         } else if (di > 0 && nb >= 2) {
-          const T a2 = e.u - xi;
+          const double a2 = (double)e.u - (double)xi;
           if (a2 <= 0)
             smin = 0.0;
-          else if (di * big > a2)
-            smin = fmin(smin, (double)(a2 / di));
+          else if (dd * big > a2)
+            smin = fmin(smin, a2 / dd);
         }
       }
     }

@@ -199,6 +199,10 @@ class Engine:
             out = np.zeros(16 if count is None else count)
         elif name == "pa":
             out = np.zeros(39)
+        elif name == "dsave":
+            out = np.zeros(13)
+        elif name == "isave":
+            out = np.zeros(2)
         else:
             out = np.zeros(n, dtype=self.np_dtype)
         rc = self.L.lbfgsb_engine_get_array(self.h, name.encode(),

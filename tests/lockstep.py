@@ -50,6 +50,88 @@ def scalar_close(a, b, rtol, what):
     assert abs(a - b) <= rtol * max(abs(b), 1e-300), f"{what}: {a!r} vs {b!r}"
 
 
+def load_state(engine, s, p):
+    """Copy the oracle state (top of an iteration) into the engine."""
+    st = s.st
+    n, m = st.n, st.m
+    dt = engine.np_dtype
+    engine.set_array("x", s.arr("x").astype(dt))
+    engine.set_array("g", s.arr("g").astype(dt))
+    engine.set_array("ws", s.arr("ws").astype(dt))
+    engine.set_array("wy", s.arr("wy").astype(dt))
+    for name in ("sy", "ss", "wt"):
+        engine.set_array(name, s.arr(name).astype(np.float64))
+    engine.set_array("wn", s.arr("wn").astype(np.float64))
+    engine.set_array("snd", s.arr("snd").astype(np.float64))
+    c = np.zeros(16)
+    wa = np.ctypeslib.as_array(st.wa)
+    c[: 2 * st.col] = wa[2 * m: 2 * m + 2 * st.col]
+    engine.set_array("c", c)
+    # the line search's saved state: a search whose first step exceeds
+    # stpmx resumes from the previous search's (lbfgsbcpu.cpp:1574-1608)
+    engine.set_array("dsave", np.array([float(v) for v in st.dsave]))
+    engine.set_array("isave", np.array([float(v) for v in st.isave]))
+    iw = s.arr("iwhere").astype(np.int64)
+    free_last = np.zeros(n, bool)
+    idx = s.arr("index")
+    free_last[idx[: st.nfree] - 1] = True
+    bits = ((iw + 4) | (free_last.astype(np.int64) << 4) | (free_last.astype(np.int64) << 5)
+            | (p.nbd.astype(np.int64) << 6))  # the class byte carries nbd
+    engine.set_array("iwbits", bits.astype(np.uint8))
+    vals = dict(col=st.col, head=st.head - 1, itail=max(st.itail - 1, 0), iupdat=st.iupdat,
+                iter=st.iter, updatd=st.updatd, theta=st.theta, f=st.f, cnstnd=st.cnstnd,
+                boxed=st.boxed, nfree=st.nfree, sbgnrm=st.sbgnrm, nint=st.nint, nfgv=st.nfgv,
+                nskip=st.nskip, restarts=st.restarts, phase=0,
+                status=0, halt=0, eps_g=1e-10, eps_f=1e-10, eps_x=1e-10, max_iteration=200,
+                has_after_iteration=0, has_custom_stop=0, upd_done=0)
+    for k, v in vals.items():
+        engine.set_scalar(k, v)
+
+
+_SCALARS_I = ["col", "head", "itail", "iupdat", "iter", "nfgv", "nint", "nintol", "nskip",
+              "nfree", "nact", "ileave", "nenter", "iword", "ifun", "iback", "info", "task",
+              "csave", "internalinfo", "prjctd", "cnstnd", "boxed", "updatd", "wrk", "k_err",
+              "restarts"]
+_SCALARS_R = ["f", "fold", "theta", "dr", "rr", "dnrm", "xstep", "sbgnrm", "ddum", "dtd", "gd",
+              "gdold", "stp", "stpmx", "tf"]
+
+
+def as_f64(oracle, s32, p):
+    """The float32 oracle state widened to an FP64 oracle state: the
+    reference for the mixed-precision engine (FP32 data, FP64 scalar and
+    small-dense logic)."""
+    p64 = p.cast(np.float32).cast(np.float64)
+    s = oracle.Solver(p64, dtype=np.float64, eps=1e-10, max_iter=200)
+    a, b = s32.st, s.st
+    for k in _SCALARS_I:
+        setattr(b, k, getattr(a, k))
+    for k in _SCALARS_R:
+        setattr(b, k, float(getattr(a, k)))
+    for i in range(2):
+        b.isave[i] = a.isave[i]
+    for i in range(13):
+        b.dsave[i] = float(a.dsave[i])
+    for i in range(64):
+        b.wa[i] = float(a.wa[i])
+    for name in ("x", "g", "l", "u", "xold", "xdiff", "z", "r", "d", "t", "ws", "wy", "sy",
+                 "ss", "wt", "wn", "snd"):
+        s.arr(name)[:] = s32.arr(name).astype(np.float64)
+    for name in ("nbd", "index", "iwhere", "indx2"):
+        s.arr(name)[:] = s32.arr(name)
+    return s
+
+
+def states(oracle, p, dtype, iters):
+    s = oracle.Solver(p, dtype=dtype, eps=1e-10, max_iter=200)
+    out, k = [], 0
+    while s.more and k <= max(iters):
+        if k in iters:
+            out.append((k, s.clone()))
+        s.step()
+        k += 1
+    return out
+
+
 class Lockstep:
     """One engine and one oracle state driven together.
 
@@ -81,6 +163,9 @@ class Lockstep:
         self.status = None
         self.iters = 0
         self.counts = dict(trials=0, restarts=0, skips=0, rounds=0, max_cand=0)
+        # feed(lockstep, f, x_trial, g) -> (f, g or None): replaces the value
+        # (and gradient) of an evaluation on both sides
+        self.feed = None
 
     # ---------------------------------------------------------------- utils
     @property
@@ -209,14 +294,27 @@ class Lockstep:
         e, ref, rs, rtol = self.eng, self.ref, self.rs, self.rtol
         while rs.internalinfo == 0 and rs.iback < 20 and rs.task == 1:
             assert self.status == S.ST_NEED_EVAL, ("status", self.status)
-            scalar_close(self.es("stp"), rs.stp, rtol, "stp")
+            if abs(self.es("stp") - rs.stp) > rtol * abs(rs.stp):
+                names = ("stp", "f", "fold", "gd", "gdold", "stpmx", "dtd", "iback", "csave",
+                         "iter", "col")
+                got = {k: (self.es(k), float(getattr(rs, k))) for k in names}
+                raise AssertionError(f"line search differs (engine, oracle): {got}; "
+                                     f"isave {list(rs.isave)} dsave {list(rs.dsave)}")
             for k in ("iback", "ifun", "nfgv"):
                 assert int(self.es(k)) == int(getattr(rs, k)), (k, self.es(k), getattr(rs, k))
             x_t = e.array("xb")
             close(x_t, ref.arr("x"), rtol, "trial point")
             f_t = e.evaluate(1)
+            g_t = e.array("gb")
+            if self.feed is not None:
+                # forced function values (failure-path tests): the same f
+                # and g go to both sides
+                f_t, g_new = self.feed(self, f_t, x_t, g_t)
+                if g_new is not None:
+                    g_t = np.ascontiguousarray(g_new, dtype=e.np_dtype)
+                    e.set_array("gb", g_t)
             ref.arr("x")[:] = x_t
-            ref.arr("g")[:] = e.array("gb")
+            ref.arr("g")[:] = g_t
             rs.f = f_t
             ref.fl.lnsrlb_step(ref.s)
             self.counts["trials"] += 1

@@ -12,7 +12,7 @@ input is rejected as lbfgsbminimize rejects it.
 import numpy as np
 import pytest
 
-from conftest import gpu_available
+from conftest import gpu_available, rel_error, solve_tolerance
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
@@ -70,11 +70,11 @@ def test_batch_matches_oracle(dtype, oracle):
         slack = 1e-12 if dtype == np.float64 else 1e-6
         assert np.all(x >= lo - slack) and np.all(x <= hi + slack), name
         f_ref = ref.st.f
-        rel = abs(f_x - f_ref) / max(abs(f_ref), 1.0)
+        rel = rel_error(f_x, f_ref, p.kind)
         # the reported f is the objective at the returned x (rounded to the
         # bulk type in fp32, as a callback hands it over)
-        rel_rep = abs(f - f_x) / max(abs(f_x), 1.0)
-        tol = 1e-6 if dtype == np.float64 else 1e-5
+        rel_rep = rel_error(f, f_x, p.kind)
+        tol = solve_tolerance(eps, f_ref, p.kind, 1e-6 if dtype == np.float64 else 1e-5)
         print(name, np.dtype(dtype).name, "batch f", f_x, "oracle f", f_ref, "rel", rel,
               "iters", it, ref.st.iter, "info", info, ref.st.info)
         assert rel < tol, name
@@ -179,9 +179,9 @@ def test_batch_gangs_match_oracle(dtype, oracle):
         slack = 1e-12 if dtype == np.float64 else 1e-6
         assert np.all(x >= lo - slack) and np.all(x <= hi + slack), name
         f_ref = ref.st.f
-        rel = abs(f_x - f_ref) / max(abs(f_ref), 1.0)
-        rel_rep = abs(f - f_x) / max(abs(f_x), 1.0)
-        tol = 1e-6 if dtype == np.float64 else 1e-5
+        rel = rel_error(f_x, f_ref, p.kind)
+        rel_rep = rel_error(f, f_x, p.kind)
+        tol = solve_tolerance(eps, f_ref, p.kind, 1e-6 if dtype == np.float64 else 1e-5)
         print(name, np.dtype(dtype).name, "gang f", f_x, "oracle f", f_ref, "rel", rel,
               "iters", it, ref.st.iter, "info", info, ref.st.info)
         assert rel < tol, name

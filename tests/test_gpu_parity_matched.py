@@ -6,7 +6,15 @@ both then run the same iteration phase by phase:
                                                 set sizes bit-exact; x_cp, wn
   subspace step + first line-search call     -> z, d, stp, d'd, g'd
   first trial point                          -> x_trial
-  line search to acceptance + memory update  -> x_{k+1}, sy, ss, theta
+  line search to acceptance                  -> stp, iback, ifun, nfgv at
+                                                each trial, the trial points
+  stopping tests + memory update             -> info / iter, sbgnrm (bit-exact
+                                                in the bulk precision), tf,
+                                                the update-or-skip decision,
+                                                col/head/itail, theta, S'Y,
+                                                S'S, wt and the new S/Y pair
+(the last two through tests/lockstep.py, with the GPU's f and g at each trial
+handed to the oracle)
 FP tolerances: fp64 rtol 1e-9.  For fp32 the engine keeps n-vectors in FP32
 but runs the Cauchy-search scalars, the small dense algebra and the line
 search in FP64, so its reference is the FP64 oracle applied to the same
@@ -18,6 +26,7 @@ import numpy as np
 import pytest
 
 from conftest import gpu_available
+from lockstep import Lockstep, as_f64 as _as_f64, load_state as _load, states as _states
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
@@ -33,83 +42,6 @@ def _problems():
             "penta": pb.penta_qp(4000, seed=102),
             "rosen": pb.rosenbrock(4000, seed=103),
             "grid_quad": pb.grid_quadratic(48, 50, seed=104)}
-
-
-def _load(engine, s, p):
-    """Copy the oracle state (top of an iteration) into the engine."""
-    st = s.st
-    n, m = st.n, st.m
-    dt = engine.np_dtype
-    engine.set_array("x", s.arr("x").astype(dt))
-    engine.set_array("g", s.arr("g").astype(dt))
-    engine.set_array("ws", s.arr("ws").astype(dt))
-    engine.set_array("wy", s.arr("wy").astype(dt))
-    for name in ("sy", "ss", "wt"):
-        engine.set_array(name, s.arr(name).astype(np.float64))
-    engine.set_array("wn", s.arr("wn").astype(np.float64))
-    engine.set_array("snd", s.arr("snd").astype(np.float64))
-    c = np.zeros(16)
-    wa = np.ctypeslib.as_array(st.wa)
-    c[: 2 * st.col] = wa[2 * m: 2 * m + 2 * st.col]
-    engine.set_array("c", c)
-    iw = s.arr("iwhere").astype(np.int64)
-    free_last = np.zeros(n, bool)
-    idx = s.arr("index")
-    free_last[idx[: st.nfree] - 1] = True
-    bits = ((iw + 4) | (free_last.astype(np.int64) << 4) | (free_last.astype(np.int64) << 5)
-            | (p.nbd.astype(np.int64) << 6))  # the class byte carries nbd
-    engine.set_array("iwbits", bits.astype(np.uint8))
-    vals = dict(col=st.col, head=st.head - 1, itail=max(st.itail - 1, 0), iupdat=st.iupdat,
-                iter=st.iter, updatd=st.updatd, theta=st.theta, f=st.f, cnstnd=st.cnstnd,
-                boxed=st.boxed, nfree=st.nfree, sbgnrm=st.sbgnrm, nint=st.nint, phase=0,
-                status=0, halt=0, eps_g=1e-10, eps_f=1e-10, eps_x=1e-10, max_iteration=200,
-                has_after_iteration=0, has_custom_stop=0, upd_done=0)
-    for k, v in vals.items():
-        engine.set_scalar(k, v)
-
-
-_SCALARS_I = ["col", "head", "itail", "iupdat", "iter", "nfgv", "nint", "nintol", "nskip",
-              "nfree", "nact", "ileave", "nenter", "iword", "ifun", "iback", "info", "task",
-              "csave", "internalinfo", "prjctd", "cnstnd", "boxed", "updatd", "wrk", "k_err",
-              "restarts"]
-_SCALARS_R = ["f", "fold", "theta", "dr", "rr", "dnrm", "xstep", "sbgnrm", "ddum", "dtd", "gd",
-              "gdold", "stp", "stpmx", "tf"]
-
-
-def _as_f64(oracle, s32, p):
-    """The float32 oracle state widened to an FP64 oracle state: the
-    reference for the mixed-precision engine (FP32 data, FP64 scalar and
-    small-dense logic)."""
-    p64 = p.cast(np.float32).cast(np.float64)
-    s = oracle.Solver(p64, dtype=np.float64, eps=1e-10, max_iter=200)
-    a, b = s32.st, s.st
-    for k in _SCALARS_I:
-        setattr(b, k, getattr(a, k))
-    for k in _SCALARS_R:
-        setattr(b, k, float(getattr(a, k)))
-    for i in range(2):
-        b.isave[i] = a.isave[i]
-    for i in range(13):
-        b.dsave[i] = float(a.dsave[i])
-    for i in range(64):
-        b.wa[i] = float(a.wa[i])
-    for name in ("x", "g", "l", "u", "xold", "xdiff", "z", "r", "d", "t", "ws", "wy", "sy",
-                 "ss", "wt", "wn", "snd"):
-        s.arr(name)[:] = s32.arr(name).astype(np.float64)
-    for name in ("nbd", "index", "iwhere", "indx2"):
-        s.arr(name)[:] = s32.arr(name)
-    return s
-
-
-def _states(oracle, p, dtype, iters):
-    s = oracle.Solver(p, dtype=dtype, eps=1e-10, max_iter=200)
-    out, k = [], 0
-    while s.more and k <= max(iters):
-        if k in iters:
-            out.append((k, s.clone()))
-        s.step()
-        k += 1
-    return out
 
 
 def _close(a, b, rtol, what):
@@ -190,39 +122,32 @@ def test_matched_iteration(oracle, pname, dtype):
         # ---- GPU through the first line-search call and trial point
         status = eng.run(PROG_ITER_START, K_P5A, K_END)
         assert status == ST_NEED_EVAL, status
-        _close(eng.array("z"), z_ref, rtol, "z (subspace)")
+        z_gpu, d_gpu = eng.array("z"), eng.array("d")
+        _close(z_gpu, z_ref, rtol, "z (subspace)")
         d_ref = z_ref - ref.arr("x")
-        _close(eng.array("d"), d_ref, 10 * rtol, "d")
+        _close(d_gpu, d_ref, 10 * rtol, "d")
         xt = eng.array("xb")
-        # oracle first lnsrlb call
+        # the oracle's first lnsrlb call (lbfgsbcpu.cpp:343-351), from the
+        # GPU's z and d (compared above): the line search and the memory
+        # update are then compared on the same direction
+        ref.arr("z")[:] = z_gpu
+        ref.arr("d")[:] = d_gpu
         rs.task = 0
-        from oracle import pyoracle  # noqa: F401
-        ref_x0 = ref.arr("x").copy()
-        _lnsrlb_first(ref)
+        ref.fl.lnsrlb_step(ref.s)
         assert abs(eng.scalar("stp") - rs.stp) <= rtol * max(1.0, abs(rs.stp)), \
             (eng.scalar("stp"), rs.stp)
         assert abs(eng.scalar("dtd") - rs.dtd) <= 10 * rtol * abs(rs.dtd)
         _close(xt, ref.arr("x"), rtol, "x_trial")
-        assert np.all(np.isfinite(ref_x0))
+        # ---- line search to acceptance (the GPU's f and g at each trial
+        # handed to the oracle), stopping tests and memory update
+        ls = Lockstep(oracle, p, dtype, eps=1e-10, max_iter=200, spec=False, eng=eng,
+                      ref=ref, dp=dp)
+        ls.status = status
+        ls.search()
+        r = ls.tail()
+        print(f"  accepted stp {rs.stp:.6g} after {rs.ifun} trial(s); tail {r}, "
+              f"updatd {rs.updatd}, col {rs.col}", flush=True)
+        assert r in (oracle.TAIL_NEXT, oracle.TAIL_END)
         checked += 1
         eng.close()
     assert checked >= 4
-
-
-def _lnsrlb_first(ref):
-    """First lnsrlb call of the oracle on its own state: the reference
-    iteration sets d = z - x, task = 0 and calls lnsrlb (lbfgsbcpu.cpp:343-351)."""
-    import ctypes as C
-    fl = ref.fl
-    n = ref.st.n
-    x = ref.arr("x")
-    z = ref.arr("z")
-    d = ref.arr("d")
-    d[:] = z - x
-    ref.st.task = 0
-    lib = fl.lib
-    f = getattr(lib, f"orc_lnsrlb_step_{fl.sfx}")
-    f.restype = C.c_int
-    f.argtypes = [C.POINTER(fl.State)]
-    f(ref.s)
-    assert n == len(x)

@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from conftest import gpu_available
+from conftest import gpu_available, rel_error, solve_tolerance
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
@@ -138,10 +138,10 @@ def test_generated_problem_solves_like_uploaded(oracle):
         eps, mi = pb.KIND_SETTINGS[kind]
         ref = oracle.Solver(p, dtype=np.float64, eps=eps, max_iter=mi).run()
         f_x, _ = pb.objective_numpy(p, x)
-        rel = abs(f_x - ref.st.f) / max(abs(ref.st.f), 1.0)
+        rel = rel_error(f_x, ref.st.f, p.kind)
         print(p.name, "f", f_x, "oracle", ref.st.f, "rel", rel, "iters", it, ref.st.iter)
         assert done == 1 and info in (1, 2, 4, 5)
-        assert rel < 1e-6, p.name
+        assert rel < solve_tolerance(eps, ref.st.f, kind, 1e-6), p.name
 
 
 def _stream(count=14, seed=31):
@@ -207,7 +207,7 @@ def test_serve_reaches_oracle_minima(oracle):
     for q, r in zip(reqs, res):
         p = pb.problem_of(q)
         ref = oracle.Solver(p, dtype=np.float64, eps=q["eps"], max_iter=q["max_iteration"]).run()
-        rel = abs(r["f"] - ref.st.f) / max(abs(ref.st.f), 1.0)
+        rel = rel_error(r["f"], ref.st.f, p.kind)
         tol = 1e-6 if q["dtype"] == 1 else 1e-5
         print(p.name, q["dtype"], "served f", r["f"], "oracle", ref.st.f, "rel", rel)
         assert rel < tol, p.name
