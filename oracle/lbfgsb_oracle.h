@@ -18,7 +18,10 @@
  *
  * Pinning status (see tests/test_oracle_*.py): every component is checked
  * against an independent statement of the mathematics (brute force, closed
- * forms, long-double factorizations, MINPACK-2 invariants).  The reference
+ * forms, long-double factorizations, MINPACK-2 invariants); the driver's
+ * parts (active's projection and classes, lnsrlb's largest feasible step and
+ * first step, the order of the stopping tests and their info codes) against
+ * their definitions (tests/test_oracle_pins_driver.py).  The reference
  * itself cannot be compiled in this image (its CPU path includes CUDA and
  * cuBLAS headers through culbfgsb/lbfgsbcuda.h:13 and cutil_inline.h:13), so
  * no function is pinned against reference outputs.

@@ -355,7 +355,8 @@ def test_kkt_at_termination(oracle):
 
 
 def test_objectives_match_numpy(oracle):
-    for p in [pb.grid_quadratic(9, 7), pb.grid_cos(8, 8), pb.rosenbrock(20), pb.penta_qp(33)]:
+    for p in [pb.grid_quadratic(9, 7), pb.grid_cos(8, 8), pb.rosenbrock(20), pb.penta_qp(33),
+              pb.dsscfg(9, 7, lam=2.5)]:
         fl = oracle.load()[0]
         obj = oracle.Objective(fl, p)
         x = RNG.uniform(-0.5, 0.5, p.n)
@@ -365,6 +366,13 @@ def test_objectives_match_numpy(oracle):
         fr, gr = pb.objective_numpy(p, x)
         assert np.isclose(f.value, fr, rtol=1e-12)
         assert np.allclose(g, gr, rtol=1e-12, atol=1e-14)
+        # g is the gradient of f: directional derivative by a 4th-order
+        # central difference along a random direction
+        v = RNG.standard_normal(p.n)
+        h = 1e-3
+        fs = [pb.objective_numpy(p, x + k * h * v)[0] for k in (-2, -1, 1, 2)]
+        fd = (fs[0] - 8 * fs[1] + 8 * fs[2] - fs[3]) / (12 * h)
+        assert abs(fd - gr @ v) <= 1e-7 * max(1.0, abs(gr @ v)), (p.name, fd, gr @ v)
 
 
 # ------------------------------------------------------- failure paths
