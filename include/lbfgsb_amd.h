@@ -92,6 +92,25 @@ int lbfgsb_minimize_f32(int n, const lbfgsb_callbacks_f32* cb, const lbfgsb_opti
                         float* x, const int* nbd, const float* l, const float* u,
                         lbfgsb_summary_f32* summary, void* stream);
 
+/* lbfgsb_minimize_{f64,f32} without any allocation (no counterpart in the
+ * reference, which allocates and frees its workspace on every call,
+ * culbfgsb/lbfgsbcuda.cpp:127-179, 399-440): for callers that cannot
+ * allocate -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Same arguments, results and info codes as
+ * lbfgsb_minimize; the workspace is a fixed 64 MiB region of device memory
+ * per GPU, held until the process ends, which admits fp64 solves up to
+ * 286,624 variables and fp32 up to 485,936 (m = 8).  A larger n returns -3 (info
+ * -3) with a message, leaving x untouched.  Every call on one GPU shares that
+ * GPU's region: calls that overlap on one GPU race. */
+int lbfgsb_minimize_noalloc_f64(int n, const lbfgsb_callbacks_f64* cb,
+                                const lbfgsb_options_f64* opt, double* x, const int* nbd,
+                                const double* l, const double* u, lbfgsb_summary_f64* summary,
+                                void* stream);
+int lbfgsb_minimize_noalloc_f32(int n, const lbfgsb_callbacks_f32* cb,
+                                const lbfgsb_options_f32* opt, float* x, const int* nbd,
+                                const float* l, const float* u, lbfgsb_summary_f32* summary,
+                                void* stream);
+
 /* ---- Stepping engine: the per-step wrappers of the reference GPU driver
  * (culbfgsb/lbfgsbcuda.cpp:443-770: lbfgsbactive, projgr, cauchy, freev,
  * formk, cmprlb, subsm, lnsrlb, matupd, formt) become programs of fused

@@ -25,7 +25,7 @@ EXAMPLE_SRC = os.path.join(HERE, "examples", "dsscfg.hip")
 EXAMPLE = os.path.join(LIBDIR, "dsscfg_example")
 
 SOURCES = ["batch.hip", "bulk.hip", "small.hip", "gcp.hip", "freev.hip", "objectives.hip",
-           "generate.hip", "engine.cpp", "capi.cpp", "batch_host.cpp", "serve.cpp"]
+           "generate.hip", "noalloc.hip", "engine.cpp", "capi.cpp", "batch_host.cpp", "serve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LBFGSB_AMD_ARCH", "gfx950")

@@ -176,13 +176,14 @@ int drive(EngineBase* e, const Cb* cb, const Opt* opt, Summary* summary, void* s
   return info;
 }
 
-template <typename R, typename Opt, typename Cb, typename Summary>
-int minimize_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x, const int* nbd,
-                  const R* l, const R* u, Summary* summary, void* stream) {
+// The argument checks of lbfgsbminimize; returns m, or 0 after setting
+// summary->info = -1.
+template <typename Opt, typename Cb, typename Summary>
+int check_minimize_args(int n, const Cb* cb, const Opt* opt, Summary* summary) {
   std::memset(summary, 0, sizeof(Summary));
   if (!cb || !cb->funcgrad || !opt) {
     summary->info = -1;
-    return -1;
+    return 0;
   }
   if (opt->mode != LBFGSB_AMD_MODE_GPU) {
     std::fprintf(stderr,
@@ -190,7 +191,7 @@ int minimize_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x, const in
                  "only)\n",
                  opt->mode);
     summary->info = -1;
-    return -1;
+    return 0;
   }
   const int m = opt->hessian_approximate_dimension < 8 ? opt->hessian_approximate_dimension : 8;
   // errclb's scalar checks (lbfgsbcpu.cpp:879-890) and the driver's option
@@ -198,20 +199,70 @@ int minimize_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x, const in
   if (n <= 0 || m <= 0 || m > n || opt->eps_f < 0 || opt->max_iteration < 0 ||
       opt->eps_g < 0 || opt->eps_x < 0) {
     summary->info = -1;
-    return -1;
+    return 0;
   }
-  EngineBase* e = make_engine(dtype, n, m, x, nbd, l, u, (hipStream_t)stream);
-  if (!e) {
-    summary->info = -2;
-    return -2;
-  }
+  return m;
+}
+
+template <typename R, typename Opt, typename Cb, typename Summary>
+int solve_on(EngineBase* e, int n, int m, const Cb* cb, const Opt* opt, Summary* summary,
+             void* stream) {
   DevScalars& h = e->host_scalars();
   std::memset(&h, 0, sizeof(DevScalars));
   h.n = n;
   h.m = m;
   e->write_scalars();
-  const int info = drive<R>(e, cb, opt, summary, stream, nullptr, 0);
+  return drive<R>(e, cb, opt, summary, stream, nullptr, 0);
+}
+
+template <typename R, typename Opt, typename Cb, typename Summary>
+int minimize_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x, const int* nbd,
+                  const R* l, const R* u, Summary* summary, void* stream) {
+  const int m = check_minimize_args(n, cb, opt, summary);
+  if (!m) return -1;
+  EngineBase* e = make_engine(dtype, n, m, x, nbd, l, u, (hipStream_t)stream);
+  if (!e) {
+    summary->info = -2;
+    return -2;
+  }
+  const int info = solve_on<R>(e, n, m, cb, opt, summary, stream);
   delete e;
+  return info;
+}
+
+// The same solve with no allocation: the engine object on this stack, its
+// device buffers carved from this GPU's scratch region (noalloc.hip).
+template <typename R, typename Opt, typename Cb, typename Summary>
+int minimize_noalloc_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x,
+                          const int* nbd, const R* l, const R* u, Summary* summary,
+                          void* stream) {
+  const int m = check_minimize_args(n, cb, opt, summary);
+  if (!m) return -1;
+  void* region = nullptr;
+  size_t cap = 0;
+  if (noalloc_region(&region, &cap) != 0) {
+    std::fprintf(stderr, "lbfgsb_amd: no scratch region on the current device\n");
+    summary->info = -2;
+    return -2;
+  }
+  const size_t need = engine_region_bytes(dtype, n, m);
+  if (need > cap) {
+    std::fprintf(stderr,
+                 "lbfgsb_amd: lbfgsb_minimize_noalloc: n = %d needs %zu bytes of device "
+                 "scratch, the allocation-free region holds %zu; use lbfgsb_minimize\n",
+                 n, need, cap);
+    summary->info = -3;
+    return -3;
+  }
+  alignas(64) unsigned char storage[kEngineObjectBytes];
+  EngineBase* e = make_engine_in(storage, dtype, n, m, x, nbd, l, u, (hipStream_t)stream,
+                                 region, cap);
+  if (!e) {
+    summary->info = -2;
+    return -2;
+  }
+  const int info = solve_on<R>(e, n, m, cb, opt, summary, stream);
+  e->~EngineBase();
   return info;
 }
 
@@ -260,6 +311,20 @@ int lbfgsb_minimize_f32(int n, const lbfgsb_callbacks_f32* cb, const lbfgsb_opti
                         float* x, const int* nbd, const float* l, const float* u,
                         lbfgsb_summary_f32* summary, void* stream) {
   return minimize_impl<float>(0, n, cb, opt, x, nbd, l, u, summary, stream);
+}
+
+int lbfgsb_minimize_noalloc_f64(int n, const lbfgsb_callbacks_f64* cb,
+                                const lbfgsb_options_f64* opt, double* x, const int* nbd,
+                                const double* l, const double* u, lbfgsb_summary_f64* summary,
+                                void* stream) {
+  return minimize_noalloc_impl<double>(1, n, cb, opt, x, nbd, l, u, summary, stream);
+}
+
+int lbfgsb_minimize_noalloc_f32(int n, const lbfgsb_callbacks_f32* cb,
+                                const lbfgsb_options_f32* opt, float* x, const int* nbd,
+                                const float* l, const float* u, lbfgsb_summary_f32* summary,
+                                void* stream) {
+  return minimize_noalloc_impl<float>(0, n, cb, opt, x, nbd, l, u, summary, stream);
 }
 
 lbfgsb_engine* lbfgsb_engine_create(int dtype, long long n, int m, void* x, const int* nbd,

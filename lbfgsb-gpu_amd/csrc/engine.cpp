@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "common.hpp"
@@ -92,6 +93,14 @@ struct Engine final : EngineBase {
   unsigned* exp_counts = nullptr;
   unsigned long long* exp_totals = nullptr;
   bool ok = false;
+  // Device memory: hipMalloc per buffer (default), or every buffer carved,
+  // 256-byte aligned, from one region the caller owns (region != nullptr;
+  // then no allocation is made, the status is read with a plain copy and
+  // hs is the host copy below).  measuring: only sizes the region.
+  unsigned char* region = nullptr;
+  size_t region_cap = 0, region_off = 0;
+  bool measuring = false;
+  DevScalars hs_store;
 
   // ---- optional per-step timing (HIP events on st)
   struct Pending {
@@ -105,33 +114,45 @@ struct Engine final : EngineBase {
   double step_ms[kNumSteps] = {};
   long long step_n[kNumSteps] = {};
 
-  Engine(long long n_, int m_, T* x, const int* nbd_, const T* l_, const T* u_, hipStream_t s)
-      : n(n_), m(m_), st(s), l(l_), u(u_), nbd(nbd_) {
+  Engine(long long n_, int m_, T* x, const int* nbd_, const T* l_, const T* u_, hipStream_t s,
+         unsigned char* region_ = nullptr, size_t region_bytes = 0, bool measure = false)
+      : n(n_), m(m_), st(s), l(l_), u(u_), nbd(nbd_), region(region_),
+        region_cap(region_bytes), measuring(measure) {
     X[0] = x;
     X[1] = Gr[0] = Gr[1] = nullptr;
     long long blocks = (n + kBlock - 1) / kBlock;
     G = (int)std::max<long long>(1, std::min<long long>(blocks, kMaxBlocks));
-    ok = alloc() == 0;
+    ok = measuring ? carve_all() == 0 : alloc() == 0;
   }
+
+  bool owns_memory() const { return !region && !measuring; }
 
   template <typename P>
   int dmalloc(P*& p, size_t count) {
-    HIPCHK(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(P)));
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(P);
+    if (!owns_memory()) {
+      const size_t at = (region_off + 255) & ~(size_t)255;
+      region_off = at + bytes;
+      if (measuring) {
+        p = nullptr;
+        return 0;
+      }
+      if (region_off > region_cap) return -1;
+      p = (P*)(region + at);
+      return 0;
+    }
+    HIPCHK(hipMalloc((void**)&p, bytes));
     return 0;
   }
 
-  int alloc() {
+  // Every device buffer of a solve, in one order for all three modes.
+  int carve_all() {
     const size_t nn = (size_t)n;
     if (dmalloc(X[1], nn) || dmalloc(Gr[0], nn) || dmalloc(Gr[1], nn)) return -1;
     if (dmalloc(W, wsize(nn)) || dmalloc(d, nn) || dmalloc(dsub, nn) ||
         dmalloc(tbrk, nn) || dmalloc(iwhere, nn))
       return -1;
-    HIPCHK(hipMemsetAsync(W, 0, wsize(nn) * sizeof(T), st));
     if (dmalloc(ds, 1)) return -1;
-    HIPCHK(hipHostMalloc((void**)&hs, sizeof(DevScalars), hipHostMallocCoherent));
-    std::memset(hs, 0, sizeof(DevScalars));
-    HIPCHK(hipHostMalloc((void**)&mseq, sizeof(MailSeq), hipHostMallocCoherent));
-    std::memset(mseq, 0, sizeof(MailSeq));
     const size_t nv = std::max<int>(P4_COUNT, PA_COUNT);
     if (dmalloc(part, nv * G) || dmalloc(pmax, G) || dmalloc(pmin, G) || dmalloc(pmini, G))
       return -1;
@@ -141,6 +162,25 @@ struct Engine final : EngineBase {
         dmalloc(tpre, 33 * ntiles) || dmalloc(total, 40) || dmalloc(tile_state, 33 * ntiles) ||
         dmalloc(last_state, 40) || dmalloc(tile_stop, ntiles))
       return -1;
+    return 0;
+  }
+
+  int alloc() {
+    if (carve_all()) return -1;
+    const size_t wbytes = wsize((size_t)n) * sizeof(T);
+    if (owns_memory()) {
+      HIPCHK(hipMemsetAsync(W, 0, wbytes, st));
+      HIPCHK(hipHostMalloc((void**)&hs, sizeof(DevScalars), hipHostMallocCoherent));
+      HIPCHK(hipHostMalloc((void**)&mseq, sizeof(MailSeq), hipHostMallocCoherent));
+      std::memset(mseq, 0, sizeof(MailSeq));
+    } else {
+      // the region keeps the previous call's contents: the ring must start
+      // at zero (a fill kernel, no memset node for callers under capture)
+      launch_fill_zero(st, W, wbytes);
+      HIPCHK(hipGetLastError());
+      hs = &hs_store;
+    }
+    std::memset(hs, 0, sizeof(DevScalars));
     dX = Pair<T>{X[0], X[1]};
     dG = Pair<T>{Gr[0], Gr[1]};
     HIPCHK(hipStreamSynchronize(st));
@@ -195,9 +235,11 @@ struct Engine final : EngineBase {
   }
 
   ~Engine() override {
+    if (measuring) return;
     hipStreamSynchronize(st);
     drain_events();
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
+    if (!owns_memory()) return;
     T* bufs[] = {X[1], Gr[0], Gr[1], W, d, dsub, tbrk, insp};
     for (T* p : bufs) hipFree(p);
     hipFree(iwhere);
@@ -236,6 +278,7 @@ struct Engine final : EngineBase {
   // The same at the end of a chain: a kernel writes the state into hs and
   // publishes a sequence number the host spins on (host_mailbox.hpp).
   int read_status() {
+    if (!owns_memory()) return read_scalars();  // no pinned mailbox in a region
     launch_publish(st, ds, hs, mseq, ++mail_seq);
     HIPCHK(hipGetLastError());
     HIPCHK(mail_wait(st, mseq, mail_seq));
@@ -465,6 +508,9 @@ struct Engine final : EngineBase {
     if (name == "g") return dcopy(Gr[c], nn * sizeof(T));
     if (name == "xb") return dcopy(X[c ^ 1], nn * sizeof(T));
     if (name == "gb") return dcopy(Gr[c ^ 1], nn * sizeof(T));
+    if (!owns_memory() && (name == "xcp" || name == "z" || name == "d" || name == "index" ||
+                           name == "indx2"))
+      return -3;  // inspection buffers are allocated on demand: not in a region
     if (name == "xcp" || name == "z" || name == "d") {
       // the Cauchy point, the subspace point or the line-search direction
       // as the passes compute them from the current state (d is stored only
@@ -589,6 +635,43 @@ struct Engine final : EngineBase {
     return -2;
   }
 };
+
+size_t engine_region_bytes(int dtype, long long n, int m) {
+  if (dtype == 1) {
+    Engine<double> e(n, m, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, true);
+    return e.region_off;
+  }
+  Engine<float> e(n, m, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, true);
+  return e.region_off;
+}
+
+static_assert(sizeof(Engine<double>) <= kEngineObjectBytes, "engine object storage");
+static_assert(sizeof(Engine<float>) <= kEngineObjectBytes, "engine object storage");
+
+EngineBase* make_engine_in(void* storage, int dtype, long long n, int m, void* x,
+                           const int* nbd, const void* l, const void* u, hipStream_t stream,
+                           void* region, size_t region_bytes) {
+  EngineBase* e;
+  bool ok;
+  if (dtype == 1) {
+    auto* p = new (storage) Engine<double>(n, m, (double*)x, nbd, (const double*)l,
+                                           (const double*)u, stream, (unsigned char*)region,
+                                           region_bytes);
+    ok = p->ok;
+    e = p;
+  } else {
+    auto* p = new (storage) Engine<float>(n, m, (float*)x, nbd, (const float*)l,
+                                          (const float*)u, stream, (unsigned char*)region,
+                                          region_bytes);
+    ok = p->ok;
+    e = p;
+  }
+  if (!ok) {
+    e->~EngineBase();
+    return nullptr;
+  }
+  return e;
+}
 
 EngineBase* make_engine(int dtype, long long n, int m, void* x, const int* nbd, const void* l,
                         const void* u, hipStream_t stream) {

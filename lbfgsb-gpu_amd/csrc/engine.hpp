@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstddef>
 #include <string>
 
 #include "launch.hpp"
@@ -42,5 +43,21 @@ struct EngineBase {
 
 EngineBase* make_engine(int dtype, long long n, int m, void* x, const int* nbd, const void* l,
                         const void* u, hipStream_t stream);
+
+// An engine that allocates nothing: its object is built in `storage`
+// (kEngineObjectBytes, 64-byte aligned) and every device buffer is carved
+// from the device region [region, region + region_bytes).  nullptr when the
+// region is too small.  Destroy with e->~EngineBase().
+constexpr size_t kEngineObjectBytes = 16384;
+EngineBase* make_engine_in(void* storage, int dtype, long long n, int m, void* x,
+                           const int* nbd, const void* l, const void* u, hipStream_t stream,
+                           void* region, size_t region_bytes);
+// Bytes of device region an engine of this size needs.
+size_t engine_region_bytes(int dtype, long long n, int m);
+
+// Device scratch region for allocation-free solves (noalloc.hip): this
+// device's address and size, looked up once per device.
+int noalloc_region(void** ptr, size_t* bytes);
+void launch_fill_zero(hipStream_t s, void* p, size_t bytes);
 
 }  // namespace lbfgsb

@@ -90,6 +90,14 @@ SIGNATURES = {
     "lbfgsb_minimize_f32": (C.c_int, [C.c_int, C.POINTER(CallbacksF32), C.POINTER(OptionsF32),
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.POINTER(SummaryF32), C.c_void_p]),
+    "lbfgsb_minimize_noalloc_f64": (C.c_int, [C.c_int, C.POINTER(CallbacksF64),
+                                              C.POINTER(OptionsF64), C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.POINTER(SummaryF64),
+                                              C.c_void_p]),
+    "lbfgsb_minimize_noalloc_f32": (C.c_int, [C.c_int, C.POINTER(CallbacksF32),
+                                              C.POINTER(OptionsF32), C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.POINTER(SummaryF32),
+                                              C.c_void_p]),
     "lbfgsb_engine_create": (C.c_void_p, [C.c_int, C.c_longlong, C.c_int, C.c_void_p,
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lbfgsb_engine_destroy": (None, [C.c_void_p]),

@@ -76,9 +76,10 @@ class DeviceProblem:
 
 
 def minimize(dp: DeviceProblem, eps=1e-8, max_iter=1000, m=8, x0=None, eps_g=None,
-             eps_f=None, eps_x=None, after_iteration=None, stop=None):
-    """Plain solve through lbfgsb_minimize_{f64,f32} with the built-in
-    objective as funcgrad.  Returns (x tensor, summary dict)."""
+             eps_f=None, eps_x=None, after_iteration=None, stop=None, noalloc=False):
+    """Plain solve through lbfgsb_minimize_{f64,f32} (noalloc: through
+    lbfgsb_minimize_noalloc_{f64,f32}) with the built-in objective as
+    funcgrad.  Returns (x tensor, summary dict with the return code)."""
     torch = _torch()
     L = _lib.lib()
     x = (dp.tensors["x0"] if x0 is None else x0).clone()
@@ -105,14 +106,17 @@ def minimize(dp: DeviceProblem, eps=1e-8, max_iter=1000, m=8, x0=None, eps_g=Non
         keep.append(cb2)
         cbs.customized_stopping = C.cast(cb2, C.c_void_p)
     summ = _lib.SummaryF64() if f64 else _lib.SummaryF32()
-    fn = L.lbfgsb_minimize_f64 if f64 else L.lbfgsb_minimize_f32
+    if noalloc:
+        fn = L.lbfgsb_minimize_noalloc_f64 if f64 else L.lbfgsb_minimize_noalloc_f32
+    else:
+        fn = L.lbfgsb_minimize_f64 if f64 else L.lbfgsb_minimize_f32
     t = dp.tensors
-    fn(dp.problem.n, C.byref(cbs), C.byref(opt), C.c_void_p(x.data_ptr()),
-       C.c_void_p(t["nbd"].data_ptr()), C.c_void_p(t["l"].data_ptr()),
-       C.c_void_p(t["u"].data_ptr()), C.byref(summ), _stream_ptr(torch))
+    rc = fn(dp.problem.n, C.byref(cbs), C.byref(opt), C.c_void_p(x.data_ptr()),
+            C.c_void_p(t["nbd"].data_ptr()), C.c_void_p(t["l"].data_ptr()),
+            C.c_void_p(t["u"].data_ptr()), C.byref(summ), _stream_ptr(torch))
     torch.cuda.synchronize()
     return x, dict(info=summ.info, iters=summ.num_iteration, residual_g=summ.residual_g,
-                   residual_f=summ.residual_f, residual_x=summ.residual_x)
+                   residual_f=summ.residual_f, residual_x=summ.residual_x, rc=rc)
 
 
 class Engine:
