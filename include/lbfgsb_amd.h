@@ -237,6 +237,11 @@ int lbfgsb_batch_clocks(lbfgsb_batch* b, long long* start, long long* end,
                         double* ticks_per_second);
 /* The slot's current iterate into a device array of n elements. */
 int lbfgsb_batch_get_x(lbfgsb_batch* b, int slot, void* out);
+/* Start slot dst_slot of dst (added, not yet run) from the current iterate
+ * of slot src_slot of src (same n; the batches may differ in precision, the
+ * values are converted).  The serving loop's FP64 polish of fp32 answers.
+ * Returns 0, -1 on bad slots, -2 on a device error. */
+int lbfgsb_batch_warm_start(lbfgsb_batch* dst, int dst_slot, lbfgsb_batch* src, int src_slot);
 int lbfgsb_batch_remove(lbfgsb_batch* b, int slot);
 long long lbfgsb_batch_free_elems(lbfgsb_batch* b);
 /* Device bytes an arena of this dtype, capacity and slot count holds. */
@@ -285,7 +290,15 @@ int lbfgsb_generate(int dtype, int kind, long long n, long long nx, long long ny
  * device stays within hbm_budget_bytes (fp32_share of it for fp32).  A
  * problem's result is the one it has alone (the batched form's results do
  * not depend on the other problems).  Times are host clock seconds from the
- * start of the stream; requests are taken in arrival order. */
+ * start of the stream; requests are taken in arrival order.
+ * FP64 polish (polish = 1): an fp32 request that finishes is admitted again
+ * into the fp64 arena, ahead of waiting fp64 requests, with the same
+ * problem and tolerance and the iterations its fp32 phase left of the cap,
+ * started from its fp32 answer
+ * (lbfgsb_batch_warm_start); its answer is that FP64 solve's (info, f), so
+ * it stops on the FP64 form's own tests.  The fp32 slot is held until the
+ * polish is admitted.  A request larger than the fp64 arena keeps its fp32
+ * answer (polished = 0). */
 typedef struct {
   int kind;                 /* objective kind, as lbfgsb_generate */
   int dtype;                /* 0 fp32 bulk, 1 fp64 */
@@ -299,11 +312,13 @@ typedef struct {
 
 typedef struct {
   double f;
-  int iter;
+  int iter;                 /* iterations, both phases of a polished answer */
   int info;                 /* summary info; -1 bad request, -3 larger than its arena */
-  int gang;                 /* workgroups that solved it */
+  int gang;                 /* workgroups that solved it (the last phase) */
   int dtype;
   double t_arrival, t_admit, t_done;
+  int polished;             /* 1: an fp32 solve finished by an FP64 polish */
+  int iter_fp32;            /* iterations of the fp32 phase of a polished answer */
 } lbfgsb_response;
 
 typedef struct {
@@ -315,7 +330,7 @@ typedef struct {
   int max_gang;
   int quantum;              /* iterations per launch (> 0) */
   int use_graph;
-  int pad;
+  int polish;               /* 1: fp32 answers are finished in FP64 (below) */
 } lbfgsb_serve_config;
 
 typedef struct {

@@ -52,6 +52,11 @@ struct BatchBase {
   virtual int configure(long long segment, int max_gang) = 0;
   virtual void set_graph(bool on) = 0;
   virtual int gang_of(int slot) const = 0;
+  // the slot's iterate in the arena (device), its size, the bulk type
+  virtual void* slot_x(int slot) = 0;
+  virtual long long slot_n(int slot) const = 0;
+  virtual int dtype() const = 0;
+  virtual hipStream_t stream() const = 0;
   bool ok = false;
 };
 
@@ -426,6 +431,16 @@ struct Arena final : BatchBase {
     }
     return 0;
   }
+  void* slot_x(int slot) override {
+    if (slot < 0 || slot >= maxp || !slots[slot].used) return nullptr;
+    return x + slots[slot].off;
+  }
+  long long slot_n(int slot) const override {
+    if (slot < 0 || slot >= maxp || !slots[slot].used) return -1;
+    return slots[slot].n;
+  }
+  int dtype() const override { return sizeof(T) == 8 ? 1 : 0; }
+  hipStream_t stream() const override { return st; }
   int get_x(int slot, void* out) override {
     if (slot < 0 || slot >= maxp || !slots[slot].used) return -1;
     BCHK(hipMemcpyAsync(out, x + slots[slot].off, (size_t)slots[slot].n * sizeof(T),
@@ -529,6 +544,20 @@ int lbfgsb_batch_clocks(lbfgsb_batch* b, long long* start, long long* end,
 int lbfgsb_batch_get_x(lbfgsb_batch* b, int slot, void* out) {
   if (!b) return -1;
   return b->b->get_x(slot, out);
+}
+
+int lbfgsb_batch_warm_start(lbfgsb_batch* dst, int dst_slot, lbfgsb_batch* src,
+                            int src_slot) {
+  if (!dst || !src) return -1;
+  void* xd = dst->b->slot_x(dst_slot);
+  const void* xs = src->b->slot_x(src_slot);
+  const long long n = dst->b->slot_n(dst_slot);
+  if (!xd || !xs || n != src->b->slot_n(src_slot)) return -1;
+  // the source's iterate is complete on its stream before the copy reads it
+  if (hipStreamSynchronize(src->b->stream()) != hipSuccess) return -2;
+  hipStream_t st = dst->b->stream();
+  if (launch_convert_copy(st, xs, src->b->dtype(), xd, dst->b->dtype(), n) != 0) return -2;
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : -2;
 }
 
 int lbfgsb_batch_remove(lbfgsb_batch* b, int slot) {

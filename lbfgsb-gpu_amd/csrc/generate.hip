@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.hpp"
 #include "generate.hpp"
 #include "objectives.hpp"
 
@@ -137,5 +138,34 @@ template int launch_generate<float>(hipStream_t, const GenSpec&, float*, int*, f
                                     float*, float*, float*, float*, float*);
 template int launch_generate<double>(hipStream_t, const GenSpec&, double*, int*, double*,
                                      double*, double*, double*, double*, double*, double*);
+
+template <typename S, typename D>
+__global__ void __launch_bounds__(kBlock)
+k_convert_copy(const S* __restrict__ src, D* __restrict__ dst, long long n) {
+  for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n;
+       i += (long long)gridDim.x * kBlock)
+    dst[i] = (D)src[i];
+}
+
+int launch_convert_copy(hipStream_t s, const void* src, int src_dtype, void* dst, int dst_dtype,
+                        long long n) {
+  if (n <= 0) return 0;
+  long long blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > kMaxBlocks) blocks = kMaxBlocks;
+  const dim3 g((unsigned)blocks), b(kBlock);
+  if (src_dtype == 0 && dst_dtype == 1)
+    hipLaunchKernelGGL((k_convert_copy<float, double>), g, b, 0, s, (const float*)src,
+                       (double*)dst, n);
+  else if (src_dtype == 1 && dst_dtype == 0)
+    hipLaunchKernelGGL((k_convert_copy<double, float>), g, b, 0, s, (const double*)src,
+                       (float*)dst, n);
+  else if (src_dtype == 1)
+    hipLaunchKernelGGL((k_convert_copy<double, double>), g, b, 0, s, (const double*)src,
+                       (double*)dst, n);
+  else
+    hipLaunchKernelGGL((k_convert_copy<float, float>), g, b, 0, s, (const float*)src,
+                       (float*)dst, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 
 }  // namespace lbfgsb

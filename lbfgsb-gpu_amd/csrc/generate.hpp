@@ -25,4 +25,9 @@ template <typename T>
 int launch_generate(hipStream_t s, const GenSpec& sp, T* x0, int* nbd, T* l, T* u, T* b, T* c,
                     T* a0, T* a1, T* a2);
 
+// dst[i] = src[i] for n elements, converting between the bulk types (dtype
+// 0 float, 1 double): a batch slot started from another slot's iterate.
+int launch_convert_copy(hipStream_t s, const void* src, int src_dtype, void* dst, int dst_dtype,
+                        long long n);
+
 }  // namespace lbfgsb

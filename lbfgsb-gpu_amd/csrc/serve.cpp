@@ -110,6 +110,14 @@ extern "C" int lbfgsb_serve_stream(const lbfgsb_serve_config* cfg, const lbfgsb_
   st.arena_bytes = held;
   int next = 0, answered = 0, rc = 0;
   const double t0 = now_s();
+  // fp32 answers waiting for their FP64 polish: the fp32 slot is held until
+  // the polish is admitted (its iterate is the polish's start)
+  struct Held {
+    int r, slot, iter;
+  };
+  std::deque<Held> polishq;
+  std::vector<int> polished(count, 0), iter32(count, 0);
+  const bool polish = cfg->polish != 0 && lane[1].b != nullptr;
   auto answer = [&](int r, double f, int it, int info, int gang, double t_admit) {
     lbfgsb_response& o = out[r];
     o.f = f;
@@ -120,6 +128,8 @@ extern "C" int lbfgsb_serve_stream(const lbfgsb_serve_config* cfg, const lbfgsb_
     o.t_arrival = reqs[r].arrival;
     o.t_admit = t_admit;
     o.t_done = now_s() - t0;
+    o.polished = polished[r];
+    o.iter_fp32 = iter32[r];
     answered++;
   };
   std::vector<double> admit_time(count, 0.0);
@@ -138,7 +148,31 @@ extern "C" int lbfgsb_serve_stream(const lbfgsb_serve_config* cfg, const lbfgsb_
       else
         lane[d].waiting.push_back(r);
     }
-    // admissions, first come first served per precision
+    // polishes first (their requests arrived before any waiting one), then
+    // arrivals, first come first served per precision
+    while (!polishq.empty()) {
+      const Held h = polishq.front();
+      const lbfgsb_request& q = reqs[h.r];
+      Lane& P = lane[1];
+      // the polish has the iterations the fp32 phase left of the request's cap
+      const int left = std::max(q.max_iteration - h.iter, 1);
+      const int slot = lbfgsb_batch_add_generated(P.b, q.kind, q.n, q.nx, q.ny, q.seed, q.eps,
+                                                  q.eps, q.eps, left);
+      if (slot == -3) break;  // the fp64 arena is full: wait for completions
+      polishq.pop_front();
+      if (slot < 0 || lbfgsb_batch_warm_start(P.b, slot, lane[0].b, h.slot) != 0) {
+        rc = -2;
+        break;
+      }
+      lbfgsb_batch_remove(lane[0].b, h.slot);
+      lane[0].req_of[h.slot] = -1;
+      lane[0].resident--;
+      P.req_of[slot] = h.r;
+      P.resident++;
+      polished[h.r] = 1;
+      iter32[h.r] = h.iter;
+      gang_of[h.r] = lbfgsb_batch_gang(P.b, slot);
+    }
     for (int d = 0; d < 2; d++) {
       Lane& L = lane[d];
       while (!L.waiting.empty()) {
@@ -191,7 +225,16 @@ extern "C" int lbfgsb_serve_stream(const lbfgsb_serve_config* cfg, const lbfgsb_
       for (int s = 0; s < L.slots; s++) {
         const int r = L.req_of[s];
         if (r < 0 || !L.done[s]) continue;
-        answer(r, L.f[s], L.iter[s], L.info[s], gang_of[r], admit_time[r]);
+        if (d == 0 && polish && L.info[s] >= 0) {
+          const long long npad = ((reqs[r].n + 63) / 64) * 64;
+          if (npad <= lane[1].cap) {
+            bool queued = false;
+            for (const Held& h : polishq) queued = queued || h.slot == s;
+            if (!queued) polishq.push_back({r, s, L.iter[s]});
+            continue;  // answered after the polish; the slot stays held
+          }
+        }
+        answer(r, L.f[s], L.iter[s] + iter32[r], L.info[s], gang_of[r], admit_time[r]);
         lbfgsb_batch_remove(L.b, s);
         L.req_of[s] = -1;
         L.resident--;

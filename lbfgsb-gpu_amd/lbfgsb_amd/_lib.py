@@ -65,14 +65,14 @@ class Request(C.Structure):
 class Response(C.Structure):
     _fields_ = [("f", C.c_double), ("iter", C.c_int), ("info", C.c_int), ("gang", C.c_int),
                 ("dtype", C.c_int), ("t_arrival", C.c_double), ("t_admit", C.c_double),
-                ("t_done", C.c_double)]
+                ("t_done", C.c_double), ("polished", C.c_int), ("iter_fp32", C.c_int)]
 
 
 class ServeConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_double), ("fp32_share", C.c_double),
                 ("max_slots", C.c_int), ("threads", C.c_int), ("blocks", C.c_int),
                 ("segment", C.c_longlong), ("max_gang", C.c_int), ("quantum", C.c_int),
-                ("use_graph", C.c_int), ("pad", C.c_int)]
+                ("use_graph", C.c_int), ("polish", C.c_int)]
 
 
 class ServeStats(C.Structure):
@@ -147,6 +147,7 @@ SIGNATURES = {
     "lbfgsb_batch_clocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.POINTER(C.c_double)]),
     "lbfgsb_batch_get_x": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "lbfgsb_batch_warm_start": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "lbfgsb_batch_remove": (C.c_int, [C.c_void_p, C.c_int]),
     "lbfgsb_batch_free_elems": (C.c_longlong, [C.c_void_p]),
     "lbfgsb_batch_configure": (C.c_int, [C.c_void_p, C.c_longlong, C.c_int]),

@@ -4,9 +4,10 @@ solve requests served with continuous batching under an HBM budget.
     reqs = problems.serving_stream(200, qps=5.0)
     out, stats = serve(reqs, budget_bytes=200e9)
 
-`out` is a list of dicts (f, iter, info, gang, dtype and the arrival,
-admission and completion times in seconds from the start of the stream);
-latency = t_done - t_arrival.
+`out` is a list of dicts (f, iter, info, gang, dtype, the arrival,
+admission and completion times in seconds from the start of the stream, and
+with polish=True whether the answer is the FP64 polish of an fp32 solve and
+the fp32 phase's iterations); latency = t_done - t_arrival.
 """
 from __future__ import annotations
 
@@ -26,7 +27,7 @@ def _torch():
 
 def serve(requests, budget_bytes: float = 200e9, fp32_share: float = 0.7, max_slots: int = 4096,
           threads: int = 256, blocks: int = 0, segment: int = 0, max_gang: int = 64,
-          quantum: int = 20, use_graph: bool = True):
+          quantum: int = 20, use_graph: bool = True, polish: bool = False):
     torch = _torch()
     L = _lib.lib()
     cnt = len(requests)
@@ -43,7 +44,8 @@ def serve(requests, budget_bytes: float = 200e9, fp32_share: float = 0.7, max_sl
     cfg = _lib.ServeConfig(hbm_budget_bytes=float(budget_bytes), fp32_share=float(fp32_share),
                            max_slots=int(max_slots), threads=int(threads), blocks=int(blocks),
                            segment=int(segment), max_gang=int(max_gang) if segment else 1,
-                           quantum=int(quantum), use_graph=1 if use_graph else 0)
+                           quantum=int(quantum), use_graph=1 if use_graph else 0,
+                           polish=1 if polish else 0)
     st = _lib.ServeStats()
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     rc = L.lbfgsb_serve_stream(C.byref(cfg), arr, cnt, out, C.byref(st), stream)
@@ -52,7 +54,8 @@ def serve(requests, budget_bytes: float = 200e9, fp32_share: float = 0.7, max_sl
     if rc != 0:
         raise RuntimeError(f"lbfgsb_serve_stream failed ({rc})")
     res = [dict(f=o.f, iter=o.iter, info=o.info, gang=o.gang, dtype=o.dtype,
-                t_arrival=o.t_arrival, t_admit=o.t_admit, t_done=o.t_done)
+                t_arrival=o.t_arrival, t_admit=o.t_admit, t_done=o.t_done,
+                polished=o.polished, iter_fp32=o.iter_fp32)
            for o in out[:cnt]]
     stats = dict(seconds=st.seconds, launches=st.launches, arena_bytes=st.arena_bytes,
                  max_waiting=st.max_waiting, max_resident=st.max_resident)

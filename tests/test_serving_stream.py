@@ -29,7 +29,8 @@ def test_stream_is_deterministic_and_shaped():
             assert r["nx"] * r["ny"] == r["n"]
         if r["kind"] == pb.OBJ_ROSEN:
             assert r["n"] % 2 == 0
-        assert (r["eps"], r["max_iteration"]) == pb.KIND_SETTINGS[r["kind"]]
+        assert r["eps"] == pb.SERVE_EPS
+        assert r["max_iteration"] == pb.KIND_SETTINGS[r["kind"]][1]
     assert len({r["seed"] for r in a}) == 2000
 
 
