@@ -85,11 +85,24 @@ def b_alg_per_iter(n: int, s: int) -> float:
 ALL_STEPS = 0xFFFFFFFF
 
 
-def run_gpu(n: int, f64: bool, warmup: int, steps: int, step_mask: int = ALL_STEPS):
+def b_alg_canonical(n: int, s: int) -> float:
+    """The canonical per-iteration byte count of the plain path's hot-path
+    contract, n (76 s + 32): every phase's reads and writes under maximal
+    fusion with int32 class and index arrays, against which the builder's
+    own count n (70 s + 5) (fewer stored streams, a one-byte class) is
+    reported beside it."""
+    return float(n) * (76 * s + 32)
+
+
+def run_gpu(n: int, f64: bool, warmup: int, steps: int, step_mask: int = ALL_STEPS,
+            problem=None, eps: float = 0.0, converge: bool = False):
     """A plain solve of `warmup + steps` iterations; the last `steps` are
     timed by the host clock.  HIP events on the engine stream bracket the
     launches of the steps in step_mask over the timed iterations (each timed
-    step adds two event records per iteration, a few us each)."""
+    step adds two event records per iteration, a few us each).  problem:
+    the workload (default extended Rosenbrock of size n); converge: run to
+    the solve's own end with tolerance eps and max_iteration = warmup +
+    steps, returning the final x as well."""
     import numpy as np
     import torch
     from lbfgsb_amd import _lib, solver
@@ -97,7 +110,8 @@ def run_gpu(n: int, f64: bool, warmup: int, steps: int, step_mask: int = ALL_STE
 
     L = _lib.lib()
     dtype = np.float64 if f64 else np.float32
-    p = pb.rosenbrock(n)
+    p = pb.rosenbrock(n) if problem is None else problem
+    n = p.n
     dp = solver.DeviceProblem.create(p, dtype)
     x = dp.tensors["x0"].clone()
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -110,7 +124,7 @@ def run_gpu(n: int, f64: bool, warmup: int, steps: int, step_mask: int = ALL_STE
     L.lbfgsb_engine_set_profiling_steps(eng, warmup, step_mask)
     opt = _lib.OptionsF64() if f64 else _lib.OptionsF32()
     (L.lbfgsb_default_options_f64 if f64 else L.lbfgsb_default_options_f32)(C.byref(opt))
-    opt.eps_g = opt.eps_f = opt.eps_x = 0.0
+    opt.eps_g = opt.eps_f = opt.eps_x = eps
     opt.max_iteration = warmup + steps
     cbs = _lib.CallbacksF64()
     cbs.funcgrad = C.cast(L.lbfgsb_problem_funcgrad_f64 if f64 else
@@ -126,40 +140,183 @@ def run_gpu(n: int, f64: bool, warmup: int, steps: int, step_mask: int = ALL_STE
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     iters = summ.num_iteration
-    if iters < warmup + steps:
-        raise RuntimeError(f"solve stopped early: iters={iters} info={summ.info}")
-    timed = clock[warmup + steps - 1] - clock[warmup - 1]
+    out = dict(iters=iters, wall=wall, info=summ.info)
+    if converge:
+        L.lbfgsb_engine_finalize(eng)
+        torch.cuda.synchronize()
+        out["x"] = x.double().cpu().numpy()
+        k0 = min(warmup, max(iters - 1, 1))
+        out["timed"] = clock[iters - 1] - clock[k0 - 1] if iters > k0 else wall
+        out["timed_iters"] = max(iters - k0, 1)
+    else:
+        if iters < warmup + steps:
+            raise RuntimeError(f"solve stopped early: iters={iters} info={summ.info}")
+        out["timed"] = clock[warmup + steps - 1] - clock[warmup - 1]
     kern = {}
     for name, sid in STEPS.items():
         ms, cnt = C.c_double(0), C.c_longlong(0)
         L.lbfgsb_engine_step_time(eng, sid, C.byref(ms), C.byref(cnt))
         kern[name] = (ms.value, cnt.value)
+    out["kern"] = kern
     L.lbfgsb_engine_destroy(eng)
     dp.close()
-    return dict(timed=timed, iters=iters, wall=wall, kern=kern, info=summ.info)
+    return out
 
 
-def run_cpu_baseline(n_target: int, n_sample: int, warm: int = 9, timed: int = 12):
-    """The oracle (a 1-thread C restatement of the reference CPU path) on the
-    same generator at n_sample, steady-state iterations (col = m) timed and
-    the per-iteration cost scaled linearly to n_target."""
+def run_plain_configs(a):
+    """The plain-form workloads beside the headline one, on the GPU:
+      * the 256 x 256 fp64 grid quadratic, eps 1e-8, 1000 iterations, solved
+        to its end: microseconds per iteration (the state is cache-resident,
+        so latency, not bandwidth, bounds it);
+      * the 2048 x 2048 cos-energy grid with mixed bounds, fp32 and fp64:
+        steady-state iterations per second and the iteration's roofline,
+        then the solve to its end (tol 1e-7, 2000 iterations);
+      * the 16.8M-variable Rosenbrock solved to its end in fp32 and fp64
+        (eps 1e-10, 1000 iterations): final f and |x - x*|_inf (x* = 1),
+        and the fp32 answer against the fp64 one."""
+    import numpy as np
+    from lbfgsb_amd import problems as pb
+
+    out = {}
+    r = run_gpu(0, True, 10, 990, ALL_STEPS & 0, problem=pb.grid_quadratic(256), eps=1e-8,
+                converge=True)
+    f0, _ = pb.objective_numpy(pb.grid_quadratic(256), r["x"])
+    out["grid_quadratic_256_fp64"] = {
+        "workload": "plain, 256x256 grid quadratic (n=65,536), fp64, eps 1e-8, max 1000",
+        "iterations": r["iters"], "info": r["info"], "f": f0,
+        "us_per_iteration": r["timed"] / r["timed_iters"] * 1e6,
+        "us_per_iteration_whole_solve": r["wall"] / max(r["iters"], 1) * 1e6}
+    print(f"plain: grid quadratic 256^2 fp64 {out['grid_quadratic_256_fp64']['us_per_iteration']:.1f}"
+          f" us/iteration over {r['iters']} iterations", file=sys.stderr, flush=True)
+    gc = pb.grid_cos(2048)
+    n1 = gc.n
+    for f64 in (False, True):
+        s = 8 if f64 else 4
+        t = run_gpu(0, f64, a.warmup, a.steps, 0, problem=gc)
+        t_it = t["timed"] / a.steps
+        sol = run_gpu(0, f64, 10, 1990, 0, problem=gc, eps=1e-7, converge=True)
+        fsol, _ = pb.objective_numpy(gc, sol["x"])
+        out[f"grid_cos_2048_{'fp64' if f64 else 'fp32'}"] = {
+            "workload": "plain, 2048x2048 cos-energy grid (n=4,194,304), mixed nbd, "
+                        + ("fp64" if f64 else "fp32 bulk / fp64 small dense + line search"),
+            "iters_per_s": a.steps / t["timed"], "ms_per_iteration": t_it * 1e3,
+            "iteration_roofline_frac": b_alg_per_iter(n1, s) / t_it / 1e9 / HBM_PEAK_GBS,
+            "iteration_roofline_frac_canonical":
+                b_alg_canonical(n1, s) / t_it / 1e9 / HBM_PEAK_GBS,
+            "solve": {"tol": 1e-7, "iterations": sol["iters"], "info": sol["info"], "f": fsol}}
+        print(f"plain: grid cos 2048^2 {'fp64' if f64 else 'fp32'} {a.steps / t['timed']:.0f} "
+              f"iters/s", file=sys.stderr, flush=True)
+    g32, g64 = out["grid_cos_2048_fp32"]["solve"], out["grid_cos_2048_fp64"]["solve"]
+    g32["f_rel_error_vs_fp64"] = abs(g32["f"] - g64["f"]) / abs(g64["f"])
+    rs = {}
+    for f64 in (False, True):
+        sol = run_gpu(a.n, f64, 10, 990, 0, eps=1e-10, converge=True)
+        x = sol["x"]
+        fx, _ = pb.objective_numpy(pb.rosenbrock(a.n), x)
+        rs["fp64" if f64 else "fp32"] = dict(x=x, f=fx, iterations=sol["iters"], info=sol["info"])
+    out["rosenbrock_16M_accuracy"] = {
+        "workload": f"plain, extended Rosenbrock n={a.n:,}, solved to eps 1e-10 "
+                    f"(max 1000 iterations); x* = 1, f* = 0",
+        **{k: {"f": v["f"], "iterations": v["iterations"], "info": v["info"],
+               "x_err_inf_vs_xstar": float(np.max(np.abs(v["x"] - 1.0)))} for k, v in rs.items()},
+        "fp32_vs_fp64": {"f_abs_diff": abs(rs["fp32"]["f"] - rs["fp64"]["f"]),
+                         "x_diff_inf": float(np.max(np.abs(rs["fp32"]["x"] - rs["fp64"]["x"])))}}
+    return out
+
+
+def _cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_child(job: str, arg: int) -> dict:
+    """One CPU timing, run in a child pinned to one core (see run_cpu_baseline)."""
     import numpy as np
     from lbfgsb_amd import problems as pb
     from oracle import pyoracle
 
-    p = pb.rosenbrock(n_sample)
-    s = pyoracle.Solver(p, dtype=np.float32, eps=0.0, max_iter=warm + timed + 2)
-    for _ in range(warm):
-        s.step()
+    if job in ("rosen32", "rosen64", "gridcos32", "gridcos64", "gq256"):
+        dt = np.float64 if job.endswith("64") or job == "gq256" else np.float32
+        p = {"rosen32": pb.rosenbrock, "rosen64": pb.rosenbrock}.get(job)
+        p = p(arg) if p else (pb.grid_quadratic(256) if job == "gq256" else pb.grid_cos(arg))
+        warm, timed = (9, 12) if job.startswith("rosen") else ((9, 4) if job.startswith("gridcos")
+                                                                else (10, 60))
+        s = pyoracle.Solver(p, dtype=dt, eps=0.0, max_iter=warm + timed + 2)
+        for _ in range(warm):
+            s.step()
+        t0 = time.perf_counter()
+        for _ in range(timed):
+            s.step()
+        return {"per_iter_s": (time.perf_counter() - t0) / timed, "n": p.n,
+                "iterations": f"{warm + 1}-{warm + timed}"}
+    if job == "batched":
+        specs = pb.batched_qp_specs(4096)
+        idx = np.random.default_rng(5).choice(len(specs), size=arg, replace=False)
+        t0 = time.perf_counter()
+        for k in idx:
+            r = specs[k]
+            pyoracle.minimize(pb.penta_qp(r["n"], seed=r["seed"]), dtype=np.float64,
+                              eps=r["eps"], max_iter=r["max_iteration"])
+        return {"seconds": time.perf_counter() - t0, "solves": int(arg)}
+    raise ValueError(job)
+
+
+def _cpu_pinned(job: str, arg: int, core: int) -> dict:
+    import subprocess
+    code = (f"import sys, json; sys.path[:0] = {[REPO, os.path.join(REPO, 'lbfgsb-gpu_amd')]!r}; "
+            f"import bench; print(json.dumps(bench._cpu_child({job!r}, {arg})))")
+    cmd = [sys.executable, "-c", code]
+    if os.path.exists("/usr/bin/taskset"):
+        cmd = ["/usr/bin/taskset", "-c", str(core)] + cmd
+    r = subprocess.run(cmd, capture_output=True, text=True, check=True)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def run_cpu_baseline(n_target: int, n_sample: int, workers: int = 16):
+    """The oracle (a 1-thread C restatement of the reference CPU path, built
+    -O2 -ffp-contract=off) timed on this host, each timing in a child
+    process pinned with taskset to one core: steady-state iterations
+    (col = 8) of the Rosenbrock workload at n_sample, scaled linearly to
+    n_target, fp32 and fp64; the 256 x 256 grid quadratic fp64 and the
+    2048 x 2048 cos grid per iteration at their full sizes; and solves/s
+    of the batched QP workload on a sample of 48 problems with `workers`
+    processes, one pinned per core."""
+    import concurrent.futures as cf
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else [0]
+    r32 = _cpu_pinned("rosen32", n_sample, cores[0])
+    r64 = _cpu_pinned("rosen64", n_sample, cores[0])
+    scale = n_target / n_sample
+    out = dict(value=1.0 / (r32["per_iter_s"] * scale), unit="iters/s", cores=1, kind="port",
+               sample=(f"oracle fp32, rosenbrock n={n_sample:,}, iterations "
+                       f"{r32['iterations']} (col=8) timed on one pinned core: "
+                       f"{r32['per_iter_s'] * 1e3:.1f} ms/iter, scaled x{scale:g} to "
+                       f"n={n_target:,}"),
+               model=_cpu_model(), host_cpus=ncpu,
+               fp64={"value": 1.0 / (r64["per_iter_s"] * scale), "unit": "iters/s",
+                     "sample_ms_per_iter": r64["per_iter_s"] * 1e3})
+    gq = _cpu_pinned("gq256", 0, cores[0])
+    out["grid_quadratic_256_fp64_us_per_iteration"] = gq["per_iter_s"] * 1e6
+    for dt in ("32", "64"):
+        gcr = _cpu_pinned("gridcos" + dt, 2048, cores[0])
+        out[f"grid_cos_2048_fp{dt}_iters_per_s"] = 1.0 / gcr["per_iter_s"]
+    P = max(1, min(workers, ncpu))
+    per = 48 // P if 48 >= P else 1
     t0 = time.perf_counter()
-    for _ in range(timed):
-        s.step()
-    per_iter = (time.perf_counter() - t0) / timed
-    scaled = per_iter * (n_target / n_sample)
-    return dict(value=1.0 / scaled, unit="iters/s", cores=1, kind="port",
-                sample=(f"oracle fp32, rosenbrock n={n_sample:,}, iterations "
-                        f"{warm + 1}-{warm + timed} (col=8) timed: {per_iter * 1e3:.1f} ms/iter, "
-                        f"scaled x{n_target / n_sample:g} to n={n_target:,}"))
+    with cf.ThreadPoolExecutor(max_workers=P) as ex:
+        futs = [ex.submit(_cpu_pinned, "batched", per, cores[k % len(cores)]) for k in range(P)]
+        res = [f.result() for f in futs]
+    wall = time.perf_counter() - t0
+    out["batched_qp_solves_per_s"] = {"value": sum(r["solves"] for r in res) / wall,
+                                      "processes": P, "solves": sum(r["solves"] for r in res),
+                                      "note": "fp64 oracle, one pinned process per core, "
+                                              "problems drawn from the 4096-problem workload"}
+    return out
 
 
 def run_batched(count: int, plain_sample: int = 32):
@@ -362,6 +519,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2 ** 22)
+    ap.add_argument("--no-plain-configs", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-count", type=int, default=4096)
     ap.add_argument("--no-serving", action="store_true")
@@ -412,7 +570,12 @@ def main():
         "iteration_roofline": {
             "b_alg_bytes": b_alg_per_iter(n, s),
             "achieved": b_alg_per_iter(n, s) / t_iter / 1e9, "unit": "GB/s",
-            "frac": b_alg_per_iter(n, s) / t_iter / 1e9 / HBM_PEAK_GBS},
+            "frac": b_alg_per_iter(n, s) / t_iter / 1e9 / HBM_PEAK_GBS,
+            "b_alg_canonical_bytes": b_alg_canonical(n, s),
+            "frac_canonical": b_alg_canonical(n, s) / t_iter / 1e9 / HBM_PEAK_GBS,
+            "note": "b_alg: the bytes this engine's passes move per iteration, n(70s+5); "
+                    "canonical: the hot-path contract's count n(76s+32), which assumes "
+                    "int32 class/index arrays and more stored streams"},
         "kernels_us": {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in p32["kern"].items()},
         "kernel_launches": {k: v[1] for k, v in p32["kern"].items()},
         "kernels_profiled_run": {"steps": prof_steps,
@@ -429,8 +592,12 @@ def main():
         out["fp64"] = {"value": a.steps / r64["timed"], "unit": "iters/s",
                        "ms_per_step": t64 * 1e3,
                        "iteration_roofline_frac": b_alg_per_iter(n, 8) / t64 / 1e9 / HBM_PEAK_GBS,
+                       "iteration_roofline_frac_canonical":
+                           b_alg_canonical(n, 8) / t64 / 1e9 / HBM_PEAK_GBS,
                        "kernels_us": {k: round(v[0] * 1e3 / max(v[1], 1), 2)
                                       for k, v in p64["kern"].items()}}
+    if not a.no_plain_configs:
+        out["plain_configs"] = run_plain_configs(a)
     if not a.no_batched:
         out["batched"] = run_batched(a.batched_count)
     if not a.no_serving:
