@@ -1,22 +1,21 @@
-// Example of the C++ drop-in API: an unconstrained 2-D grid energy with an
-// exponential source, minimised in float and in double through
-// lbfgsbcuda::lbfgsbminimize with a user callback that runs its own HIP
-// kernels on the solve's stream.  Same program shape as the reference's
-// examples/dsscfg/dsscfg_main.cpp:82-200 (options, state with a funcgrad
-// lambda, device x/nbd/l/u, summary printout, CPU-vs-GPU style check), with
-// the check made between the two precisions instead.
-//
-//   f(v) = 1/2 v'A v - h^2 lambda sum_ij exp(v_ij),   g = A v - h^2 lambda exp(v)
-//
-// A is the 5-point Laplacian on an nx x ny grid with zero Dirichlet values
-// (4 on the diagonal, -1 to each neighbour), h = 1/(nx+1), lambda = 1.  For
-// lambda below the critical value (about 6.8) the energy has a local minimum
-// reached from v = 0.
+// Example of the C++ drop-in API on the reference's own example problem:
+// MINPACK-2 dsscfg (steady-state combustion) on a 256 x 256 grid with
+// lambda = 1, unconstrained, from the problem's standard starting point,
+// minimised in double and in float through lbfgsbcuda::lbfgsbminimize with a
+// user callback that runs its own HIP kernels on the solve's stream.  The
+// program follows examples/dsscfg/dsscfg_main.cpp:82-234: options
+// (eps 1e-8, 1000 iterations), a state with a funcgrad lambda that tracks
+// the least f it sees, device x/nbd/l/u, ms per iteration, and the check
+// that two solves agree on min f within 1e-4 (double) / 1e-3 (float); here
+// the two are the GPU's double and float solves (the reference's CPU
+// solve is the oracle, compared by tests/test_cpp_api.py).
 //
 //   dsscfg_example [nx] [ny]       exit status 0 when both solves converge
 //                                  and agree
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -35,25 +34,40 @@
 
 constexpr int kThreads = 256;
 
+// MINPACK-2 dsscfg energy and gradient at v (examples/dsscfg/dsscfg_cpu.cpp
+// evaluates the same discretisation on the CPU by scattering each
+// triangle's terms; here each grid point gathers its gradient from the
+// triangles that touch it, so no atomics are needed).  Point k = nx*j + i
+// owns its lower triangle (v, right, top) and upper one (v, left, bottom);
+// values outside the grid are 0.  FP64 arithmetic, g rounded to `real`.
 template <typename real>
 __global__ void __launch_bounds__(kThreads)
-energy_kernel(int nx, int ny, double h2l, const real* __restrict__ v, real* __restrict__ g,
+energy_kernel(int nx, int ny, double lambda, const real* __restrict__ v, real* __restrict__ g,
               double* __restrict__ part) {
   __shared__ double red[kThreads / 64];
   const long long n = (long long)nx * ny;
+  const double hx = 1.0 / (nx + 1), hy = 1.0 / (ny + 1), area = 0.5 * hx * hy;
   double acc = 0.0;
   for (long long k = blockIdx.x * (long long)kThreads + threadIdx.x; k < n;
        k += (long long)gridDim.x * kThreads) {
     const int i = (int)(k % nx), j = (int)(k / nx);
     const double c = v[k];
-    double av = 4.0 * c;
-    if (i > 0) av -= v[k - 1];
-    if (i + 1 < nx) av -= v[k + 1];
-    if (j > 0) av -= v[k - nx];
-    if (j + 1 < ny) av -= v[k + nx];
-    const double e = h2l * std::exp(c);
-    g[k] = (real)(av - e);
-    acc += 0.5 * c * av - e;
+    const double r = i + 1 < nx ? (double)v[k + 1] : 0.0;
+    const double t = j + 1 < ny ? (double)v[k + nx] : 0.0;
+    const double lft = i > 0 ? (double)v[k - 1] : 0.0;
+    const double b = j > 0 ? (double)v[k - nx] : 0.0;
+    const double dxl = (r - c) / hx, dyl = (t - c) / hy;   // lower triangle
+    const double dxu = (c - lft) / hx, dyu = (c - b) / hy; // upper triangle
+    const double ec = std::exp(c), le = lambda * ec / 3.0;
+    double gk = (-dxl / hx - dyl / hy - le) + (dxu / hx + dyu / hy - le);
+    if (j + 1 < ny) gk += -dyl / hy - le;  // upper triangle of the point above
+    if (i + 1 < nx) gk += -dxl / hx - le;  // upper triangle of the point right
+    if (i > 0) gk += dxu / hx - le;        // lower triangle of the point left
+    if (j > 0) gk += dyu / hy - le;        // lower triangle of the point below
+    g[k] = (real)(gk * area);
+    acc += area * (0.5 * (dxl * dxl + dyl * dyl + dxu * dxu + dyu * dyu) -
+                   lambda * (ec + std::exp(r) + std::exp(t) + std::exp(b) + std::exp(lft) + ec) /
+                       3.0);
   }
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
@@ -81,14 +95,14 @@ __global__ void __launch_bounds__(kThreads) sum_kernel(const double* part, int n
 
 struct Grid {
   int nx, ny, nb;
-  double h2l;
+  double lambda;
   double* part;
   double* fdev;
 };
 
 template <typename real>
 double evaluate(const Grid& p, const real* x, real* g, hipStream_t s) {
-  energy_kernel<real><<<p.nb, kThreads, 0, s>>>(p.nx, p.ny, p.h2l, x, g, p.part);
+  energy_kernel<real><<<p.nb, kThreads, 0, s>>>(p.nx, p.ny, p.lambda, x, g, p.part);
   sum_kernel<<<1, kThreads, 0, s>>>(p.part, p.nb, p.fdev);
   double f = 0.0;
   CHECK(hipMemcpyAsync(&f, p.fdev, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -110,12 +124,14 @@ bool run(const Grid& p, double* f_out, int* iters_out) {
   LBFGSB_CUDA_STATE<real> state;
   state.m_cublas_handle = nullptr;
   int evals = 0;
+  double min_f = 1e300;
   state.m_funcgrad_callback = [&](real* x, real& f, real* g, const hipStream_t& stream,
                                   const LBFGSB_CUDA_SUMMARY<real>& summary) {
     f = static_cast<real>(evaluate<real>(p, x, g, stream));
     if (summary.num_iteration % 100 == 0 && evals++ % 4 == 0)
       std::printf("  [%s] iteration %d f %.10g\n", sizeof(real) == 8 ? "double" : "float",
                   summary.num_iteration, (double)f);
+    min_f = std::fmin(min_f, (double)f);
     return 0;
   };
   int after = 0;
@@ -131,13 +147,29 @@ bool run(const Grid& p, double* f_out, int* iters_out) {
   CHECK(hipMalloc(&l, n * sizeof(real)));
   CHECK(hipMalloc(&u, n * sizeof(real)));
   CHECK(hipMalloc(&nbd, n * sizeof(int)));
-  CHECK(hipMemset(x, 0, n * sizeof(real)));
+  // the standard starting point of dsscfg (dsscfg_cpu.cpp:25-37):
+  // x_k = lambda/(lambda+1) sqrt(min(min(i, nx-i) hx, min(j, ny-j) hy))
+  {
+    std::vector<real> x0(n);
+    const double hx = 1.0 / (p.nx + 1), hy = 1.0 / (p.ny + 1);
+    const double c = p.lambda / (p.lambda + 1.0);
+    for (int j = 0; j < p.ny; j++)
+      for (int i = 0; i < p.nx; i++)
+        x0[(size_t)p.nx * j + i] = static_cast<real>(
+            c * std::sqrt(std::fmin(std::min(i, p.nx - i) * hx, std::min(j, p.ny - j) * hy)));
+    CHECK(hipMemcpy(x, x0.data(), n * sizeof(real), hipMemcpyHostToDevice));
+  }
   CHECK(hipMemset(l, 0, n * sizeof(real)));
   CHECK(hipMemset(u, 0, n * sizeof(real)));
   CHECK(hipMemset(nbd, 0, n * sizeof(int)));  // unconstrained
 
   LBFGSB_CUDA_SUMMARY<real> summary;
+  CHECK(hipDeviceSynchronize());
+  const auto t0 = std::chrono::steady_clock::now();
   lbfgsbcuda::lbfgsbminimize<real>(n, state, opt, x, nbd, l, u, summary);
+  CHECK(hipDeviceSynchronize());
+  const double ms = std::chrono::duration<double, std::milli>(
+                        std::chrono::steady_clock::now() - t0).count();
 
   // projected gradient at the returned point (unconstrained: the gradient)
   real* g;
@@ -147,16 +179,17 @@ bool run(const Grid& p, double* f_out, int* iters_out) {
   CHECK(hipMemcpy(hg.data(), g, n * sizeof(real), hipMemcpyDeviceToHost));
   double gmax = 0.0;
   for (real v : hg) gmax = std::fmax(gmax, std::fabs((double)v));
-  std::printf("%s: info %d, %d iterations (%d after-iteration calls), f %.12g, |g|_inf %.3e, "
-              "residual_g %.3e\n",
+  std::printf("%s: info %d, %d iterations (%d after-iteration calls), f %.12g, min f seen "
+              "%.12g, |g|_inf %.3e, residual_g %.3e, %.3f ms / iteration\n",
               sizeof(real) == 8 ? "double" : "float", summary.info, summary.num_iteration, after,
-              f, gmax, (double)summary.residual_g);
+              f, min_f, gmax, (double)summary.residual_g,
+              ms / std::max(summary.num_iteration, 1));
   CHECK(hipFree(g));
   CHECK(hipFree(x));
   CHECK(hipFree(l));
   CHECK(hipFree(u));
   CHECK(hipFree(nbd));
-  *f_out = f;
+  *f_out = min_f;  // the least f the callback saw, as the reference's check takes
   *iters_out = summary.num_iteration;
   // the eps_f / eps_x tests usually end the solve first; at those the
   // gradient of this problem is ~1e-5
@@ -172,22 +205,23 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "grid too small\n");
     return 2;
   }
-  const double h = 1.0 / (p.nx + 1);
-  p.h2l = h * h * 1.0;
+  p.lambda = 1.0;
   const long long n = (long long)p.nx * p.ny;
   p.nb = (int)std::min<long long>((n + kThreads - 1) / kThreads, 1024);
   CHECK(hipMalloc(&p.part, p.nb * sizeof(double)));
   CHECK(hipMalloc(&p.fdev, sizeof(double)));
-  std::printf("grid %d x %d (n = %lld), lambda = 1, unconstrained\n", p.nx, p.ny, n);
+  std::printf("dsscfg grid %d x %d (n = %lld), lambda = 1, unconstrained\n", p.nx, p.ny, n);
   double f64 = 0, f32 = 0;
   int i64 = 0, i32 = 0;
   const bool ok64 = run<double>(p, &f64, &i64);
   const bool ok32 = run<float>(p, &f32, &i32);
-  const double rel = std::fabs(f64 - f32) / std::fmax(std::fabs(f64), 1e-30);
-  std::printf("min f: double %.12g, float %.12g, relative difference %.3e\n", f64, f32, rel);
+  const double diff = std::fabs(f64 - f32);
+  std::printf("min f: double %.12g, float %.12g, difference %.3e\n", f64, f32, diff);
   CHECK(hipFree(p.part));
   CHECK(hipFree(p.fdev));
-  const bool ok = ok64 && ok32 && rel < 1e-4;
+  // the reference example passes when its two solves agree within 1e-3 in
+  // float (dsscfg_main.cpp:224-231)
+  const bool ok = ok64 && ok32 && diff < 1e-3;
   std::printf("%s\n", ok ? "OK" : "FAILED");
   return ok ? 0 : 1;
 }

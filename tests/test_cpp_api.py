@@ -2,9 +2,9 @@
 
 The header restates the reference's culbfgsb/culbfgsb.h types and the
 lbfgsbcuda::lbfgsbminimize / lbfgsbdefaultoption templates over the C ABI;
-examples/dsscfg.hip follows the shape of the reference's
-examples/dsscfg/dsscfg_main.cpp (user callback with its own HIP kernels on
-the solve's stream, float and double solves, cross-precision check).
+examples/dsscfg.hip is the reference's example on the reference's problem
+(MINPACK-2 dsscfg, examples/dsscfg/dsscfg_main.cpp): a user callback with its
+own HIP kernels on the solve's stream, float and double solves.
 """
 import os
 import subprocess
@@ -53,13 +53,28 @@ def test_example_is_built():
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
-def test_example_runs():
-    """Both precisions converge (|g|_inf below 1e-4 / 1e-3, info in 1..5, one
-    after-iteration call per iteration) and agree on min f within 1e-4
-    relative, the tolerance of the reference example's CPU/GPU check
-    (dsscfg_main.cpp:208-231)."""
-    print("case dsscfg_example 96 x 80", flush=True)
-    r = subprocess.run([EXAMPLE, "96", "80"], capture_output=True, text=True, timeout=120)
+def test_example_runs(oracle):
+    """The reference's only test, on the reference's problem: MINPACK-2
+    dsscfg, 256 x 256, lambda = 1, unconstrained, eps 1e-8, 1000 iterations
+    (examples/dsscfg/dsscfg_main.cpp:21-24, 35-40, 69-80).  As there, the
+    least f the callback sees must agree with the CPU path's within 1e-4 in
+    double and 1e-3 in float (:208-231); the CPU path is the oracle.  The
+    example itself also checks both precisions converge (|g|_inf, info,
+    one after-iteration call per iteration) and agree with each other."""
+    import re
+
+    import numpy as np
+    from lbfgsb_amd import problems as pb
+    print("case dsscfg_example 256 x 256", flush=True)
+    r = subprocess.run([EXAMPLE], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("OK")
+    m = re.search(r"min f: double (\S+), float (\S+),", r.stdout)
+    f64, f32 = float(m.group(1)), float(m.group(2))
+    p = pb.dsscfg(256)
+    for dt, f, tol in ((np.float64, f64, 1e-4), (np.float32, f32, 1e-3)):
+        ref = oracle.Solver(p, dtype=dt, eps=1e-8, max_iter=1000).run()
+        f_cpu = min(rec.f for rec in ref.records)
+        print(np.dtype(dt).name, "example", f, "oracle", f_cpu, "iterations", ref.st.iter)
+        assert abs(f - f_cpu) < tol
