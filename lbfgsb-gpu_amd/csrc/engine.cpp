@@ -101,6 +101,14 @@ struct Engine final : EngineBase {
   size_t region_cap = 0, region_off = 0;
   bool measuring = false;
   DevScalars hs_store;
+  // chains since the last Cauchy round one workgroup could not take
+  static constexpr int kMidHorizon = 16;
+  int handoff_age = kMidHorizon;
+  int last_handoffs = 0;
+  void note_handoffs() {
+    handoff_age = hs->gcp_handoffs != last_handoffs ? 0 : handoff_age + 1;
+    last_handoffs = hs->gcp_handoffs;
+  }
 
   // ---- optional per-step timing (HIP events on st)
   struct Pending {
@@ -352,10 +360,20 @@ struct Engine final : EngineBase {
         launch_pass_a<T>(st, G, true, n, dX, dG, l, u, iwhere, W, d, tbrk, ds, part, pmax,
                          pmin, pmini, 0);
         break;
-      case K_GCP:
+      case K_GCP: {
         launch_gcp_select<T>(st, G, n, tbrk, ds, ca, n);
         launch_gcp_round_small<T>(st, ca, dG, W, ds);
+        // a round too large for one workgroup but within kMidRound finishes
+        // here on the device; only larger ones and later ranges reach the
+        // host (run_rounds).  Enqueued only while such rounds are recent:
+        // when they are not, its idle launches would cost more than they save
+        if (handoff_age < kMidHorizon) {
+          GcpScratch w{tsum, tpre, tstride, total, tile_stop, tile_state, last_state};
+          launch_gcp_sortscan<T>(st, dG, W, ds, ca, cb, std::min<long long>(n, kMidRound), w,
+                                 1);
+        }
         break;
+      }
       case K_STOP:
         launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0, 0, pmax, pmin, pmini);
         launch_stop_tests(st, ds, part, pmax, pmin, pmini, G);
@@ -457,6 +475,7 @@ struct Engine final : EngineBase {
     for (size_t k = i; k < j; k++) enqueue(steps[k], f);
     HIPCHK(hipGetLastError());
     if (read_status()) return -1;
+    note_handoffs();
     if (hs->status == ST_GCP_ROUNDS) {
       hipEvent_t ea = (profile_from >= 0 && hs->iter >= profile_from &&
                        ((profile_mask >> K_END) & 1u))

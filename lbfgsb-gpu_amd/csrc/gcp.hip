@@ -60,6 +60,19 @@ k_gcp_select(long long n, const T* __restrict__ tbrk, DevScalars* ds, Cand* cand
   }
 }
 
+// Whether a launch of the multi-block round does work.  mid = 0: the host
+// sized it (after resuming the chain).  mid = 1: enqueued in the chain right
+// after the one-workgroup round, it takes over a round that one workgroup
+// could not sort (more than kSmallRound candidates, at most its capacity),
+// with no host round trip; every other state leaves it idle (a concluded
+// round, a range still to select, a larger round left to the host).
+__device__ inline bool round_active(const DevScalars* ds, long long cap, int mid) {
+  if (!ds->gcp_pending) return false;
+  if (!mid) return true;
+  const long long K = (long long)ds->cand_count;
+  return ds->status == ST_GCP_ROUNDS && !ds->gcp_need_select && K > kSmallRound && K <= cap;
+}
+
 __device__ inline long long round_size(const DevScalars* ds, long long cap) {
   const long long k = (long long)ds->cand_count;
   return k < cap ? k : cap;
@@ -67,9 +80,9 @@ __device__ inline long long round_size(const DevScalars* ds, long long cap) {
 
 // Sort each kSortTile-long tile of the candidates in LDS (bitonic).
 __global__ void __launch_bounds__(kBlock)
-k_sort_tile(Cand* a, const DevScalars* ds, long long cap) {
+k_sort_tile(Cand* a, const DevScalars* ds, long long cap, int mid) {
   __shared__ Cand sh[kSortTile];
-  if (!ds->gcp_pending) return;
+  if (!round_active(ds, cap, mid)) return;
   const long long K = round_size(ds, cap);
   const long long base = (long long)blockIdx.x * kSortTile;
   if (base >= K) return;
@@ -105,8 +118,8 @@ k_sort_tile(Cand* a, const DevScalars* ds, long long cap) {
 static constexpr int kMergeItems = 4;
 __global__ void __launch_bounds__(kBlock)
 k_merge(const Cand* __restrict__ src, Cand* __restrict__ dst, const DevScalars* ds,
-        long long cap, long long width) {
-  if (!ds->gcp_pending) return;
+        long long cap, long long width, int mid) {
+  if (!round_active(ds, cap, mid)) return;
   const long long K = round_size(ds, cap);
   const long long o0 = ((long long)blockIdx.x * kBlock + threadIdx.x) * kMergeItems;
   if (o0 >= K) return;
@@ -140,9 +153,9 @@ k_merge(const Cand* __restrict__ src, Cand* __restrict__ dst, const DevScalars* 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_gcp_tile_reduce(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__ W,
-                  const DevScalars* ds, long long cap, double* tsum, int tstride) {
+                  const DevScalars* ds, long long cap, double* tsum, int tstride, int mid) {
   __shared__ double sh[kWavesPerBlock * kNV];
-  if (!ds->gcp_pending) return;
+  if (!round_active(ds, cap, mid)) return;
   const long long K = round_size(ds, cap);
   if ((long long)blockIdx.x * kBlock >= K) return;
   const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1, ds->pslot_x);
@@ -168,9 +181,9 @@ k_gcp_tile_reduce(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
 
 // Exclusive scan over tiles (the carried state included) and the round total.
 __global__ void s_gcp_tile_scan(DevScalars* ds, long long cap, const double* tsum,
-                                double* tpre, int tstride, double* total) {
+                                double* tpre, int tstride, double* total, int mid) {
   __shared__ double sh[kWavesPerBlock][kWave];
-  if (!ds->gcp_pending) return;
+  if (!round_active(ds, cap, mid)) return;
   const long long K = round_size(ds, cap);
   const int ntiles = (int)((K + kBlock - 1) / kBlock);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -200,11 +213,11 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_gcp_tile_eval(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__ W,
                 const DevScalars* ds, long long cap, const double* tpre, int tstride,
-                long long* tile_stop, double* tile_state, double* last_state) {
+                long long* tile_stop, double* tile_state, double* last_state, int mid) {
   __shared__ double sh[kWavesPerBlock * kNV];
   __shared__ long long sstop;
   __shared__ GcpConst gc;
-  if (!ds->gcp_pending) return;
+  if (!round_active(ds, cap, mid)) return;
   const long long K = round_size(ds, cap);
   if ((long long)blockIdx.x * kBlock >= K) return;
   stage_gcp_const(ds, gc);
@@ -264,9 +277,9 @@ k_gcp_tile_eval(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__
 // for.
 __global__ void s_gcp_round_end(DevScalars* ds, const Cand* __restrict__ cand, long long cap,
                                 const long long* tile_stop, const double* tile_state,
-                                const double* last_state, const double* total) {
+                                const double* last_state, const double* total, int mid) {
   __shared__ long long sbest;
-  if (!ds->gcp_pending) return;
+  if (!round_active(ds, cap, mid)) return;
   if ((long long)ds->cand_count > cap) {
     // too many candidates for this launch: the host sorts them
     if (threadIdx.x == 0) {
@@ -288,6 +301,17 @@ __global__ void s_gcp_round_end(DevScalars* ds, const Cand* __restrict__ cand, l
   const bool stopped = sbest != 0x7fffffffffffffffLL;
   const double* st = stopped ? tile_state + (size_t)(sbest / kBlock) * kNV : total;
   round_conclude(ds, gc, cand, K, stopped ? sbest : -1, st, last_state);
+  if (mid) {
+    // a concluded search lets the chain go on to the Cauchy-point pass; a
+    // range still to select goes to the host as before
+    if (!ds->gcp_pending) {
+      ds->status = ST_RUNNING;
+      ds->halt = 0;
+    } else {
+      ds->status = ST_GCP_ROUNDS;
+      ds->halt = 1;
+    }
+  }
 }
 
 // A whole round in one workgroup when it has at most kSmallRound candidates
@@ -306,6 +330,7 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
     if (threadIdx.x == 0) {
       ds->status = ST_GCP_ROUNDS;
       ds->halt = 1;
+      ds->gcp_handoffs++;
     }
     return;
   }
@@ -329,30 +354,30 @@ void launch_gcp_select(hipStream_t s, int G, long long n, const T* tbrk, DevScal
 // decides how much of the launch does work).
 template <typename T>
 void launch_gcp_sortscan(hipStream_t s, Pair<T> Gv, const T* W, DevScalars* ds, Cand* ca,
-                         Cand* cb, long long cap, const GcpScratch& w) {
+                         Cand* cb, long long cap, const GcpScratch& w, int mid) {
   if (cap < 1) cap = 1;
   const long long tiles = (cap + kSortTile - 1) / kSortTile;
-  hipLaunchKernelGGL(k_sort_tile, dim3((unsigned)tiles), dim3(kBlock), 0, s, ca, ds, cap);
+  hipLaunchKernelGGL(k_sort_tile, dim3((unsigned)tiles), dim3(kBlock), 0, s, ca, ds, cap, mid);
   Cand* src = ca;
   Cand* dst = cb;
   for (long long wdt = kSortTile; wdt < cap; wdt *= 2) {
     const long long threads = (cap + kMergeItems - 1) / kMergeItems;
     const long long blocks = (threads + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_merge, dim3((unsigned)blocks), dim3(kBlock), 0, s, src, dst, ds, cap,
-                       wdt);
+                       wdt, mid);
     Cand* t = src;
     src = dst;
     dst = t;
   }
   const int ntiles = (int)((cap + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_gcp_tile_reduce<T>, dim3(ntiles), dim3(kBlock), 0, s, src, Gv, W, ds,
-                     cap, w.tsum, w.tstride);
+                     cap, w.tsum, w.tstride, mid);
   hipLaunchKernelGGL(s_gcp_tile_scan, dim3(1), dim3(kBlock), 0, s, ds, cap, w.tsum, w.tpre,
-                     w.tstride, w.total);
+                     w.tstride, w.total, mid);
   hipLaunchKernelGGL(k_gcp_tile_eval<T>, dim3(ntiles), dim3(kBlock), 0, s, src, Gv, W, ds, cap,
-                     w.tpre, w.tstride, w.tile_stop, w.tile_state, w.last_state);
+                     w.tpre, w.tstride, w.tile_stop, w.tile_state, w.last_state, mid);
   hipLaunchKernelGGL(s_gcp_round_end, dim3(1), dim3(kBlock), 0, s, ds, src, cap, w.tile_stop,
-                     w.tile_state, w.last_state, w.total);
+                     w.tile_state, w.last_state, w.total, mid);
 }
 
 template <typename T>
@@ -367,7 +392,7 @@ void launch_gcp_round_small(hipStream_t s, const Cand* ca, Pair<T> Gv, const T* 
   template void launch_gcp_select<T>(hipStream_t, int, long long, const T*, DevScalars*, \
                                      Cand*, long long);                                    \
   template void launch_gcp_sortscan<T>(hipStream_t, Pair<T>, const T*, DevScalars*,      \
-                                       Cand*, Cand*, long long, const GcpScratch&);
+                                       Cand*, Cand*, long long, const GcpScratch&, int);
 
 LBFGSB_INST_GCP(float)
 LBFGSB_INST_GCP(double)

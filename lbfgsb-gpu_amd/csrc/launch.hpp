@@ -118,7 +118,11 @@ template <typename T>
 void launch_gcp_round_small(hipStream_t s, const Cand* ca, Pair<T> Gv, const T* W,
                             DevScalars* ds);
 template <typename T>
+// mid = 0: a host-sized round; mid = 1: the in-chain round after the
+// one-workgroup round (see round_active in gcp.hip), capacity `cap`.
 void launch_gcp_sortscan(hipStream_t s, Pair<T> Gv, const T* W, DevScalars* ds, Cand* ca,
-                         Cand* cb, long long cap, const GcpScratch& w);
+                         Cand* cb, long long cap, const GcpScratch& w, int mid = 0);
+// candidates the in-chain multi-block round takes without the host
+constexpr long long kMidRound = 32768;
 
 }  // namespace lbfgsb

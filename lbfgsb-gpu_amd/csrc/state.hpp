@@ -182,6 +182,8 @@ struct DevScalars {
   int upd_done;      // the memory update of this iteration already ran
   int p4_flagged;    // k_p4 saw entering/leaving variables
   int gcp_rounds_run;  // statistics: Cauchy searches that needed sorted rounds
+  int gcp_handoffs;    // rounds too large for the one-workgroup round
+  int pad_handoffs;
   long long gcp_passed_total;  // statistics: breakpoints passed in those
   // workgroup-wide steps requested by the one-thread decisions: 1 = memory
   // update then the Cauchy controller, 2 = the controller only
