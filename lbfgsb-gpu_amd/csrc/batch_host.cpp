@@ -18,7 +18,7 @@
 #include <map>
 #include <vector>
 
-#include "../../include/lbfgsb_amd.h"
+#include "lbfgsb_amd.h"
 #include "batch.hpp"
 #include "common.hpp"
 #include "generate.hpp"

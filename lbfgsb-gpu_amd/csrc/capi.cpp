@@ -9,7 +9,7 @@
 #include <limits>
 #include <string>
 
-#include "../../include/lbfgsb_amd.h"
+#include "lbfgsb_amd.h"
 #include "common.hpp"
 #include "engine.hpp"
 #include "objectives.hpp"

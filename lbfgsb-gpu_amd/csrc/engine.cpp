@@ -134,6 +134,10 @@ struct Engine final : EngineBase {
   }
 
   bool owns_memory() const { return !region && !measuring; }
+  // per-block candidate counts and chunk masks of a selection (2 G words):
+  // tile_state, which holds nothing between rounds and has 33 n / kBlock
+  // words or more
+  unsigned long long* sel_count() const { return (unsigned long long*)tile_state; }
 
   template <typename P>
   int dmalloc(P*& p, size_t count) {
@@ -361,7 +365,7 @@ struct Engine final : EngineBase {
                          pmin, pmini, 0);
         break;
       case K_GCP: {
-        launch_gcp_select<T>(st, G, n, tbrk, ds, ca, n);
+        launch_gcp_select<T>(st, G, n, tbrk, ds, ca, n, sel_count());
         launch_gcp_round_small<T>(st, ca, dG, W, ds);
         // a round too large for one workgroup but within kMidRound finishes
         // here on the device; only larger ones and later ranges reach the
@@ -436,7 +440,7 @@ struct Engine final : EngineBase {
       // the previous kernel halted the chain: clear it for the round kernels
       launch_resume(st, ds, 0);
       if (hs->gcp_need_select) {
-        launch_gcp_select<T>(st, G, n, tbrk, ds, ca, n);
+        launch_gcp_select<T>(st, G, n, tbrk, ds, ca, n, sel_count());
         HIPCHK(hipGetLastError());
         continue;  // size the round with the count just selected
       }

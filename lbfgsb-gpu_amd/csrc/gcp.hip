@@ -26,37 +26,159 @@
 
 namespace lbfgsb {
 
-// Candidates t in [tau_lo, tau_hi) appended to cand (count in ds->cand_count;
-// entries beyond `cap` are not written).
+// Candidates t in [tau_lo, tau_hi) written to cand (count in ds->cand_count;
+// entries beyond `cap` are not written), in two launches with no global
+// atomics (one counter bumped by every wave that finds a candidate
+// serialises at about 20 ns per update, milliseconds for a round of a few
+// million).  Block b visits the kSelTile-long chunks b, b + G, ... (4
+// consecutive values per thread, vector loads).  k_gcp_count leaves the
+// block's count in sel[b] and in sel[G + b] a 64-bit mask of its chunk
+// groups that hold any; k_gcp_select re-reads only those groups and writes
+// their candidates, in index order within the block, at the offset the
+// counts of the blocks before give.
+constexpr int kSelVec = 4;
+constexpr int kSelTile = kBlock * kSelVec;
+
+template <typename T>
+__device__ __forceinline__ void sel_load(const T* __restrict__ tbrk, long long n, long long e,
+                                         double (&t)[kSelVec]) {
+  if (e + kSelVec <= n) {
+    // e is a multiple of kSelVec and tbrk 256-byte aligned
+    if constexpr (sizeof(T) == 4) {
+      const float4 v = *(const float4*)(tbrk + e);
+      t[0] = v.x;
+      t[1] = v.y;
+      t[2] = v.z;
+      t[3] = v.w;
+    } else {
+      const double2 a = *(const double2*)(tbrk + e), b = *(const double2*)(tbrk + e + 2);
+      t[0] = a.x;
+      t[1] = a.y;
+      t[2] = b.x;
+      t[3] = b.y;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < kSelVec; q++) t[q] = e + q < n ? (double)tbrk[e + q] : __builtin_inf();
+  }
+}
+
+// chunks per mask bit, the same for every block
+__device__ __forceinline__ long long sel_span(long long n) {
+  const long long nchunk = (n + kSelTile - 1) / kSelTile;
+  const long long per_block = (nchunk + gridDim.x - 1) / gridDim.x;
+  return (per_block + 63) / 64;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_gcp_count(long long n, const T* __restrict__ tbrk, const DevScalars* ds,
+            unsigned long long* __restrict__ sel) {
+  __shared__ int scnt[kBlock / kWave];
+  __shared__ unsigned long long smask[kBlock / kWave];
+  if (!ds->gcp_pending) return;
+  const double lo = ds->gcp_tau_lo, hi = ds->gcp_tau_hi;
+  const long long span = sel_span(n);
+  int cnt = 0;
+  unsigned long long mask = 0;
+  long long k = 0;
+  for (long long c = blockIdx.x; c * kSelTile < n; c += gridDim.x, k++) {
+    double t[kSelVec];
+    sel_load(tbrk, n, c * kSelTile + (long long)threadIdx.x * kSelVec, t);
+    int here = 0;
+#pragma unroll
+    for (int q = 0; q < kSelVec; q++) here += (t[q] >= lo && t[q] < hi) ? 1 : 0;  // +inf never
+    cnt += here;
+    if (here) mask |= 1ull << (k / span);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    cnt += __shfl_xor(cnt, o, kWave);
+    mask |= __shfl_xor(mask, o, kWave);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    scnt[threadIdx.x >> 6] = cnt;
+    smask[threadIdx.x >> 6] = mask;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long c = 0, msk = 0;
+    for (int w = 0; w < kBlock / kWave; w++) {
+      c += (unsigned long long)scnt[w];
+      msk |= smask[w];
+    }
+    sel[blockIdx.x] = c;
+    sel[gridDim.x + blockIdx.x] = msk;
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 k_gcp_select(long long n, const T* __restrict__ tbrk, DevScalars* ds, Cand* cand,
-             long long cap) {
+             long long cap, const unsigned long long* __restrict__ sel) {
+  __shared__ unsigned long long sred[kBlock / kWave];
+  __shared__ int swave[kBlock / kWave];
   if (!ds->gcp_pending) return;
+  const bool last = blockIdx.x == gridDim.x - 1;
+  const unsigned long long mine = sel[blockIdx.x];
+  if (!mine && !last) return;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  // offset of this block: the candidates of blocks 0 .. blockIdx.x - 1
+  unsigned long long off = 0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kBlock) off += sel[b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) off += __shfl_xor(off, o, kWave);
+  if (lane == 0) sred[wid] = off;
+  __syncthreads();
+  unsigned long long base = 0;
+  for (int w = 0; w < kBlock / kWave; w++) base += sred[w];
+  if (last && threadIdx.x == 0) {
+    ds->cand_count = base + mine;
+    // no block reads the flag; the host reads it after the launch
+    ds->gcp_need_select = 0;
+  }
+  if (!mine) return;
+  const unsigned long long mask = sel[gridDim.x + blockIdx.x];
   const double lo = ds->gcp_tau_lo, hi = ds->gcp_tau_hi;
-  const int lane = threadIdx.x & 63;
-  // no block reads the flag; the host reads it after the launch
-  if (blockIdx.x == 0 && threadIdx.x == 0) ds->gcp_need_select = 0;
-  for (long long i0 = blockIdx.x * (long long)kBlock; i0 < n;
-       i0 += (long long)gridDim.x * kBlock) {
-    const long long i = i0 + threadIdx.x;
-    const T tb = i < n ? tbrk[i] : (T)__builtin_inf();
-    const double t = (double)tb;
-    const bool take = t >= lo && t < hi;  // +inf (no breakpoint) is never taken
-    const unsigned long long mask = __ballot(take);
-    if (mask) {
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(&ds->cand_count, (unsigned long long)__popcll(mask));
-      base = __shfl(base, 0, kWave);
-      const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
-      if (take && slot < (unsigned long long)cap) {
-        Cand c;
-        c.t = (double)tb;
-        c.idx = (int)i;
-        c.pad = 0;
-        cand[slot] = c;
+  const long long span = sel_span(n);
+  long long k = 0;
+  for (long long c = blockIdx.x; c * kSelTile < n; c += gridDim.x, k++) {
+    if (!((mask >> (k / span)) & 1ull)) continue;  // the same for the whole block
+    const long long e = c * kSelTile + (long long)threadIdx.x * kSelVec;
+    double t[kSelVec];
+    sel_load(tbrk, n, e, t);
+    int here = 0;
+#pragma unroll
+    for (int q = 0; q < kSelVec; q++) here += (t[q] >= lo && t[q] < hi) ? 1 : 0;
+    // exclusive prefix of the per-thread counts over the block
+    int incl = here;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int up = __shfl_up(incl, o, kWave);
+      if (lane >= o) incl += up;
+    }
+    __syncthreads();  // swave of the previous chunk is read
+    if (lane == 63) swave[wid] = incl;
+    __syncthreads();
+    unsigned long long slot = base + (unsigned long long)(incl - here), chunk = 0;
+    for (int w = 0; w < kBlock / kWave; w++) {
+      if (w < wid) slot += (unsigned long long)swave[w];
+      chunk += (unsigned long long)swave[w];
+    }
+#pragma unroll
+    for (int q = 0; q < kSelVec; q++) {
+      if (t[q] >= lo && t[q] < hi) {
+        if (slot < (unsigned long long)cap) {
+          Cand cd;
+          cd.t = t[q];
+          cd.idx = (int)(e + q);
+          cd.pad = 0;
+          cand[slot] = cd;
+        }
+        slot++;
       }
     }
+    base += chunk;
   }
 }
 
@@ -346,8 +468,13 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
 // ---------------------------------------------------------------------------
 template <typename T>
 void launch_gcp_select(hipStream_t s, int G, long long n, const T* tbrk, DevScalars* ds,
-                       Cand* ca, long long buf_cap) {
-  hipLaunchKernelGGL(k_gcp_select<T>, dim3(G), dim3(kBlock), 0, s, n, tbrk, ds, ca, buf_cap);
+                       Cand* ca, long long buf_cap, unsigned long long* sel) {
+  // no more blocks than chunks, so that every block has at least one
+  const long long chunks = (n + kSelTile - 1) / kSelTile;
+  const int g = (int)std::max<long long>(1, std::min<long long>(G, chunks));
+  hipLaunchKernelGGL(k_gcp_count<T>, dim3(g), dim3(kBlock), 0, s, n, tbrk, ds, sel);
+  hipLaunchKernelGGL(k_gcp_select<T>, dim3(g), dim3(kBlock), 0, s, n, tbrk, ds, ca, buf_cap,
+                     sel);
 }
 
 // Sort + scan + round end for at most `cap` candidates (the device count
@@ -390,7 +517,7 @@ void launch_gcp_round_small(hipStream_t s, const Cand* ca, Pair<T> Gv, const T* 
   template void launch_gcp_round_small<T>(hipStream_t, const Cand*, Pair<T>, const T*,   \
                                           DevScalars*);                                    \
   template void launch_gcp_select<T>(hipStream_t, int, long long, const T*, DevScalars*, \
-                                     Cand*, long long);                                    \
+                                     Cand*, long long, unsigned long long*);               \
   template void launch_gcp_sortscan<T>(hipStream_t, Pair<T>, const T*, DevScalars*,      \
                                        Cand*, Cand*, long long, const GcpScratch&, int);
 

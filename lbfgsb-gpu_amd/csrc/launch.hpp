@@ -113,7 +113,7 @@ struct GcpScratch {
 };
 template <typename T>
 void launch_gcp_select(hipStream_t s, int G, long long n, const T* tbrk, DevScalars* ds,
-                       Cand* ca, long long buf_cap);
+                       Cand* ca, long long buf_cap, unsigned long long* bcount);
 template <typename T>
 void launch_gcp_round_small(hipStream_t s, const Cand* ca, Pair<T> Gv, const T* W,
                             DevScalars* ds);

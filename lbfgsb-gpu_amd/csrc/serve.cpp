@@ -23,7 +23,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/lbfgsb_amd.h"
+#include "lbfgsb_amd.h"
 
 namespace {
 
