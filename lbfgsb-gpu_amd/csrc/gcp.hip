@@ -301,34 +301,44 @@ k_gcp_tile_reduce(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
   }
 }
 
-// Exclusive scan over tiles (the carried state included) and the round total.
-__global__ void s_gcp_tile_scan(DevScalars* ds, long long cap, const double* tsum,
-                                double* tpre, int tstride, double* total, int mid) {
-  __shared__ double sh[kWavesPerBlock][kWave];
+// Exclusive scan over tiles (the carried state included) and the round total:
+// block q scans component q, each of its kScanThreads threads a contiguous
+// run of tiles (local sum, block scan of the run sums, then the run's
+// prefixes), so a round of millions of candidates takes one pass.
+constexpr int kScanThreads = 1024;
+__global__ void __launch_bounds__(kScanThreads)
+k_gcp_tile_scan(const DevScalars* ds, long long cap, const double* __restrict__ tsum,
+                double* __restrict__ tpre, int tstride, double* __restrict__ total, int mid) {
+  __shared__ double swave[kScanThreads / kWave];
   if (!round_active(ds, cap, mid)) return;
   const long long K = round_size(ds, cap);
-  const int ntiles = (int)((K + kBlock - 1) / kBlock);
+  const long long ntiles = (K + kBlock - 1) / kBlock;
+  const int q = blockIdx.x;
+  const double* row = tsum + (size_t)q * tstride;
+  double* out = tpre + (size_t)q * tstride;
+  const long long per = (ntiles + kScanThreads - 1) / kScanThreads;
+  const long long t0 = (long long)threadIdx.x * per;
+  const long long t1 = t0 + per < ntiles ? t0 + per : ntiles;
+  double run = 0.0;
+  for (long long t = t0; t < t1; t++) run += row[t];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // one wave per state component, 64 tiles per step
-  for (int q = w; q < kNV; q += kWavesPerBlock) {
-    const double* row = tsum + (size_t)q * tstride;
-    double* out = tpre + (size_t)q * tstride;
-    double run = ds->gcp_carry[q];
-    for (int t0 = 0; t0 < ntiles; t0 += kWave) {
-      const int t = t0 + lane;
-      const double v = t < ntiles ? row[t] : 0.0;
-      double incl = v;
+  double incl = run;
 #pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const double up = __shfl_up(incl, o, kWave);
-        if (lane >= o) incl += up;
-      }
-      if (t < ntiles) out[t] = run + incl - v;
-      run += __shfl(incl, 63, kWave);
-    }
-    if (lane == 0) total[q] = run;
+  for (int o = 1; o < 64; o <<= 1) {
+    const double up = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += up;
   }
-  (void)sh;
+  if (lane == 63) swave[w] = incl;
+  __syncthreads();
+  double off = ds->gcp_carry[q];
+  for (int v = 0; v < w; v++) off += swave[v];
+  off += incl - run;
+  for (long long t = t0; t < t1; t++) {
+    const double v = row[t];
+    out[t] = off;
+    off += v;
+  }
+  if (threadIdx.x == kScanThreads - 1) total[q] = off;
 }
 
 template <typename T>
@@ -499,7 +509,7 @@ void launch_gcp_sortscan(hipStream_t s, Pair<T> Gv, const T* W, DevScalars* ds, 
   const int ntiles = (int)((cap + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_gcp_tile_reduce<T>, dim3(ntiles), dim3(kBlock), 0, s, src, Gv, W, ds,
                      cap, w.tsum, w.tstride, mid);
-  hipLaunchKernelGGL(s_gcp_tile_scan, dim3(1), dim3(kBlock), 0, s, ds, cap, w.tsum, w.tpre,
+  hipLaunchKernelGGL(k_gcp_tile_scan, dim3(kNV), dim3(kScanThreads), 0, s, ds, cap, w.tsum, w.tpre,
                      w.tstride, w.total, mid);
   hipLaunchKernelGGL(k_gcp_tile_eval<T>, dim3(ntiles), dim3(kBlock), 0, s, src, Gv, W, ds, cap,
                      w.tpre, w.tstride, w.tile_stop, w.tile_state, w.last_state, mid);
