@@ -54,6 +54,7 @@ struct BatchLds {
       double sh[kMaxRoundWaves * P4_CORR];
       double corr[kMaxRoundWaves][136];
       long long shi[kMaxRoundWaves];
+      double wrow[kMaxRoundWaves][2 * kMaxM];  // the Cauchy-point pass's W rows (corr_step)
     } r;
   } u;
 };
@@ -251,6 +252,39 @@ __device__ __forceinline__ void gang_combine(Gang<ON>& G, BatchLds& L, int nsum,
 }
 
 // ---------------------------------------------------------------------------
+// A workgroup is the only one on its CU (the kernel's register allocation
+// leaves two waves per SIMD), so a pass that reads few streams per element
+// and takes one element per thread at a time waits a full memory latency
+// for every few bytes.  The light passes (the subspace point, g'd, the
+// breakpoint selection, the objective) therefore take kLightU elements per
+// thread per step, i0 + threadIdx.x + k * blockDim.x for k < kLightU
+// (coalesced for every k, and in the order a one-element loop visits them,
+// so every per-thread sum is unchanged), and issue the loads of all of them
+// before processing any.  The heavy passes (pass A, the Cauchy-point pass,
+// the subspace direction) keep one element per step: their accumulators
+// fill the registers.
+constexpr int kLightU = 4;
+
+template <int U, typename Op, typename F>
+__device__ __forceinline__ void batch_pass(Op& op, long long lo, long long hi, F&& body) {
+  const long long nt = blockDim.x;
+  for (long long i0 = lo + threadIdx.x; i0 < hi; i0 += U * nt) {
+    typename Op::In e[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      // lanes past the end load element hi - 1 (a valid address) and skip it
+      const long long i = i0 + k * nt;
+      op.load(e[k], i < hi ? i : hi - 1);
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) op.keep(e[k]);
+#pragma unroll
+    for (int k = 0; k < U; k++)
+      if (i0 + k * nt < hi) body(i0 + k * nt, e[k]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // set-up passes (lbfgsbcpu.cpp:258-278)
 template <typename T, bool GO>
 __device__ __forceinline__ void b_errclb(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
@@ -325,8 +359,27 @@ __device__ __forceinline__ void b_objective(const Prob<T>& P, BatchLds& L, int w
   const bool pairs = P.obj.kind == OBJK_ROSEN;
   const long long k0 = pairs ? P.lo / 2 : P.lo;
   const long long k1 = pairs ? P.hi / 2 : P.hi;
-  for (long long k = k0 + threadIdx.x; k < k1; k += blockDim.x)
-    acc[0] += obj_term(P.obj, P.n, x, g, k);
+  const long long nt = blockDim.x;
+  // kLightU terms per thread per step, every one evaluated before the first
+  // gradient store (the terms' loads go out together), added to the sum in
+  // the order a one-term loop adds them
+  for (long long kb = k0 + threadIdx.x; kb < k1; kb += kLightU * nt) {
+    ObjOut<T> r[kLightU];
+    double fk[kLightU];
+#pragma unroll
+    for (int q = 0; q < kLightU; q++) {
+      const long long k = kb + q * nt;
+      fk[q] = obj_eval(P.obj, P.n, x, k < k1 ? k : k1 - 1, r[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kLightU; q++) {
+      const long long k = kb + q * nt;
+      if (k < k1) {
+        obj_store(P.obj, g, k, r[q]);
+        acc[0] += fk[q];
+      }
+    }
+  }
   bsum<1>(acc, L.u.r.sh, L.red);
   gang_combine(G, L, 1, 0, 0, 0, 0, &L.gflag);
   if (threadIdx.x == 0) L.f = (double)(T)L.red[0];
@@ -386,12 +439,13 @@ __device__ __forceinline__ void b_p4(const Prob<T>& P, BatchLds& L, Gang<GO>& G)
   P4Op<T> op;
   op.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.W);
   const long long hi = P.hi;
+  double* rowbuf = L.u.r.wrow[threadIdx.x >> 6];
   for (long long i0 = P.lo; i0 < hi; i0 += blockDim.x) {
     const long long i = i0 + threadIdx.x;
     typename P4Op<T>::In e;
     op.load(e, i < hi ? i : hi - 1);
     op.keep(e);
-    op.corr_step(i, i < hi ? op.element(i, e) : 0);
+    op.corr_step(i, i < hi ? op.element(i, e) : 0, e, rowbuf);
   }
   bsum<P4_CORR>(op.acc, L.u.r.sh, L.red);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -439,12 +493,8 @@ __device__ __forceinline__ void b_p5b(const Prob<T>& P, BatchLds& L, Gang<GO>& G
   if (L.s.halt) return;
   P5bOp<T> op;
   op.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.dsub);
-  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
-    typename P5bOp<T>::In e;
-    op.load(e, i);
-    op.keep(e);
-    op.element(i, e);
-  }
+  batch_pass<kLightU>(op, P.lo, P.hi,
+                      [&](long long i, typename P5bOp<T>::In& e) { op.element(i, e); });
   bsum<P5_COUNT>(op.acc, L.u.r.sh, L.red);
   const double smin = bmin(op.smin, L.u.r.sh);
   if (threadIdx.x == 0) L.rv[3] = smin;
@@ -478,8 +528,19 @@ __device__ __forceinline__ void b_gd(const Prob<T>& P, BatchLds& L, Gang<GO>& G)
   LsDirection<T> dir;
   dir.init(&L.s, P.X, P.d);
   double acc[1] = {0.0};
-  for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x)
-    acc[0] += (double)gb[i] * (double)dir.at(i);
+  const long long nt = blockDim.x, hi = P.hi;
+  for (long long i0 = P.lo + threadIdx.x; i0 < hi; i0 += kLightU * nt) {
+    T gv[kLightU], dv[kLightU];
+#pragma unroll
+    for (int k = 0; k < kLightU; k++) {
+      const long long i = i0 + k * nt < hi ? i0 + k * nt : hi - 1;
+      gv[k] = gb[i];
+      dv[k] = dir.at(i);
+    }
+#pragma unroll
+    for (int k = 0; k < kLightU; k++)
+      if (i0 + k * nt < hi) acc[0] += (double)gv[k] * (double)dv[k];
+  }
   bsum<1>(acc, L.u.r.sh, L.red);
   gang_combine(G, L, 1, 0, 0, 0, 0, &L.gflag);
 }
@@ -497,33 +558,50 @@ __device__ __forceinline__ void b_select(const Prob<T>& P, BatchLds& L, Gang<GO>
   const int lane = threadIdx.x & 63;
   __syncthreads();
   if (threadIdx.x == 0) s->gcp_need_select = 0;
-  for (long long i0 = P.lo; i0 < P.hi; i0 += blockDim.x) {
-    const long long i = i0 + threadIdx.x;
-    bool take = false;
-    T tb = 0;
-    const uint8_t iwb = i < P.hi ? P.iwhere[i] : (uint8_t)0;
-    if (i < P.hi && breakpoint_of(x[i], g[i], P.l[i], P.u[i], iw_nbd(iwb), iw_get(iwb), tb)) {
-      const double t = (double)tb;
-      take = t >= lo && t < hi;
+  const long long nt = blockDim.x, ph = P.hi;
+  for (long long c0 = P.lo; c0 < ph; c0 += kLightU * nt) {
+    // the loads of kLightU elements per thread first
+    T xv[kLightU], gv[kLightU], lv[kLightU], uv[kLightU];
+    uint8_t bv[kLightU];
+#pragma unroll
+    for (int q = 0; q < kLightU; q++) {
+      const long long i = c0 + threadIdx.x + q * nt < ph ? c0 + threadIdx.x + q * nt : ph - 1;
+      xv[q] = x[i];
+      gv[q] = g[i];
+      lv[q] = P.l[i];
+      uv[q] = P.u[i];
+      bv[q] = P.iwhere[i];
     }
-    const unsigned long long mask = __ballot(take);
-    if (mask) {
-      unsigned long long base = 0;
-      if (lane == 0) {
-        const unsigned long long cnt = (unsigned long long)__popcll(mask);
-        if constexpr (GO)
-          base = atomicAdd(&G.ctl->cand, cnt);
-        else
-          base = atomicAdd(&s->cand_count, cnt);
+#pragma unroll
+    for (int q = 0; q < kLightU; q++) {
+      if (c0 + q * nt >= ph) break;  // uniform: the whole run is past the end
+      const long long i = c0 + threadIdx.x + q * nt;
+      bool take = false;
+      T tb = 0;
+      const uint8_t iwb = bv[q];
+      if (i < ph && breakpoint_of(xv[q], gv[q], lv[q], uv[q], iw_nbd(iwb), iw_get(iwb), tb)) {
+        const double t = (double)tb;
+        take = t >= lo && t < hi;
       }
-      base = __shfl(base, 0, kWave);
-      const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
-      if (take && slot < (unsigned long long)P.n) {
-        Cand c;
-        c.t = (double)tb;
-        c.idx = (int)i;
-        c.pad = 0;
-        P.ca[slot] = c;
+      const unsigned long long mask = __ballot(take);
+      if (mask) {
+        unsigned long long base = 0;
+        if (lane == 0) {
+          const unsigned long long cnt = (unsigned long long)__popcll(mask);
+          if constexpr (GO)
+            base = atomicAdd(&G.ctl->cand, cnt);
+          else
+            base = atomicAdd(&s->cand_count, cnt);
+        }
+        base = __shfl(base, 0, kWave);
+        const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+        if (take && slot < (unsigned long long)P.n) {
+          Cand c;
+          c.t = (double)tb;
+          c.idx = (int)i;
+          c.pad = 0;
+          P.ca[slot] = c;
+        }
       }
     }
   }

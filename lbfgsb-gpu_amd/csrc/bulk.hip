@@ -216,7 +216,9 @@ k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
      double* part, int rev) {
   __shared__ double sh[kWavesPerBlock * P4_CORR];
   __shared__ double s_corr[kWavesPerBlock][136];
+  __shared__ double s_row[kWavesPerBlock][2 * kMaxM];
   if (ds->halt) return;
+  double* rowbuf = s_row[threadIdx.x >> 6];
   P4Op<T> op;
   op.init(ds, X, Gv, l, u, iwhere, W);
   if constexpr (sizeof(T) == 4) {
@@ -238,10 +240,10 @@ k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
       const long long na = (cn < nc ? chunk_start(cn, nc, 2 * kBlock, rev) : i0) + threadIdx.x;
       op.load(eb, ib < n ? ib : n - 1);
       op.keep(ea);
-      op.corr_step(ia, ia < n ? op.element(ia, ea) : 0);
+      op.corr_step(ia, ia < n ? op.element(ia, ea) : 0, ea, rowbuf);
       op.load(ea, na < n ? na : n - 1);  // unconditional, as in k_pass_a
       op.keep(eb);
-      op.corr_step(ib, ib < n ? op.element(ib, eb) : 0);
+      op.corr_step(ib, ib < n ? op.element(ib, eb) : 0, eb, rowbuf);
     }
   } else {
     // FP64: one element per step (two would not fit two waves per SIMD)
@@ -251,7 +253,7 @@ k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
       typename P4Op<T>::In ea;
       op.load(ea, ia < n ? ia : n - 1);
       op.keep(ea);
-      op.corr_step(ia, ia < n ? op.element(ia, ea) : 0);
+      op.corr_step(ia, ia < n ? op.element(ia, ea) : 0, ea, rowbuf);
     }
   }
   block_sum_store<P4_CORR>(op.acc, sh, part + blockIdx.x, gridDim.x);

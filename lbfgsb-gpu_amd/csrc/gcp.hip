@@ -288,9 +288,10 @@ k_gcp_tile_reduce(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
   const long long pos = (long long)blockIdx.x * kBlock + threadIdx.x;
   if (pos < K) contribution(cand[pos], g, W, rr, ds->theta, v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int col = ds->col;
 #pragma unroll
   for (int q = 0; q < kNV; q++) {
-    double s = wave_sum(v[q]);
+    const double s = nv_used(q, col) ? wave_sum(v[q]) : 0.0;
     if (lane == 0) sh[wid * kNV + q] = s;
   }
   __syncthreads();
@@ -368,14 +369,17 @@ k_gcp_tile_eval(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__
     contribution(c, g, W, rr, ds->theta, v);
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int col = ds->col;
   if (threadIdx.x == 0) sstop = 0x7fffffffffffffffLL;
 #pragma unroll
   for (int q = 0; q < kNV; q++) {
     double incl = v[q];
+    if (nv_used(q, col)) {
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const double up = __shfl_up(incl, o, kWave);
-      if (lane >= o) incl += up;
+      for (int o = 1; o < 64; o <<= 1) {
+        const double up = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += up;
+      }
     }
     ex[q] = incl - v[q];
     if (lane == 63) sh[wid * kNV + q] = incl;

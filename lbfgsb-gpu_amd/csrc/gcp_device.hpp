@@ -37,17 +37,32 @@ __device__ inline bool breakpoint_of(T x, T g, T l, T u, int nb, int iw, T& t) {
   return false;
 }
 
+// Whether component q of a state vector [P_y P_s Q_y Q_s D] can be nonzero
+// with col pairs in the memory: the positions j < col of each block, and D.
+// The others are exact zeros, so their sums and scans are skipped.
+__device__ __forceinline__ bool nv_used(int q, int col) { return q == kNV - 1 || (q & 7) < col; }
+
 // Contribution of candidate c: [d w (2col) ; t d w (2col) ; d^2].
 template <typename T>
 __device__ inline void contribution(const Cand& c, const T* __restrict__ g,
                                     const T* __restrict__ W, const RingReads& rr,
                                     double theta, double (&v)[kNV]) {
   const long long i = c.idx;
-  // the whole W row and g_i are loaded before any use (positions j >= col
-  // read as zero)
+  // g_i and the used positions of the W row (a gather: every entry is its
+  // own cache line) are loaded before any use; positions j >= col are zero
+  // and not read
   T yv[kMaxM], sv[kMaxM];
   const T gi = g[i];
-  load_ring(W, i, rr, yv, sv);
+  const T* __restrict__ row = W + widx((size_t)i, 0);
+#pragma unroll
+  for (int j = 0; j < kMaxM; j++) {
+    yv[j] = (T)0;
+    sv[j] = (T)0;
+    if (rr.use[j]) {
+      yv[j] = row[rr.rs[j] * kTile];
+      sv[j] = row[(kWPhys + rr.rs[j]) * kTile];
+    }
+  }
   const double d = -(double)gi;
 #pragma unroll
   for (int j = 0; j < kMaxM; j++) {
@@ -368,6 +383,7 @@ __device__ __forceinline__ void block_scan_range(const DevScalars* ds, RoundLds&
                                         long long c_lo, long long c_hi, long long K, bool test,
                                         const T* g, const T* W) {
   const RingReads rr = ring_reads(ds->head, ds->m, ds->col, -1, ds->pslot_x);
+  const int col = ds->col;
   const double theta = ds->theta;
   const long long kpassed = ds->gcp_kpassed;
   const double t_prev0 = ds->gcp_t_prev;
@@ -391,10 +407,12 @@ __device__ __forceinline__ void block_scan_range(const DevScalars* ds, RoundLds&
 #pragma unroll
     for (int q = 0; q < kNV; q++) {
       double incl = v[q];
+      if (nv_used(q, col)) {
 #pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const double up = __shfl_up(incl, o, kWave);
-        if (lane >= o) incl += up;
+        for (int o = 1; o < 64; o <<= 1) {
+          const double up = __shfl_up(incl, o, kWave);
+          if (lane >= o) incl += up;
+        }
       }
       ex[q] = incl - v[q];
       if (lane == 63) L.swave[wid * kNV + q] = incl;

@@ -22,10 +22,18 @@ __device__ inline long long obj_terms(const ObjDesc& o, long long n) {
   return o.kind == OBJK_ROSEN ? n / 2 : n;
 }
 
-// Term k: writes the gradient entries it owns and returns its share of f.
+// The gradient entries term k owns (ROSEN: the pair 2k, 2k + 1; the other
+// kinds: entry k in g0).
 template <typename T>
-__device__ inline double obj_term(const ObjDesc& o, long long n, const T* __restrict__ x,
-                                  T* __restrict__ g, long long k) {
+struct ObjOut {
+  T g0, g1;
+};
+
+// Term k: its share of f, with its gradient entries in out (not stored, so
+// that a caller can evaluate several terms before the first store).
+template <typename T>
+__device__ inline double obj_eval(const ObjDesc& o, long long n, const T* __restrict__ x,
+                                  long long k, ObjOut<T>& out) {
   if (o.kind == OBJK_GRID_QUAD || o.kind == OBJK_GRID_COS) {
     const T* b = (const T*)o.b;
     const double h2 = o.h * o.h;
@@ -48,7 +56,7 @@ __device__ inline double obj_term(const ObjDesc& o, long long n, const T* __rest
       fk += h2 * ck * cos(v);
       gk -= h2 * ck * sin(v);
     }
-    g[k] = (T)gk;
+    out.g0 = (T)gk;
     return fk;
   }
   if (o.kind == OBJK_DSSCFG) {
@@ -76,22 +84,20 @@ __device__ inline double obj_term(const ObjDesc& o, long long n, const T* __rest
     gk = gk + (ix != nx - 1 ? -dxl / hx - ler : 0.0);
     gk = gk + (ix != 0 ? dxu / hx - ler : 0.0);
     gk = gk + (iy != 0 ? dyu / hy - ler : 0.0);
-    g[k] = (T)(gk * area);
+    out.g0 = (T)(gk * area);
     const double fq = dxl * dxl + dyl * dyl + (dxu * dxu + dyu * dyu);
     const double fe = lam * (ev + exp(vr) + exp(vt)) / 3.0 + lam * (exp(vb) + exp(vl) + ev) / 3.0;
     return area * (0.5 * fq - fe);
   }
   if (o.kind == OBJK_ROSEN) {
-    // the pair (x_{2k}, x_{2k+1}) as one 2-vector load and store: a wave
-    // moves one contiguous run instead of two strided ones
+    // the pair (x_{2k}, x_{2k+1}) as one 2-vector load: a wave reads one
+    // contiguous run instead of two strided ones
     using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
     const V2 xv = reinterpret_cast<const V2*>(x)[k];
     const double a = (double)xv.x, bb = (double)xv.y;
     const double t1 = bb - a * a, t2 = 1.0 - a;
-    V2 gv;
-    gv.x = (T)(-400.0 * t1 * a - 2.0 * t2);
-    gv.y = (T)(200.0 * t1);
-    reinterpret_cast<V2*>(g)[k] = gv;
+    out.g0 = (T)(-400.0 * t1 * a - 2.0 * t2);
+    out.g1 = (T)(200.0 * t1);
     return 100.0 * t1 * t1 + t2 * t2;
   }
   const T* a0 = (const T*)o.a0;
@@ -104,8 +110,34 @@ __device__ inline double obj_term(const ObjDesc& o, long long n, const T* __rest
   if (k + 2 < n) ax += (double)a2[k] * (double)x[k + 2];
   if (k >= 1) ax += (double)a1[k - 1] * (double)x[k - 1];
   if (k >= 2) ax += (double)a2[k - 2] * (double)x[k - 2];
-  g[k] = (T)(ax - (double)b[k]);
+  out.g0 = (T)(ax - (double)b[k]);
   return 0.5 * ax * xi - (double)b[k] * xi;
+}
+
+// Stores the gradient entries of term k.
+template <typename T>
+__device__ inline void obj_store(const ObjDesc& o, T* __restrict__ g, long long k,
+                                 const ObjOut<T>& out) {
+  if (o.kind == OBJK_ROSEN) {
+    // the pair as one 2-vector store: a wave writes one contiguous run
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    V2 gv;
+    gv.x = out.g0;
+    gv.y = out.g1;
+    reinterpret_cast<V2*>(g)[k] = gv;
+  } else {
+    g[k] = out.g0;
+  }
+}
+
+// Term k: writes the gradient entries it owns and returns its share of f.
+template <typename T>
+__device__ inline double obj_term(const ObjDesc& o, long long n, const T* __restrict__ x,
+                                  T* __restrict__ g, long long k) {
+  ObjOut<T> r;
+  const double f = obj_eval(o, n, x, k, r);
+  obj_store(o, g, k, r);
+  return f;
 }
 
 }  // namespace lbfgsb

@@ -301,7 +301,7 @@ struct P4Op {
   // every wave owns pairs L, L+64, L+128 (< 136); wa/wb are the W columns
   // (0..7 = Y slots, 8..15 = S slots) of the pair's two factors
   double corr[3];
-  int wa[3], wb[3];
+  int ra[3], rb[3];  // positions in rowbuf of the pair's two factors
   bool pact[3];
 
   struct In {
@@ -339,14 +339,10 @@ struct P4Op {
       int cat = 0, a = 0, b = 0;
       if (t < 136) corr_pair(t, cat, a, b);
       pact[q] = t < 136 && a < upcl && b < upcl;
-      int sa = head + a, sb = head + b;
-      if (sa >= m) sa -= m;
-      if (sb >= m) sb -= m;
-      // columns of the pair's two factors: Y of a physical slot p is column
-      // p, S is column kWPhys + p
-      const int pa = ring_phys(ds, sa), pb = ring_phys(ds, sb);
-      wa[q] = cat == 0 ? pa : kWPhys + pa;
-      wb[q] = cat == 1 ? kWPhys + pb : pb;
+      // the pair's two factors by memory position: YY (cat 0), SS (cat 1),
+      // SY (cat 2, a the S position, b the Y position)
+      ra[q] = cat == 0 ? a : kMaxM + a;
+      rb[q] = cat == 1 ? kMaxM + b : b;
     }
   }
   __device__ __forceinline__ void load(In& e, long long i) const {
@@ -417,7 +413,12 @@ struct P4Op {
   }
   // Signed Gram over this wave's entering / leaving variables, one flagged
   // variable at a time in lane order (all lanes of the wave take part).
-  __device__ __forceinline__ void corr_step(long long i, int flag) {
+  // The flagged variable's lane has its W row in registers (e, masked by
+  // element()); it puts the row into the wave's 16 doubles of LDS (rowbuf:
+  // Y positions 0..7, S positions 8..15) and every lane reads the two
+  // factors of its pairs from there, so no flagged variable waits for a
+  // load from memory.
+  __device__ __forceinline__ void corr_step(long long i, int flag, const In& e, double* rowbuf) {
     if (!(track && col > 0)) return;
     const int lane = threadIdx.x & 63;
     unsigned long long fm = __ballot(flag != 0);
@@ -425,15 +426,24 @@ struct P4Op {
     while (fm) {
       const int src = __ffsll((long long)fm) - 1;
       fm &= fm - 1ull;
-      const long long ei = __shfl(i, src, kWave);
       const double sg = (double)__shfl(flag, src, kWave);
-      const double wrow = lane < kWCols ? (double)W[widx(ei, lane)] : 0.0;
+      if (lane == src) {
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++) {
+          rowbuf[j] = (double)e.yv[j];
+          rowbuf[kMaxM + j] = (double)e.sv[j];
+        }
+      }
+      // the wave's LDS operations complete in order: the reads below see the
+      // row (the wave barrier keeps the compiler from moving them up)
+      __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int q = 0; q < 3; q++) {
-        const double va = __shfl(wrow, wa[q], kWave);
-        const double vb = __shfl(wrow, wb[q], kWave);
+        const double va = rowbuf[ra[q]];
+        const double vb = rowbuf[rb[q]];
         if (pact[q]) corr[q] += sg * va * vb;
       }
+      __builtin_amdgcn_wave_barrier();
     }
   }
 };
