@@ -451,7 +451,7 @@ def run_serving(count: int, qps: float, check: int = 48, polish: bool = True):
 
     reqs = pb.serving_stream(20000, qps=qps)[:count]
     cfg = dict(budget_bytes=200e9, fp32_share=0.7, max_slots=4096, threads=256, segment=65536,
-               max_gang=64, quantum=20, use_graph=True, polish=polish)
+               max_gang=64, quantum=20, use_graph=True, polish=polish, polish_eps_scale=1e-2)
     print(f"serving: {count} requests at {qps} per second", file=sys.stderr, flush=True)
     res, st = serve(reqs, **cfg)
     done = [r for r in res if r["info"] >= 0]
@@ -470,27 +470,51 @@ def run_serving(count: int, qps: float, check: int = 48, polish: bool = True):
         out["fp32_phase_iterations_mean"] = float(np.mean([r["iter_fp32"] for r in pol]))
     print(f"serving: {out['solves_per_s']:.2f} solves/s, latency {out['latency']}",
           file=sys.stderr, flush=True)
-    # the sample: fp32 and fp64 requests, each solved cold in FP64 (batched)
+    # the sample: fp32 and fp64 requests, each solved cold in FP64 (batched),
+    # and solved tightly in FP64 (tolerance x 1e-4, up to 5000 iterations)
+    # for the optimum f* both answers are measured against
     idx = ([k for k, q in enumerate(reqs) if q["dtype"] == 0][:check] +
            [k for k, q in enumerate(reqs) if q["dtype"] == 1][:check // 3])
-    b = bt.Batch(np.float64, sum(reqs[k]["n"] + 64 for k in idx), len(idx))
-    b.configure(65536, 64)
-    b.use_graph(True)
-    slots = [bt.add_any(b, reqs[k]) for k in idx]
-    b.run()
-    f64, it64, info64, _ = b.results()
-    b.close()
+
+    def solve64(eps_scale, max_it):
+        b = bt.Batch(np.float64, sum(reqs[k]["n"] + 64 for k in idx), len(idx))
+        b.configure(65536, 64)
+        b.use_graph(True)
+        slots = [bt.add_any(b, dict(reqs[k], eps=reqs[k]["eps"] * eps_scale,
+                                    max_iteration=max_it or reqs[k]["max_iteration"]))
+                 for k in idx]
+        b.run()
+        fv, _, iv, _ = b.results()
+        b.close()
+        return np.array([fv[s] for s in slots]), np.array([iv[s] for s in slots])
+
+    ref, info64 = solve64(1.0, 0)
+    fopt, _ = solve64(1e-4, 5000)
     kinds = np.array([reqs[k]["kind"] for k in idx])
     dts = np.array([reqs[k]["dtype"] for k in idx])
     served = np.array([res[k]["f"] for k in idx])
-    ref = np.array([f64[s] for s in slots])
-    capped = np.array([res[k]["info"] == 5 or info64[s] == 5 for k, s in zip(idx, slots)])
+    capped = np.array([res[k]["info"] == 5 or info64[j] == 5 for j, k in enumerate(idx)])
     rel = _rel_errors(kinds, served, ref)
-    worse = (served - ref) / np.where(kinds == pb.OBJ_ROSEN, 1.0, np.abs(ref))
+    den = np.where(kinds == pb.OBJ_ROSEN, 1.0, np.abs(ref))
+    worse = (served - ref) / den
     m32 = dts == 0
     out["f32_vs_f64_rel_error"] = _error_summary(kinds[m32], rel[m32], worse[m32], capped[m32])
     out["f64_served_vs_f64_rel_error"] = _error_summary(kinds[~m32], rel[~m32], worse[~m32],
                                                         capped[~m32])
+    # both answers against the optimum: (f - f*) / |f*| (absolute for
+    # Rosenbrock, f* = 0); the bar is that a served fp32 answer is not above
+    # the FP64 solve's objective by more than 1e-6 of it
+    dopt = np.where(kinds == pb.OBJ_ROSEN, 1.0, np.abs(fopt))
+    e_served, e_64 = (served - fopt) / dopt, (ref - fopt) / dopt
+    for name, sel in (("converged", m32 & ~capped), ("capped", m32 & capped)):
+        if sel.any():
+            out["f32_vs_f64_rel_error"][f"vs_optimum_{name}"] = {
+                "count": int(sel.sum()),
+                "served_max": float(e_served[sel].max()),
+                "served_median": float(np.median(e_served[sel])),
+                "fp64_max": float(e_64[sel].max()),
+                "fp64_median": float(np.median(e_64[sel])),
+                "frac_served_not_above_fp64_by_1e-6": float(np.mean(worse[sel] <= 1e-6))}
     # the floor: FP64 batched vs FP64 plain on the fp32 part of the sample
     # (grid problems of at most 2^20 elements, to bound the plain solves' time)
     sel = [j for j, k in enumerate(idx) if dts[j] == 0 and reqs[k]["n"] <= (1 << 20)][:16]

@@ -293,12 +293,15 @@ int lbfgsb_generate(int dtype, int kind, long long n, long long nx, long long ny
  * start of the stream; requests are taken in arrival order.
  * FP64 polish (polish = 1): an fp32 request that finishes is admitted again
  * into the fp64 arena, ahead of waiting fp64 requests, with the same
- * problem and tolerance and the iterations its fp32 phase left of the cap,
- * started from its fp32 answer
- * (lbfgsb_batch_warm_start); its answer is that FP64 solve's (info, f), so
- * it stops on the FP64 form's own tests.  The fp32 slot is held until the
- * polish is admitted.  A request larger than the fp64 arena keeps its fp32
- * answer (polished = 0). */
+ * problem, the tolerance eps * polish_eps_scale (a scale of 0 means 1) and
+ * the iterations its fp32 phase left of the cap, started from its fp32
+ * answer (lbfgsb_batch_warm_start); its answer is that FP64 solve's (info,
+ * f), so it stops on the FP64 form's own tests.  The polish restarts the
+ * memory, and a restarted solve that stops on the same decrease test as a
+ * cold FP64 solve can stop above the cold solve's objective; a scale below
+ * 1 carries it past that point, within the request's iteration cap.  The
+ * fp32 slot is held until the polish is admitted.  A request larger than
+ * the fp64 arena keeps its fp32 answer (polished = 0). */
 typedef struct {
   int kind;                 /* objective kind, as lbfgsb_generate */
   int dtype;                /* 0 fp32 bulk, 1 fp64 */
@@ -331,6 +334,7 @@ typedef struct {
   int quantum;              /* iterations per launch (> 0) */
   int use_graph;
   int polish;               /* 1: fp32 answers are finished in FP64 (below) */
+  double polish_eps_scale;  /* tolerance of the polish over the request's (0: 1) */
 } lbfgsb_serve_config;
 
 typedef struct {

@@ -118,6 +118,7 @@ extern "C" int lbfgsb_serve_stream(const lbfgsb_serve_config* cfg, const lbfgsb_
   std::deque<Held> polishq;
   std::vector<int> polished(count, 0), iter32(count, 0);
   const bool polish = cfg->polish != 0 && lane[1].b != nullptr;
+  const double polish_scale = cfg->polish_eps_scale > 0 ? cfg->polish_eps_scale : 1.0;
   auto answer = [&](int r, double f, int it, int info, int gang, double t_admit) {
     lbfgsb_response& o = out[r];
     o.f = f;
@@ -154,10 +155,12 @@ extern "C" int lbfgsb_serve_stream(const lbfgsb_serve_config* cfg, const lbfgsb_
       const Held h = polishq.front();
       const lbfgsb_request& q = reqs[h.r];
       Lane& P = lane[1];
-      // the polish has the iterations the fp32 phase left of the request's cap
+      // the polish has the iterations the fp32 phase left of the request's
+      // cap, and the request's tolerance scaled by polish_eps_scale
       const int left = std::max(q.max_iteration - h.iter, 1);
-      const int slot = lbfgsb_batch_add_generated(P.b, q.kind, q.n, q.nx, q.ny, q.seed, q.eps,
-                                                  q.eps, q.eps, left);
+      const double pe = q.eps * polish_scale;
+      const int slot = lbfgsb_batch_add_generated(P.b, q.kind, q.n, q.nx, q.ny, q.seed, pe, pe,
+                                                  pe, left);
       if (slot == -3) break;  // the fp64 arena is full: wait for completions
       polishq.pop_front();
       if (slot < 0 || lbfgsb_batch_warm_start(P.b, slot, lane[0].b, h.slot) != 0) {

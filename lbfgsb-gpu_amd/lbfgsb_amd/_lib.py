@@ -72,7 +72,8 @@ class ServeConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_double), ("fp32_share", C.c_double),
                 ("max_slots", C.c_int), ("threads", C.c_int), ("blocks", C.c_int),
                 ("segment", C.c_longlong), ("max_gang", C.c_int), ("quantum", C.c_int),
-                ("use_graph", C.c_int), ("polish", C.c_int)]
+                ("use_graph", C.c_int), ("polish", C.c_int),
+                ("polish_eps_scale", C.c_double)]
 
 
 class ServeStats(C.Structure):

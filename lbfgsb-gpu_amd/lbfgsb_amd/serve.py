@@ -7,7 +7,9 @@ solve requests served with continuous batching under an HBM budget.
 `out` is a list of dicts (f, iter, info, gang, dtype, the arrival,
 admission and completion times in seconds from the start of the stream, and
 with polish=True whether the answer is the FP64 polish of an fp32 solve and
-the fp32 phase's iterations); latency = t_done - t_arrival.
+the fp32 phase's iterations); latency = t_done - t_arrival.  The polish
+runs to the request's tolerance times polish_eps_scale (default 1e-2),
+within the request's iteration cap.
 """
 from __future__ import annotations
 
@@ -27,7 +29,8 @@ def _torch():
 
 def serve(requests, budget_bytes: float = 200e9, fp32_share: float = 0.7, max_slots: int = 4096,
           threads: int = 256, blocks: int = 0, segment: int = 0, max_gang: int = 64,
-          quantum: int = 20, use_graph: bool = True, polish: bool = False):
+          quantum: int = 20, use_graph: bool = True, polish: bool = False,
+          polish_eps_scale: float = 1e-2):
     torch = _torch()
     L = _lib.lib()
     cnt = len(requests)
@@ -45,7 +48,8 @@ def serve(requests, budget_bytes: float = 200e9, fp32_share: float = 0.7, max_sl
                            max_slots=int(max_slots), threads=int(threads), blocks=int(blocks),
                            segment=int(segment), max_gang=int(max_gang) if segment else 1,
                            quantum=int(quantum), use_graph=1 if use_graph else 0,
-                           polish=1 if polish else 0)
+                           polish=1 if polish else 0,
+                           polish_eps_scale=float(polish_eps_scale))
     st = _lib.ServeStats()
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     rc = L.lbfgsb_serve_stream(C.byref(cfg), arr, cnt, out, C.byref(st), stream)
