@@ -99,7 +99,7 @@ int lbfgsb_minimize_f32(int n, const lbfgsb_callbacks_f32* cb, const lbfgsb_opti
  * that forbid hipMalloc.  Same arguments, results and info codes as
  * lbfgsb_minimize; the workspace is a fixed 64 MiB region of device memory
  * per GPU, held until the process ends, which admits fp64 solves up to
- * 286,624 variables and fp32 up to 485,936 (m = 8).  A larger n returns -3 (info
+ * 291,584 variables and fp32 up to 485,888 (m = 8).  A larger n returns -3 (info
  * -3) with a message, leaving x untouched.  Every call on one GPU shares that
  * GPU's region: calls that overlap on one GPU race. */
 int lbfgsb_minimize_noalloc_f64(int n, const lbfgsb_callbacks_f64* cb,

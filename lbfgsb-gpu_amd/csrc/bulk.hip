@@ -138,13 +138,17 @@ template <typename T, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
 k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
          const T* __restrict__ u, uint8_t* __restrict__ iwhere, T* __restrict__ W, const T* __restrict__ d,
-         T* __restrict__ tbrk, DevScalars* ds, double* part, double* pmax, double* pmin,
-         long long* pmini, int rev, int spec) {
+         Cand* __restrict__ nearc, unsigned* __restrict__ nearn, int near_cap, DevScalars* ds,
+         double* part, double* pmax, double* pmin, long long* pmini, int rev, int spec) {
   __shared__ double sh[kWavesPerBlock * PA_COUNT];
+  __shared__ unsigned s_near;
   if (ds->halt && !spec) return;
+  if (threadIdx.x == 0) s_near = 0u;
+  __syncthreads();
   using Op = PassAOp<T, FUSED>;
   Op op;
-  op.init(ds, X, Gv, l, u, iwhere, W, d, tbrk, spec != 0);
+  op.init(ds, X, Gv, l, u, iwhere, W, d, nearc + (size_t)blockIdx.x * near_cap, &s_near,
+          (unsigned)near_cap, spec != 0);
   if constexpr (sizeof(T) == 4) {
     // FP32: two consecutive runs of 256 elements per block step, software
     // pipelined: the loads of one run are in flight while the other run is
@@ -183,6 +187,8 @@ k_pass_a(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
   }
   op.finish();
   block_sum_store<PA_COUNT>(op.acc, sh, part + blockIdx.x, gridDim.x);
+  // the barriers of block_sum_store order every list append before this read
+  if (threadIdx.x == 0) nearn[blockIdx.x] = s_near;
   if (FUSED) block_max_store(op.sbg, pmax + blockIdx.x);
   double bkmin = op.bkmin;
   long long ibk = op.ibk;
@@ -457,15 +463,18 @@ void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g,
 }
 template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X,
-                   Pair<T> Gv, const T* l, const T* u, uint8_t* iwhere, T* W, const T* d, T* tbrk, DevScalars* ds,
+                   Pair<T> Gv, const T* l, const T* u, uint8_t* iwhere, T* W, const T* d,
+                   Cand* nearc, unsigned* nearn, int near_cap, DevScalars* ds,
                    double* part, double* pmax, double* pmin, long long* pmini, int rev,
                    int spec) {
   if (fused)
     hipLaunchKernelGGL((k_pass_a<T, true>), dim3(G), dim3(kBlock), 0, s, n, X, Gv,
-                       l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini, rev, spec);
+                       l, u, iwhere, W, d, nearc, nearn, near_cap, ds, part, pmax, pmin, pmini,
+                       rev, spec);
   else
     hipLaunchKernelGGL((k_pass_a<T, false>), dim3(G), dim3(kBlock), 0, s, n, X,
-                       Gv, l, u, iwhere, W, d, tbrk, ds, part, pmax, pmin, pmini, rev, 0);
+                       Gv, l, u, iwhere, W, d, nearc, nearn, near_cap, ds, part, pmax, pmin,
+                       pmini, rev, 0);
 }
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv,
@@ -537,8 +546,8 @@ void launch_direction(hipStream_t s, int G, long long n, Pair<T> X, const T* d,
                                  const T*, const T*, const int*, double*);          \
   template void launch_pass_a<T>(hipStream_t, int, bool, long long, Pair<T>,      \
                                  Pair<T>, const T*, const T*,         \
-                                 uint8_t*, T*, const T*, T*, DevScalars*, double*,  \
-                                 double*, double*, long long*, int, int);                \
+                                 uint8_t*, T*, const T*, Cand*, unsigned*, int, DevScalars*, \
+                                 double*, double*, double*, long long*, int, int);       \
   template void launch_p4<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,     \
                              const T*, const T*, uint8_t*, const T*,    \
                              DevScalars*, double*, int);                                 \

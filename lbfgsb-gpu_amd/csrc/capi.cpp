@@ -58,7 +58,8 @@ const ScalarField kFields[] = {
     SF_I(gcp_floor_hits), SF_D(alpha), SF_L(ibd), SF_I(write_trial), SF_I(trial_src), SF_I(trial_has_z), SF_I(d_stored), SF_I(trial_d),
     SF_I(trial_dst), SF_I(has_after_iteration), SF_I(has_custom_stop), SF_I(gcp_pending),
     SF_I(upd_done), SF_I(p4_flagged), SF_I(gcp_rounds_run), SF_L(gcp_passed_total),
-    SF_L(cand_count), SF_I(gcp_need_select), SF_I(gcp_ranges)};
+    SF_L(cand_count), SF_I(gcp_need_select), SF_I(gcp_ranges), SF_D(gcp_near_tau),
+    SF_D(gcp_near_grow), SF_I(gcp_sel_near), SF_I(gcp_near_misses), SF_L(gcp_near_listed)};
 
 const ScalarField* find_field(const char* name) {
   for (const auto& f : kFields)

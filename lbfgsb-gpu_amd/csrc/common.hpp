@@ -48,6 +48,13 @@ struct Pair {
   }
 };
 
+// Breakpoint candidate of the Cauchy search: sorted by (t, idx).
+struct Cand {
+  double t;
+  int idx;
+  int pad;
+};
+
 // ---------------------------------------------------------------------------
 // iwhere is kept as one byte per variable:
 //   bits 0..3: iwhere + 4   (values -3, -1, 0, 1, 2, 3 -> 1, 3, 4, 5, 6, 7)

@@ -72,7 +72,13 @@ struct Engine final : EngineBase {
   const T* u;
   const int* nbd;
   T *W = nullptr, *d = nullptr, *dsub = nullptr;
-  T* tbrk = nullptr;  // breakpoint t per variable, written by pass A
+  // pass A's breakpoint lists (per block, near_cap each) and their counts
+  Cand* nearc = nullptr;
+  unsigned* nearn = nullptr;
+  int near_cap = 0;
+  // the last chain's round took its candidates from the lists: the next
+  // chain's selection starts there (a miss halts it for the host)
+  bool near_hint = false;
   // x_cp or z on request (named arrays "xcp", "z"): the iteration stores no
   // x_cp, and z only in the trial buffer until a trial overwrites it
   T* insp = nullptr;
@@ -138,6 +144,7 @@ struct Engine final : EngineBase {
   // tile_state, which holds nothing between rounds and has 33 n / kBlock
   // words or more
   unsigned long long* sel_count() const { return (unsigned long long*)tile_state; }
+  NearLists near_lists() const { return NearLists{nearc, nearn, G, near_cap}; }
 
   template <typename P>
   int dmalloc(P*& p, size_t count) {
@@ -162,8 +169,11 @@ struct Engine final : EngineBase {
     const size_t nn = (size_t)n;
     if (dmalloc(X[1], nn) || dmalloc(Gr[0], nn) || dmalloc(Gr[1], nn)) return -1;
     if (dmalloc(W, wsize(nn)) || dmalloc(d, nn) || dmalloc(dsub, nn) ||
-        dmalloc(tbrk, nn) || dmalloc(iwhere, nn))
+        dmalloc(iwhere, nn))
       return -1;
+    // room for an eighth of a block's elements in its list (64 at least)
+    near_cap = (int)std::max<long long>(64, (n + 8LL * G - 1) / (8LL * G));
+    if (dmalloc(nearc, (size_t)G * near_cap) || dmalloc(nearn, G)) return -1;
     if (dmalloc(ds, 1)) return -1;
     const size_t nv = std::max<int>(P4_COUNT, PA_COUNT);
     if (dmalloc(part, nv * G) || dmalloc(pmax, G) || dmalloc(pmin, G) || dmalloc(pmini, G))
@@ -252,8 +262,10 @@ struct Engine final : EngineBase {
     drain_events();
     for (hipEvent_t e : ev_pool) hipEventDestroy(e);
     if (!owns_memory()) return;
-    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, dsub, tbrk, insp};
+    T* bufs[] = {X[1], Gr[0], Gr[1], W, d, dsub, insp};
     for (T* p : bufs) hipFree(p);
+    hipFree(nearc);
+    hipFree(nearn);
     hipFree(iwhere);
     hipFree(ds);
     hipHostFree(hs);
@@ -353,20 +365,25 @@ struct Engine final : EngineBase {
         launch_resume(st, ds, 0);
         break;
       case K_PASSA_SA:
-        launch_pass_a<T>(st, G, false, n, dX, dG, l, u, iwhere, W, d, tbrk, ds, part, pmax,
-                         pmin, pmini, 0);
+        launch_pass_a<T>(st, G, false, n, dX, dG, l, u, iwhere, W, d, nearc, nearn, near_cap,
+                         ds, part, pmax, pmin, pmini, 0);
         break;
       case K_UPD_REDUCE:
         launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0);
         launch_update_gcp(st, ds, part, pmin, pmini, G, 1);
         break;
       case K_PASSA_FUSED:
-        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, iwhere, W, d, tbrk, ds, part, pmax,
-                         pmin, pmini, 0);
+        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, iwhere, W, d, nearc, nearn, near_cap,
+                         ds, part, pmax, pmin, pmini, 0);
         break;
       case K_GCP: {
-        launch_gcp_select<T>(st, G, n, tbrk, ds, ca, n, sel_count());
-        launch_gcp_round_small<T>(st, ca, dG, W, ds);
+        // straight from pass A's lists when the last round could (a miss
+        // halts the chain for the host's selection); otherwise the gather
+        // and the full selection, whichever covers the round, before it
+        if (!near_hint)
+          launch_gcp_select<T>(st, G, n, dX, dG, l, u, iwhere, near_lists(), ds, ca, n,
+                               sel_count(), true);
+        launch_gcp_round_small<T>(st, ca, dG, W, ds, near_lists(), near_hint);
         // a round too large for one workgroup but within kMidRound finishes
         // here on the device; only larger ones and later ranges reach the
         // host (run_rounds).  Enqueued only while such rounds are recent:
@@ -383,8 +400,8 @@ struct Engine final : EngineBase {
         launch_stop_tests(st, ds, part, pmax, pmin, pmini, G);
         break;
       case K_PASSA_SPEC:
-        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, iwhere, W, d, tbrk, ds, part, pmax,
-                         pmin, pmini, 0, 1);
+        launch_pass_a<T>(st, G, true, n, dX, dG, l, u, iwhere, W, d, nearc, nearn, near_cap,
+                         ds, part, pmax, pmin, pmini, 0, 1);
         break;
       case K_LSSTOP:
         launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0, 1, pmax, pmin, pmini);
@@ -440,7 +457,8 @@ struct Engine final : EngineBase {
       // the previous kernel halted the chain: clear it for the round kernels
       launch_resume(st, ds, 0);
       if (hs->gcp_need_select) {
-        launch_gcp_select<T>(st, G, n, tbrk, ds, ca, n, sel_count());
+        launch_gcp_select<T>(st, G, n, dX, dG, l, u, iwhere, near_lists(), ds, ca, n,
+                             sel_count(), true);
         HIPCHK(hipGetLastError());
         continue;  // size the round with the count just selected
       }
@@ -505,6 +523,7 @@ struct Engine final : EngineBase {
         if (read_status()) return -1;
       }
     }
+    near_hint = hs->gcp_sel_near != 0;
     return hs->status;
   }
 

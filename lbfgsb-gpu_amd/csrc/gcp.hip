@@ -23,43 +23,220 @@
 // larger or later rounds are
 // sized by the host.
 #include "gcp_device.hpp"
+#include "pass_device.hpp"
 
 namespace lbfgsb {
 
 // Candidates t in [tau_lo, tau_hi) written to cand (count in ds->cand_count;
-// entries beyond `cap` are not written), in two launches with no global
-// atomics (one counter bumped by every wave that finds a candidate
-// serialises at about 20 ns per update, milliseconds for a round of a few
-// million).  Block b visits the kSelTile-long chunks b, b + G, ... (4
-// consecutive values per thread, vector loads).  k_gcp_count leaves the
-// block's count in sel[b] and in sel[G + b] a 64-bit mask of its chunk
-// groups that hold any; k_gcp_select re-reads only those groups and writes
-// their candidates, in index order within the block, at the offset the
+// entries beyond `cap` are not written).
+//
+// From pass A's lists (k_gcp_near_gather): when the t-range lies below the
+// bound pass A listed breakpoints under (ds->gcp_near_tau) and no block's
+// list overflowed, the round's candidates are read from the lists.  Each
+// block of the gather takes a run of lists, filters them into LDS and
+// reserves its range of cand with one atomic (the lists are short: a few
+// candidates per thousand variables once the search settles).  The gather
+// decides for the whole round (gcp_sel_near); when the lists do not cover
+// it, the selection below runs in the chain (in_chain), or the chain halts
+// with the range to be selected by the host.
+//
+// From the variables (k_gcp_count, k_gcp_select): every variable is
+// classified again from x, g, l, u and its stored class (the same t pass A
+// forms), in two launches with no global atomics (one counter bumped by
+// every wave that finds a candidate serialises at about 20 ns per update,
+// milliseconds for a round of a few million).  Block b visits the
+// kSelTile-long chunks b, b + G, ... (kSelVec elements per thread, a block
+// stride apart).  k_gcp_count leaves the block's count in sel[b] and in
+// sel[G + b] a 64-bit mask of its chunk groups that hold any; k_gcp_select
+// re-reads only those groups and writes their candidates at the offset the
 // counts of the blocks before give.
 constexpr int kSelVec = 4;
 constexpr int kSelTile = kBlock * kSelVec;
+constexpr int kGatherLists = 64;  // pass-A lists per block of the gather
 
-template <typename T>
-__device__ __forceinline__ void sel_load(const T* __restrict__ tbrk, long long n, long long e,
-                                         double (&t)[kSelVec]) {
-  if (e + kSelVec <= n) {
-    // e is a multiple of kSelVec and tbrk 256-byte aligned
-    if constexpr (sizeof(T) == 4) {
-      const float4 v = *(const float4*)(tbrk + e);
-      t[0] = v.x;
-      t[1] = v.y;
-      t[2] = v.z;
-      t[3] = v.w;
-    } else {
-      const double2 a = *(const double2*)(tbrk + e), b = *(const double2*)(tbrk + e + 2);
-      t[0] = a.x;
-      t[1] = a.y;
-      t[2] = b.x;
-      t[3] = b.y;
-    }
-  } else {
+// Whether pass A's lists hold every candidate of the current t-range (the
+// largest list count in `nmax`).
+__device__ __forceinline__ bool near_covers(const DevScalars* ds, unsigned nmax, int cap) {
+  return ds->gcp_near_tau > 0.0 && ds->gcp_tau_hi <= ds->gcp_near_tau &&
+         nmax <= (unsigned)cap;
+}
+
+// The largest list count (every thread: its share of the G <= kMaxBlocks
+// counts, loaded together).
+__device__ __forceinline__ unsigned near_max_count(const unsigned* __restrict__ nearn, int G) {
+  unsigned mx = 0u;
 #pragma unroll
-    for (int q = 0; q < kSelVec; q++) t[q] = e + q < n ? (double)tbrk[e + q] : __builtin_inf();
+  for (int j = 0; j < kMaxBlocks / kBlock; j++) {
+    const int b = threadIdx.x + j * kBlock;
+    if (b < G) mx = max(mx, nearn[b]);
+  }
+  return mx;
+}
+
+// The lists do not cover the round (one thread): a list that overflowed asks
+// for a lower bound next time, a range beyond the bound for a higher one;
+// unless the full selection follows in the chain (in_chain), the host
+// selects the range (run_rounds).
+__device__ __forceinline__ void near_miss(DevScalars* ds, unsigned nmax, int near_cap,
+                                          int in_chain) {
+  ds->gcp_sel_near = 0;
+  if (ds->gcp_near_tau > 0.0) {
+    ds->gcp_near_misses++;
+    const double gr = ds->gcp_near_grow > 0.0 ? ds->gcp_near_grow : kNearGrow0;
+    ds->gcp_near_grow = nmax > (unsigned)near_cap ? fmax(gr * 0.25, 1.0) : fmin(gr * 4.0, 1024.0);
+  }
+  if (!in_chain) {
+    ds->gcp_need_select = 1;
+    ds->status = ST_GCP_ROUNDS;
+    ds->halt = 1;
+  }
+}
+
+// Exclusive offsets of the G lists' entries in a flat order into off[0 .. G]
+// (LDS, off[G] the total) and the largest count into *cmax (LDS, zero on
+// entry): thread t scans the counts of lists 8t .. 8t + 7.
+constexpr int kGatherU = 4;
+__device__ __forceinline__ void near_offsets(const NearLists& nl, unsigned* off, unsigned* cmax) {
+  constexpr int kPer = kMaxBlocks / kBlock;
+  __shared__ unsigned s_wsum[kBlock / kWave];
+  unsigned c[kPer];
+  unsigned run = 0, mx = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; j++) {
+    const int b = threadIdx.x * kPer + j;
+    c[j] = b < nl.G ? nl.n[b] : 0u;
+    run += c[j];
+    mx = max(mx, c[j]);
+  }
+  __syncthreads();  // *cmax is zero
+  atomicMax(cmax, mx);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned incl = run;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned up = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += up;
+  }
+  if (lane == 63) s_wsum[w] = incl;
+  __syncthreads();
+  unsigned base = incl - run;
+  for (int v = 0; v < w; v++) base += s_wsum[v];
+#pragma unroll
+  for (int j = 0; j < kPer; j++) {
+    const int b = threadIdx.x * kPer + j;
+    if (b <= nl.G) off[b] = base;
+    base += c[j];
+  }
+  if (threadIdx.x == kBlock - 1) off[nl.G] = base;  // the total (also when G = kMaxBlocks)
+  __syncthreads();
+}
+
+// Position in the lists of flat entry q < off[G]: the list b with
+// off[b] <= q < off[b + 1] (binary search in LDS), entry q - off[b].
+__device__ __forceinline__ size_t near_entry(const unsigned* off, const NearLists& nl,
+                                             unsigned q) {
+  int lo = 0, hi = nl.G - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= q)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return (size_t)lo * nl.cap + (q - off[lo]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_gcp_near_gather(const Cand* __restrict__ nearc, const unsigned* __restrict__ nearn, int G,
+                  int near_cap, DevScalars* ds, Cand* cand, long long cap, int in_chain) {
+  __shared__ unsigned s_max, s_cnt;
+  __shared__ unsigned long long s_base;
+  __shared__ Cand s_c[kGatherLists * 8];
+  if (!ds->gcp_pending || ds->halt) return;
+  if (threadIdx.x == 0) {
+    s_max = 0u;
+    s_cnt = 0u;
+  }
+  __syncthreads();
+  atomicMax(&s_max, near_max_count(nearn, G));
+  __syncthreads();
+  const bool covers = near_covers(ds, s_max, near_cap);
+  if (!covers) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) near_miss(ds, s_max, near_cap, in_chain);
+    return;
+  }
+  const double lo = ds->gcp_tau_lo, hi = ds->gcp_tau_hi;
+  // this block's lists: kGatherLists of them, kBlock / kGatherLists threads
+  // per list taking every such entry (the lists' loads are independent)
+  constexpr int kPerList = kBlock / kGatherLists;
+  const int b = blockIdx.x * kGatherLists + (threadIdx.x % kGatherLists);
+  const unsigned c = b < G ? nearn[b] : 0u;
+  const Cand* lst = nearc + (size_t)(b < G ? b : 0) * near_cap;
+  // candidates in range: LDS first (up to its size), the rest straight to
+  // cand after the reservation
+  for (unsigned k = threadIdx.x / kGatherLists; k < c; k += kPerList) {
+    const Cand e = lst[k];
+    if (e.t >= lo && e.t < hi) {
+      const unsigned q = atomicAdd(&s_cnt, 1u);
+      if (q < kGatherLists * 8) s_c[q] = e;
+    }
+  }
+  __syncthreads();
+  const unsigned cnt = s_cnt;
+  if (threadIdx.x == 0) s_base = cnt ? atomicAdd(&ds->cand_count, (unsigned long long)cnt) : 0ull;
+  __syncthreads();
+  const unsigned long long base = s_base;
+  if (cnt <= kGatherLists * 8) {
+    for (unsigned q = threadIdx.x; q < cnt; q += kBlock)
+      if (base + q < (unsigned long long)cap) cand[base + q] = s_c[q];
+  } else {
+    // more than the LDS holds: filter the lists again into the reservation
+    // (the order inside it need not be the first pass's: rounds sort by
+    // (t, index))
+    if (threadIdx.x == 0) s_cnt = 0u;
+    __syncthreads();
+    for (unsigned k = threadIdx.x / kGatherLists; k < c; k += kPerList) {
+      const Cand e = lst[k];
+      if (e.t >= lo && e.t < hi) {
+        const unsigned long long q = base + atomicAdd(&s_cnt, 1u);
+        if (q < (unsigned long long)cap) cand[q] = e;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ds->gcp_sel_near = 1;
+    // no block reads the flag; the host reads it after the launch
+    ds->gcp_need_select = 0;
+  }
+}
+
+// The breakpoints of kSelVec elements e, e + kBlock, ... of a chunk (t, or
+// +inf where there is none), classified from x, g, l, u and the stored class
+// as pass A classifies them.
+template <typename T>
+__device__ __forceinline__ void sel_load(const T* __restrict__ x, const T* __restrict__ g,
+                                         const T* __restrict__ l, const T* __restrict__ u,
+                                         const uint8_t* __restrict__ iwhere, long long n,
+                                         long long e, double (&t)[kSelVec]) {
+  T xv[kSelVec], gv[kSelVec], lv[kSelVec], uv[kSelVec];
+  uint8_t bv[kSelVec];
+#pragma unroll
+  for (int q = 0; q < kSelVec; q++) {
+    const long long i = e + (long long)q * kBlock < n ? e + (long long)q * kBlock : n - 1;
+    xv[q] = x[i];
+    gv[q] = g[i];
+    lv[q] = l[i];
+    uv[q] = u[i];
+    bv[q] = iwhere[i];
+  }
+#pragma unroll
+  for (int q = 0; q < kSelVec; q++) {
+    int iw;
+    T neggi, tb;
+    bool isfree, isbrk;
+    classify(xv[q], gv[q], lv[q], uv[q], iw_nbd(bv[q]), iw_get(bv[q]), iw, neggi, isfree, isbrk,
+             tb);
+    t[q] = (isbrk && e + (long long)q * kBlock < n) ? (double)tb : __builtin_inf();
   }
 }
 
@@ -72,11 +249,15 @@ __device__ __forceinline__ long long sel_span(long long n) {
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_gcp_count(long long n, const T* __restrict__ tbrk, const DevScalars* ds,
+k_gcp_count(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
+            const T* __restrict__ u, const uint8_t* __restrict__ iwhere, const DevScalars* ds,
             unsigned long long* __restrict__ sel) {
   __shared__ int scnt[kBlock / kWave];
   __shared__ unsigned long long smask[kBlock / kWave];
-  if (!ds->gcp_pending) return;
+  if (!ds->gcp_pending || ds->halt || ds->gcp_sel_near) return;
+  const int cur = ds->phase & 1;
+  const T* __restrict__ x = X[cur];
+  const T* __restrict__ g = Gv[cur];
   const double lo = ds->gcp_tau_lo, hi = ds->gcp_tau_hi;
   const long long span = sel_span(n);
   int cnt = 0;
@@ -84,7 +265,7 @@ k_gcp_count(long long n, const T* __restrict__ tbrk, const DevScalars* ds,
   long long k = 0;
   for (long long c = blockIdx.x; c * kSelTile < n; c += gridDim.x, k++) {
     double t[kSelVec];
-    sel_load(tbrk, n, c * kSelTile + (long long)threadIdx.x * kSelVec, t);
+    sel_load(x, g, l, u, iwhere, n, c * kSelTile + threadIdx.x, t);
     int here = 0;
 #pragma unroll
     for (int q = 0; q < kSelVec; q++) here += (t[q] >= lo && t[q] < hi) ? 1 : 0;  // +inf never
@@ -114,11 +295,15 @@ k_gcp_count(long long n, const T* __restrict__ tbrk, const DevScalars* ds,
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_gcp_select(long long n, const T* __restrict__ tbrk, DevScalars* ds, Cand* cand,
-             long long cap, const unsigned long long* __restrict__ sel) {
+k_gcp_select(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
+             const T* __restrict__ u, const uint8_t* __restrict__ iwhere, DevScalars* ds,
+             Cand* cand, long long cap, const unsigned long long* __restrict__ sel) {
   __shared__ unsigned long long sred[kBlock / kWave];
   __shared__ int swave[kBlock / kWave];
-  if (!ds->gcp_pending) return;
+  if (!ds->gcp_pending || ds->halt || ds->gcp_sel_near) return;
+  const int cur = ds->phase & 1;
+  const T* __restrict__ x = X[cur];
+  const T* __restrict__ g = Gv[cur];
   const bool last = blockIdx.x == gridDim.x - 1;
   const unsigned long long mine = sel[blockIdx.x];
   if (!mine && !last) return;
@@ -144,9 +329,9 @@ k_gcp_select(long long n, const T* __restrict__ tbrk, DevScalars* ds, Cand* cand
   long long k = 0;
   for (long long c = blockIdx.x; c * kSelTile < n; c += gridDim.x, k++) {
     if (!((mask >> (k / span)) & 1ull)) continue;  // the same for the whole block
-    const long long e = c * kSelTile + (long long)threadIdx.x * kSelVec;
+    const long long e = c * kSelTile + threadIdx.x;
     double t[kSelVec];
-    sel_load(tbrk, n, e, t);
+    sel_load(x, g, l, u, iwhere, n, e, t);
     int here = 0;
 #pragma unroll
     for (int q = 0; q < kSelVec; q++) here += (t[q] >= lo && t[q] < hi) ? 1 : 0;
@@ -171,7 +356,7 @@ k_gcp_select(long long n, const T* __restrict__ tbrk, DevScalars* ds, Cand* cand
         if (slot < (unsigned long long)cap) {
           Cand cd;
           cd.t = t[q];
-          cd.idx = (int)(e + q);
+          cd.idx = (int)(e + (long long)q * kBlock);
           cd.pad = 0;
           cand[slot] = cd;
         }
@@ -454,16 +639,77 @@ __global__ void s_gcp_round_end(DevScalars* ds, const Cand* __restrict__ cand, l
 // (the common case once the active set settles): bitonic sort in LDS, then
 // chunks scanned in order (gcp_device.hpp).  Larger rounds are left to the
 // multi-block path.
+//
+// from_near: the round's candidates come straight from pass A's lists (no
+// selection launch before it): the workgroup checks that the lists cover the
+// t-range, filters them into LDS, and sorts them there; a round of more than
+// kSmallRound candidates is written to cand for the multi-block path, and a
+// range the lists do not cover halts the chain for the host's selection.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__ W,
-                  DevScalars* ds) {
+s_gcp_round_small(Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__ W,
+                  DevScalars* ds, NearLists nl, int from_near) {
   __shared__ RoundLds L;
   __shared__ DevScalars sds;
+  __shared__ unsigned s_max, s_cnt;
+  __shared__ unsigned s_off[kMaxBlocks + 1];
   if (!ds->gcp_pending || ds->halt) return;
-  const long long K = (long long)ds->cand_count;
+  long long K;
+  if (from_near) {
+    if (threadIdx.x == 0) {
+      s_max = 0u;
+      s_cnt = 0u;
+    }
+    // every listed entry in range, into LDS while it has room.  The entries
+    // are taken in a flat order (list offsets from a scan of the counts,
+    // which also gives the largest count), kGatherU per thread per step,
+    // their loads issued together
+    unsigned* off = s_off;
+    near_offsets(nl, off, &s_max);
+    if (!near_covers(ds, s_max, nl.cap)) {
+      if (threadIdx.x == 0) near_miss(ds, s_max, nl.cap, 0);
+      return;
+    }
+    const double lo = ds->gcp_tau_lo, hi = ds->gcp_tau_hi;
+    const unsigned total = off[nl.G];
+    for (unsigned q0 = threadIdx.x; q0 < total; q0 += kGatherU * kBlock) {
+      Cand e[kGatherU];
+#pragma unroll
+      for (int u = 0; u < kGatherU; u++) {
+        const unsigned q = q0 + u * kBlock;
+        e[u].t = __builtin_inf();
+        if (q < total) e[u] = nl.c[near_entry(off, nl, q)];
+      }
+#pragma unroll
+      for (int u = 0; u < kGatherU; u++) {
+        if (e[u].t >= lo && e[u].t < hi) {
+          const unsigned k = atomicAdd(&s_cnt, 1u);
+          if (k < (unsigned)kSmallRound) L.sc[k] = e[u];
+        }
+      }
+    }
+    __syncthreads();
+    K = (long long)s_cnt;
+    if (K > kSmallRound) {
+      // too many for this workgroup: all of them to cand (in any order: the
+      // rounds sort by (t, index)) for the multi-block path
+      if (threadIdx.x == 0) s_cnt = 0u;
+      __syncthreads();
+      for (unsigned q = threadIdx.x; q < total; q += kBlock) {
+        const Cand e = nl.c[near_entry(off, nl, q)];
+        if (e.t >= lo && e.t < hi) cand[atomicAdd(&s_cnt, 1u)] = e;
+      }
+    }
+  } else {
+    K = (long long)ds->cand_count;
+  }
   if (K > kSmallRound) {
     if (threadIdx.x == 0) {
+      if (from_near) {
+        ds->cand_count = (unsigned long long)K;
+        ds->gcp_sel_near = 1;
+        ds->gcp_need_select = 0;
+      }
       ds->status = ST_GCP_ROUNDS;
       ds->halt = 1;
       ds->gcp_handoffs++;
@@ -473,22 +719,35 @@ s_gcp_round_small(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict
   // the round's scalar logic (stop tests, conclusion) on an LDS copy
   ds_load(&sds, ds);
   DevScalars* s = &sds;
+  if (from_near && threadIdx.x == 0) {
+    s->cand_count = (unsigned long long)K;
+    s->gcp_sel_near = 1;
+    s->gcp_need_select = 0;
+    s->gcp_near_listed = (long long)s_off[nl.G];
+  }
   stage_gcp_const(s, L.gc);
-  block_sort_small(cand, K, L.sc);
+  // from the lists the candidates are in LDS already (sorted in place)
+  block_sort_small(from_near ? L.sc : cand, K, L.sc);
   block_round_scan(s, L, L.sc, K, (const T*)Gv[s->phase & 1], W);
   ds_store(ds, s);
 }
 
 // ---------------------------------------------------------------------------
 template <typename T>
-void launch_gcp_select(hipStream_t s, int G, long long n, const T* tbrk, DevScalars* ds,
-                       Cand* ca, long long buf_cap, unsigned long long* sel) {
+void launch_gcp_select(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
+                       const T* u, const uint8_t* iwhere, const NearLists& nl, DevScalars* ds,
+                       Cand* ca, long long buf_cap, unsigned long long* sel, bool full) {
+  const int gg = (nl.G + kGatherLists - 1) / kGatherLists;
+  hipLaunchKernelGGL(k_gcp_near_gather, dim3(gg), dim3(kBlock), 0, s, nl.c, nl.n, nl.G, nl.cap,
+                     ds, ca, buf_cap, full ? 1 : 0);
+  if (!full) return;
   // no more blocks than chunks, so that every block has at least one
   const long long chunks = (n + kSelTile - 1) / kSelTile;
   const int g = (int)std::max<long long>(1, std::min<long long>(G, chunks));
-  hipLaunchKernelGGL(k_gcp_count<T>, dim3(g), dim3(kBlock), 0, s, n, tbrk, ds, sel);
-  hipLaunchKernelGGL(k_gcp_select<T>, dim3(g), dim3(kBlock), 0, s, n, tbrk, ds, ca, buf_cap,
+  hipLaunchKernelGGL(k_gcp_count<T>, dim3(g), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere, ds,
                      sel);
+  hipLaunchKernelGGL(k_gcp_select<T>, dim3(g), dim3(kBlock), 0, s, n, X, Gv, l, u, iwhere, ds,
+                     ca, buf_cap, sel);
 }
 
 // Sort + scan + round end for at most `cap` candidates (the device count
@@ -522,16 +781,19 @@ void launch_gcp_sortscan(hipStream_t s, Pair<T> Gv, const T* W, DevScalars* ds, 
 }
 
 template <typename T>
-void launch_gcp_round_small(hipStream_t s, const Cand* ca, Pair<T> Gv, const T* W,
-                            DevScalars* ds) {
-  hipLaunchKernelGGL(s_gcp_round_small<T>, dim3(1), dim3(kBlock), 0, s, ca, Gv, W, ds);
+void launch_gcp_round_small(hipStream_t s, Cand* ca, Pair<T> Gv, const T* W, DevScalars* ds,
+                            const NearLists& nl, bool from_near) {
+  hipLaunchKernelGGL(s_gcp_round_small<T>, dim3(1), dim3(kBlock), 0, s, ca, Gv, W, ds, nl,
+                     from_near ? 1 : 0);
 }
 
 #define LBFGSB_INST_GCP(T)                                                                  \
-  template void launch_gcp_round_small<T>(hipStream_t, const Cand*, Pair<T>, const T*,   \
-                                          DevScalars*);                                    \
-  template void launch_gcp_select<T>(hipStream_t, int, long long, const T*, DevScalars*, \
-                                     Cand*, long long, unsigned long long*);               \
+  template void launch_gcp_round_small<T>(hipStream_t, Cand*, Pair<T>, const T*,         \
+                                          DevScalars*, const NearLists&, bool);            \
+  template void launch_gcp_select<T>(hipStream_t, int, long long, Pair<T>, Pair<T>,      \
+                                     const T*, const T*, const uint8_t*, const NearLists&, \
+                                     DevScalars*, Cand*, long long, unsigned long long*,   \
+                                     bool);                                                \
   template void launch_gcp_sortscan<T>(hipStream_t, Pair<T>, const T*, DevScalars*,      \
                                        Cand*, Cand*, long long, const GcpScratch&, int);
 

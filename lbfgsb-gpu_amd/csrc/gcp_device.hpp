@@ -135,7 +135,18 @@ __device__ inline bool stop_test(const DevScalars* ds, const GcpConst& gc,
   return dtm < t_next - t_prev;
 }
 
+// The bound below which the next pass A lists breakpoints: gcp_near_grow
+// (8 to start with) times the upper end of the t-range this search ended in,
+// none when that range is unbounded.
+static constexpr double kNearGrow0 = 8.0;
+__device__ __forceinline__ void near_next(DevScalars* ds) {
+  const double gr = ds->gcp_near_grow > 0.0 ? ds->gcp_near_grow : kNearGrow0;
+  const double t = ds->gcp_tau_hi * gr;
+  ds->gcp_near_tau = t < 1e300 ? t : 0.0;
+}
+
 __device__ __forceinline__ void gcp_done(DevScalars* ds) {
+  near_next(ds);
   const int col = ds->col;
   if (col > 0 && d_bmv(ds->sy, ds->wt, col, ds->c, ds->mc) != 0) ds->internalinfo = -8;
   ds->nintol += ds->nint;

@@ -11,13 +11,6 @@
 
 namespace lbfgsb {
 
-// Breakpoint candidate of the Cauchy search: sorted by (t, idx).
-struct Cand {
-  double t;
-  int idx;
-  int pad;
-};
-
 // bulk.hip (rev: the pass takes its chunks in descending order; see
 // chunk_start in common.hpp)
 template <typename T>
@@ -29,13 +22,15 @@ void launch_active(hipStream_t s, int G, long long n, T* x, const T* l, const T*
 template <typename T>
 void launch_projgr(hipStream_t s, int G, long long n, const T* x, const T* g, const T* l,
                    const T* u, const int* nbd, double* pmax);
-// tbrk receives every variable's breakpoint t (+inf: none) for the
-// Cauchy search's selections.
+// Block b of the pass lists the breakpoints of its elements with t below
+// ds->gcp_near_tau in nearc[b * near_cap ..] (at most near_cap of them) and
+// their count (listed or not) in nearn[b], for the Cauchy search's
+// selections.
 template <typename T>
 void launch_pass_a(hipStream_t s, int G, bool fused, long long n, Pair<T> X, Pair<T> Gv,
                    const T* l, const T* u, uint8_t* iwhere, T* W, const T* d,
-                   T* tbrk, DevScalars* ds, double* part, double* pmax, double* pmin,
-                   long long* pmini, int rev, int spec = 0);
+                   Cand* nearc, unsigned* nearn, int near_cap, DevScalars* ds, double* part,
+                   double* pmax, double* pmin, long long* pmini, int rev, int spec = 0);
 template <typename T>
 void launch_p4(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
                const T* u, uint8_t* iwhere, const T* W, DevScalars* ds,
@@ -111,12 +106,29 @@ struct GcpScratch {
   double* tile_state;
   double* last_state;
 };
+// Pass A's breakpoint lists: G lists of up to `cap` candidates (c) and
+// their counts (n).
+struct NearLists {
+  const Cand* c;
+  const unsigned* n;
+  int G;
+  int cap;
+};
+// The candidates of the current t-range into ca: from pass A's lists when
+// they cover it, else (full) from every variable's classification; without
+// `full`, a range the lists do not cover halts the chain for the host to
+// select.
 template <typename T>
-void launch_gcp_select(hipStream_t s, int G, long long n, const T* tbrk, DevScalars* ds,
-                       Cand* ca, long long buf_cap, unsigned long long* bcount);
+void launch_gcp_select(hipStream_t s, int G, long long n, Pair<T> X, Pair<T> Gv, const T* l,
+                       const T* u, const uint8_t* iwhere, const NearLists& nl, DevScalars* ds,
+                       Cand* ca, long long buf_cap, unsigned long long* bcount, bool full);
+// One workgroup's round of at most kSmallRound candidates (larger ones are
+// handed to the multi-block path): from ca, or (from_near) straight from
+// pass A's lists, halting the chain for the host's selection when they do
+// not cover the round.
 template <typename T>
-void launch_gcp_round_small(hipStream_t s, const Cand* ca, Pair<T> Gv, const T* W,
-                            DevScalars* ds);
+void launch_gcp_round_small(hipStream_t s, Cand* ca, Pair<T> Gv, const T* W, DevScalars* ds,
+                            const NearLists& nl, bool from_near);
 template <typename T>
 // mid = 0: a host-sized round; mid = 1: the in-chain round after the
 // one-workgroup round (see round_active in gcp.hip), capacity `cap`.

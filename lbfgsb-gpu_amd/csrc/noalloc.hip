@@ -18,7 +18,7 @@
 
 namespace lbfgsb {
 
-// 64 MiB per GPU: an fp64 solve of up to 286,624 variables, fp32 485,936
+// 64 MiB per GPU: an fp64 solve of up to 291,584 variables, fp32 485,888
 // (m = 8; engine_region_bytes gives the exact need of a size).
 constexpr size_t kNoallocBytes = size_t(64) << 20;
 

@@ -30,7 +30,13 @@ struct PassAOp {
   uint8_t* __restrict__ iwhere;
   T* __restrict__ W;
   const T* __restrict__ d;
-  T* __restrict__ tout;  // breakpoint t of every variable (+inf: none), or null
+  // plain form: the breakpoints with t below ntau are listed in nout (at most
+  // ncap, the count in the block's LDS counter ncnt) and the class is not
+  // stored; null: the class is stored (batched form)
+  Cand* __restrict__ nout;
+  unsigned* ncnt;
+  unsigned ncap;
+  double ntau;
   int col, itail, wnew;
   bool upd, stp1, dstored, spec;
   T stp;
@@ -53,10 +59,14 @@ struct PassAOp {
   // pair goes to the spare physical slot, and g'd is summed as well.
   __device__ __forceinline__ void init(const DevScalars* ds, Pair<T> X, Pair<T> Gv, const T* l_,
                               const T* u_, uint8_t* iwhere_, T* W_,
-                              const T* d_, T* tout_ = nullptr, bool spec_ = false) {
+                              const T* d_, Cand* nout_ = nullptr, unsigned* ncnt_ = nullptr,
+                              unsigned ncap_ = 0, bool spec_ = false) {
     spec = FUSED && spec_;
     const int cur = (ds->phase & 1) ^ (spec ? 1 : 0);
-    tout = tout_;
+    nout = nout_;
+    ncnt = ncnt_;
+    ncap = ncap_;
+    ntau = ds->gcp_near_tau;
     x = X[cur];
     g = Gv[cur];
     xold = X[cur ^ 1];
@@ -175,13 +185,23 @@ struct PassAOp {
         if (fabs(neggi) > 0) acc[PA_UNBND] = 1.0;  // an OR, kept 0/1 through the sum
       }
     }
-    if (tout) {
-      // the Cauchy search selects its candidates from these (the same t as
-      // breakpoint_of computes from x, g, l, u and the new class); the
-      // class itself is not stored: the Cauchy-point pass reclassifies from
-      // the stored one (classification is idempotent), so this pass writes
-      // one stream less
-      st_nt(&tout[i], isbrk ? tbrk : (T)__builtin_inf());
+    if (nout) {
+      // the Cauchy search selects its candidates from these when its t-range
+      // lies below ntau (the same t the selection pass computes from x, g,
+      // l, u and the stored class); the class itself is not stored: the
+      // Cauchy-point pass reclassifies from the stored one (classification
+      // is idempotent), so this pass writes no per-variable stream for the
+      // search
+      if (isbrk && (double)tbrk < ntau) {
+        const unsigned k = atomicAdd(ncnt, 1u);
+        if (k < ncap) {
+          Cand c;
+          c.t = (double)tbrk;
+          c.idx = (int)i;
+          c.pad = 0;
+          nout[k] = c;
+        }
+      }
     } else {
       iwhere[i] = iw_make(iw, iw_wasfree(iwb), iw_prevfree(iwb), nb);
     }

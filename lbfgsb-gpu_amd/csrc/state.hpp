@@ -132,13 +132,20 @@ struct DevScalars {
   double gcp_tau_lo, gcp_tau_hi;
   int gcp_round;
   int gcp_floor_hits;      // f2 floor engaged (closed form may deviate)
-  long long gcp_ncand;
+  long long gcp_near_listed;  // statistics: entries in pass A's lists at the last round they fed
   long long gcp_stop;      // first stop position in the current round (or -1)
-  long long gcp_nsample;
   int gcp_last_round;      // the current round covers all remaining breakpoints
   int gcp_cand_in_b;       // sorted candidates ended in the second buffer
   double gcp_t_prev;       // t of the last passed breakpoint (0 before any)
   long long gcp_idx_prev;
+  // Breakpoints pass A lists (plain form): every variable with t below
+  // gcp_near_tau (0: none listed), per block of pass A; a round whose
+  // t-range lies below it takes its candidates from that list instead of
+  // classifying every variable again (gcp_sel_near: the current round did)
+  double gcp_near_tau;
+  double gcp_near_grow;    // near_tau over the t-range the last search ended in
+  int gcp_sel_near;
+  int gcp_near_misses;     // statistics: rounds the list did not cover
 
   // ---- small dense algebra (row-major, leading dimension 8 or 16)
   double sy[64], ss[64], wt[64];

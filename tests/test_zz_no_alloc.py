@@ -41,7 +41,7 @@ def test_noalloc_refuses_beyond_the_region():
     from lbfgsb_amd import problems as pb
     from lbfgsb_amd import solver
     print("case noalloc too large", flush=True)
-    p = pb.rosenbrock(400000)                # fp64 region holds 286,624 variables
+    p = pb.rosenbrock(400000)                # fp64 region holds 291,584 variables
     dp = solver.DeviceProblem.create(p, np.float64)
     x, s = solver.minimize(dp, eps=1e-6, max_iter=10, noalloc=True)
     assert s["rc"] == -3 and s["info"] == -3
