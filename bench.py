@@ -40,9 +40,11 @@ def kernel_bytes_per_elem(s: int) -> dict:
     (col = m = 8, one memory update per iteration; the class byte iwhere also
     carries nbd, so a pass reads one byte per variable for both).
       pass_a_spec:  read x, x_old, g, g_old, l, u (6s), iwhere, the 14
-                    retained W entries (14s); write the new S/Y pair (2s) and
-                    the breakpoint t (s) the Cauchy search selects from (the
-                    step d of a unit step is x - x_old).  It runs on the
+                    retained W entries (14s); write the new S/Y pair (2s)
+                    (the step d of a unit step is x - x_old).  The
+                    breakpoints below a predicted bound go to short per-block
+                    lists the Cauchy search selects from (a few hundred
+                    entries at n = 16.8M: not counted).  It runs on the
                     evaluated trial point ahead of the line-search step and
                     also sums the g'd that step needs
       pass_a_fused: the same pass after an acceptance it could not run ahead
@@ -59,7 +61,7 @@ def kernel_bytes_per_elem(s: int) -> dict:
                     the speculative pass A sums g'd of every evaluated point
     x_cp and r are recomputed per element by the passes that need them (from
     x, g, l, u, the class and W) instead of being stored and read back."""
-    return {"pass_a_spec": 23 * s + 1, "pass_a_fused": 23 * s + 1, "p4": 20 * s + 2,
+    return {"pass_a_spec": 22 * s + 1, "pass_a_fused": 22 * s + 1, "p4": 20 * s + 2,
             "p5a": 21 * s + 1, "p5b": 6 * s + 1, "trial": 0, "gd": 3 * s}
 
 
@@ -77,7 +79,7 @@ MEASURED_TRAFFIC_16M_FP32 = {"pass_a_spec": 1563517572, "p4": 1390007887,
 
 def b_alg_per_iter(n: int, s: int) -> float:
     """Algorithmic bytes of one steady-state iteration with one (unit-step)
-    trial point: the passes of STEADY_STATE, n (70 s + 5)."""
+    trial point: the passes of STEADY_STATE, n (69 s + 5)."""
     kb = kernel_bytes_per_elem(s)
     return float(n) * sum(kb[k] for k in STEADY_STATE)
 
@@ -89,7 +91,7 @@ def b_alg_canonical(n: int, s: int) -> float:
     """The canonical per-iteration byte count of the plain path's hot-path
     contract, n (76 s + 32): every phase's reads and writes under maximal
     fusion with int32 class and index arrays, against which the builder's
-    own count n (70 s + 5) (fewer stored streams, a one-byte class) is
+    own count n (69 s + 5) (fewer stored streams, a one-byte class) is
     reported beside it."""
     return float(n) * (76 * s + 32)
 
@@ -597,7 +599,7 @@ def main():
             "frac": b_alg_per_iter(n, s) / t_iter / 1e9 / HBM_PEAK_GBS,
             "b_alg_canonical_bytes": b_alg_canonical(n, s),
             "frac_canonical": b_alg_canonical(n, s) / t_iter / 1e9 / HBM_PEAK_GBS,
-            "note": "b_alg: the bytes this engine's passes move per iteration, n(70s+5); "
+            "note": "b_alg: the bytes this engine's passes move per iteration, n(69s+5); "
                     "canonical: the hot-path contract's count n(76s+32), which assumes "
                     "int32 class/index arrays and more stored streams"},
         "kernels_us": {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in p32["kern"].items()},
