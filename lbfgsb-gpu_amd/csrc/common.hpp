@@ -17,6 +17,18 @@ constexpr int kWPhys = kMaxM + 1;
 constexpr int kWCols = 2 * kWPhys;
 constexpr int kMaxBlocks = 2048;      // grid cap for the bulk passes
 
+// Control routines that one thread runs once per step, or a workgroup once
+// per large breakpoint round.  The batched kernel is one persistent
+// function, so its register allocation is the largest any of its steps
+// needs; it defines LBFGSB_OUTLINE_CONTROL before its includes, which keeps
+// these out of line there, so that their registers and spills stay out of
+// the bulk passes' loops.  Everywhere else they are inlined.
+#if defined(LBFGSB_OUTLINE_CONTROL) && defined(__HIP_DEVICE_COMPILE__)
+#define LBFGSB_COLD __noinline__
+#else
+#define LBFGSB_COLD inline
+#endif
+
 // ---------------------------------------------------------------------------
 // W = [Y | S] in HBM: element i, column j (j < kWPhys: Y of physical slot
 // j; j >= kWPhys: S of physical slot j - kWPhys) lives at

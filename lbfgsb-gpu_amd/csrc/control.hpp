@@ -625,7 +625,7 @@ __device__ inline void post_update_middle(DevScalars* s) {
 // triangular solves of subsm (lbfgsbcpu.cpp:1459-1471) on thread 0 with the
 // vector in registers.  r = the pass's sums (P4_* layout).  The caller
 // synchronises before.
-__device__ inline void logic_formk_subsm(DevScalars* s, const double* r, bool flagged) {
+__device__ LBFGSB_COLD void logic_formk_subsm(DevScalars* s, const double* r, bool flagged) {
   const int col = s->col;
   if (threadIdx.x == 0) {
     const long long n = s->n;
@@ -733,7 +733,7 @@ __device__ inline void logic_ls_init(DevScalars* s, double dtd, double gd, doubl
 
 // Later calls of lnsrlb, after an evaluation with value f and g'd
 // (lbfgsbcpu.cpp:1312-1339 and the loop exit at :352).
-__device__ inline void logic_ls_step(DevScalars* s, double f, double gd) {
+__device__ LBFGSB_COLD void logic_ls_step(DevScalars* s, double f, double gd) {
   s->halt = 0;
   s->status = ST_RUNNING;
   s->f = f;

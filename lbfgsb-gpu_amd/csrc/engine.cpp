@@ -79,6 +79,9 @@ struct Engine final : EngineBase {
   // the last chain's round took its candidates from the lists: the next
   // chain's selection starts there (a miss halts it for the host)
   bool near_hint = false;
+  // lists (in entries, at the last round they fed) the one-workgroup round
+  // filters itself; longer ones go through the multi-block gather
+  static constexpr long long kRoundGatherMax = 4096;
   // x_cp or z on request (named arrays "xcp", "z"): the iteration stores no
   // x_cp, and z only in the trial buffer until a trial overwrites it
   T* insp = nullptr;
@@ -377,13 +380,16 @@ struct Engine final : EngineBase {
                          ds, part, pmax, pmin, pmini, 0);
         break;
       case K_GCP: {
-        // straight from pass A's lists when the last round could (a miss
-        // halts the chain for the host's selection); otherwise the gather
-        // and the full selection, whichever covers the round, before it
-        if (!near_hint)
+        // from pass A's lists when the last round could (a miss halts the
+        // chain for the host's selection): inside the one-workgroup round
+        // while the lists are short, by the multi-block gather when they
+        // are long; otherwise the gather and the full selection, whichever
+        // covers the round
+        const bool in_round = near_hint && hs->gcp_near_listed <= kRoundGatherMax;
+        if (!in_round)
           launch_gcp_select<T>(st, G, n, dX, dG, l, u, iwhere, near_lists(), ds, ca, n,
-                               sel_count(), true);
-        launch_gcp_round_small<T>(st, ca, dG, W, ds, near_lists(), near_hint);
+                               sel_count(), !near_hint);
+        launch_gcp_round_small<T>(st, ca, dG, W, ds, near_lists(), in_round);
         // a round too large for one workgroup but within kMidRound finishes
         // here on the device; only larger ones and later ranges reach the
         // host (run_rounds).  Enqueued only while such rounds are recent:

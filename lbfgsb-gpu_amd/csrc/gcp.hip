@@ -61,14 +61,18 @@ __device__ __forceinline__ bool near_covers(const DevScalars* ds, unsigned nmax,
          nmax <= (unsigned)cap;
 }
 
-// The largest list count (every thread: its share of the G <= kMaxBlocks
-// counts, loaded together).
-__device__ __forceinline__ unsigned near_max_count(const unsigned* __restrict__ nearn, int G) {
+// The largest list count and the sum of this thread's share of the
+// G <= kMaxBlocks counts (loaded together).
+__device__ __forceinline__ unsigned near_max_count(const unsigned* __restrict__ nearn, int G,
+                                                   unsigned& sum) {
   unsigned mx = 0u;
+  sum = 0u;
 #pragma unroll
   for (int j = 0; j < kMaxBlocks / kBlock; j++) {
     const int b = threadIdx.x + j * kBlock;
-    if (b < G) mx = max(mx, nearn[b]);
+    const unsigned c = b < G ? nearn[b] : 0u;
+    mx = max(mx, c);
+    sum += c;
   }
   return mx;
 }
@@ -149,17 +153,21 @@ __device__ __forceinline__ size_t near_entry(const unsigned* off, const NearList
 __global__ void __launch_bounds__(kBlock)
 k_gcp_near_gather(const Cand* __restrict__ nearc, const unsigned* __restrict__ nearn, int G,
                   int near_cap, DevScalars* ds, Cand* cand, long long cap, int in_chain) {
-  __shared__ unsigned s_max, s_cnt;
+  __shared__ unsigned s_max, s_cnt, s_sum;
   __shared__ unsigned long long s_base;
   __shared__ Cand s_c[kGatherLists * 8];
   if (!ds->gcp_pending || ds->halt) return;
   if (threadIdx.x == 0) {
     s_max = 0u;
     s_cnt = 0u;
+    s_sum = 0u;
   }
   __syncthreads();
-  atomicMax(&s_max, near_max_count(nearn, G));
+  unsigned part;
+  atomicMax(&s_max, near_max_count(nearn, G, part));
+  atomicAdd(&s_sum, part);
   __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) ds->gcp_near_listed = (long long)s_sum;
   const bool covers = near_covers(ds, s_max, near_cap);
   if (!covers) {
     if (blockIdx.x == 0 && threadIdx.x == 0) near_miss(ds, s_max, near_cap, in_chain);

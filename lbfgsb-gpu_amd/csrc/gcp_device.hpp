@@ -332,7 +332,7 @@ __device__ inline void block_sort_small(const Cand* cand, long long K, Cand* sc)
 // Sort K candidates of a (global) with one workgroup: kSmallRound-long runs
 // in LDS, then merge-path passes between a and b.  Returns the buffer that
 // holds the sorted sequence.
-__device__ inline Cand* block_sort_large(Cand* a, Cand* b, long long K, Cand* sc) {
+__device__ LBFGSB_COLD Cand* block_sort_large(Cand* a, Cand* b, long long K, Cand* sc) {
   for (long long base = 0; base < K; base += kSmallRound) {
     const long long len = K - base < kSmallRound ? K - base : kSmallRound;
     block_sort_small(a + base, len, sc);
@@ -473,6 +473,173 @@ __device__ inline void block_round_scan(DevScalars* ds, RoundLds& L, const Cand*
                    stopped ? L.sstop_state : L.scarry, L.slast);
   }
   __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// A round of more than kSmallRound candidates in one workgroup, without a
+// full sort: the candidates are counted into kBuckets buckets by their t's
+// bit pattern (monotone in t >= 0, so equal t share a bucket and every
+// bucket holds a t-interval), scattered into b in bucket order, and then
+// taken as consecutive groups of whole buckets of at most kSmallRound
+// candidates: each group is sorted in LDS and scanned from the carried
+// state, and the first group that holds the stop concludes the round (the
+// groups after it are never sorted).  Group by group this is the scan of the
+// fully sorted round: the groups' t-intervals are disjoint and ascending, and
+// between groups the carried state, the passed count and the last passed
+// breakpoint advance as the scan's positions do.  Returns false, having
+// changed nothing, when one bucket holds more than kSmallRound candidates
+// (the caller sorts the round whole).
+static constexpr int kBuckets = 2048;
+
+struct BucketLds {
+  unsigned cnt[kBuckets];  // counts, then (scanned) each bucket's end in b
+  unsigned long long kmin, kmax;
+  unsigned big;
+};
+
+__device__ __forceinline__ unsigned long long cand_key(double t) {
+  return (unsigned long long)__double_as_longlong(t);  // t >= 0: ordered as t
+}
+
+template <typename T>
+__device__ LBFGSB_COLD bool block_round_bucketed(DevScalars* ds, RoundLds& L, BucketLds& B,
+                                            const Cand* __restrict__ a, Cand* __restrict__ b,
+                                            long long K, const T* g, const T* W) {
+  constexpr int U = 4;  // candidates per thread per step, loads issued together
+  const long long nt = blockDim.x;
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) {
+    B.kmin = ~0ull;
+    B.kmax = 0ull;
+    B.big = 0u;
+  }
+  for (int q = threadIdx.x; q < kBuckets; q += blockDim.x) B.cnt[q] = 0u;
+  __syncthreads();
+  // the key range
+  unsigned long long kmn = ~0ull, kmx = 0ull;
+  for (long long i0 = threadIdx.x; i0 < K; i0 += U * nt) {
+    double tv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) tv[u] = i0 + u * nt < K ? a[i0 + u * nt].t : -1.0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (tv[u] >= 0.0) {
+        const unsigned long long k = cand_key(tv[u]);
+        kmn = k < kmn ? k : kmn;
+        kmx = k > kmx ? k : kmx;
+      }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long on = __shfl_xor(kmn, o, kWave), ox = __shfl_xor(kmx, o, kWave);
+    kmn = on < kmn ? on : kmn;
+    kmx = ox > kmx ? ox : kmx;
+  }
+  if (lane == 0) {
+    atomicMin(&B.kmin, kmn);
+    atomicMax(&B.kmax, kmx);
+  }
+  __syncthreads();
+  const unsigned long long kmin = B.kmin, span = B.kmax - B.kmin;
+  int shift = 0;
+  while (shift < 64 && (span >> shift) >= (unsigned long long)kBuckets) shift++;
+  // counts
+  for (long long i0 = threadIdx.x; i0 < K; i0 += U * nt) {
+    double tv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) tv[u] = i0 + u * nt < K ? a[i0 + u * nt].t : -1.0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (tv[u] >= 0.0) atomicAdd(&B.cnt[(cand_key(tv[u]) - kmin) >> shift], 1u);
+  }
+  __syncthreads();
+  // a bucket too large for one group: the whole sort instead
+  for (int q = threadIdx.x; q < kBuckets; q += blockDim.x)
+    if (B.cnt[q] > (unsigned)kSmallRound) B.big = 1u;
+  __syncthreads();
+  if (B.big) return false;
+  // exclusive scan of the counts in place (each thread a contiguous run)
+  {
+    const int per = (kBuckets + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int q0 = threadIdx.x * per;
+    unsigned run = 0;
+    for (int q = q0; q < q0 + per && q < kBuckets; q++) run += B.cnt[q];
+    unsigned incl = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned up = __shfl_up(incl, o, kWave);
+      if (lane >= o) incl += up;
+    }
+    unsigned* wsum = (unsigned*)L.swave;  // one word per wave, free until the scans
+    if (lane == 63) wsum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    unsigned base = incl - run;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) base += wsum[w];
+    __syncthreads();
+    for (int q = q0; q < q0 + per && q < kBuckets; q++) {
+      const unsigned c = B.cnt[q];
+      B.cnt[q] = base;
+      base += c;
+    }
+  }
+  __syncthreads();
+  // scatter in bucket order; cnt[q] ends as bucket q's end
+  for (long long i0 = threadIdx.x; i0 < K; i0 += U * nt) {
+    Cand c[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      c[u].t = -1.0;
+      if (i0 + u * nt < K) c[u] = a[i0 + u * nt];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (c[u].t >= 0.0) b[atomicAdd(&B.cnt[(cand_key(c[u].t) - kmin) >> shift], 1u)] = c[u];
+  }
+  __syncthreads();
+  // groups of whole buckets, in order
+  unsigned start = 0;
+  int b0 = 0;
+  while (b0 < kBuckets) {
+    // the last bucket b >= b0 that still ends within kSmallRound of start
+    // (bucket ends ascend; bucket b0 alone always fits)
+    int lo = b0, hi = kBuckets - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (B.cnt[mid] - start <= (unsigned)kSmallRound)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const unsigned end = B.cnt[lo];
+    const long long len = (long long)(end - start);
+    const bool last = lo == kBuckets - 1;
+    if (len > 0) {
+      for (long long j = threadIdx.x; j < len; j += nt) L.sc[j] = b[start + j];
+      __syncthreads();
+      block_sort_small(L.sc, len, L.sc);
+      if (threadIdx.x < kNV) L.scarry[threadIdx.x] = ds->gcp_carry[threadIdx.x];
+      block_scan_range(ds, L, L.sc, 0, len, len, true, g, W);
+      const bool stopped = L.sstop != 0x7fffffffffffffffLL;
+      if (stopped || last || end >= (unsigned)K) {
+        if (threadIdx.x == 0)
+          round_conclude(ds, L.gc, L.sc, len, stopped ? L.sstop : -1,
+                         stopped ? L.sstop_state : L.scarry, L.slast);
+        __syncthreads();
+        return true;
+      }
+      // every candidate of the group passed: carry on from its end
+      if (threadIdx.x == 0) {
+        for (int q = 0; q < kNV; q++) ds->gcp_carry[q] = L.scarry[q];
+        ds->gcp_kpassed += len;
+        ds->gcp_t_prev = L.sc[len - 1].t;
+        ds->gcp_idx_prev = L.sc[len - 1].idx;
+      }
+      __syncthreads();
+    }
+    start = end;
+    b0 = lo + 1;
+  }
+  return true;
 }
 
 }  // namespace lbfgsb

@@ -60,6 +60,10 @@ def test_cmake_build_install_and_consume(tmp_path):
     for name in ("lbfgsb_minimize_f64", "lbfgsb_minimize_f32", "lbfgsb_default_options_f64",
                  "lbfgsb_minimize_noalloc_f64"):
         assert name in syms, name
+    # built for gfx950 when the configure names no architecture (the device
+    # code objects carry their target name)
+    blob = (inst / "lib" / "liblbfgsb_amd.so").read_bytes()
+    assert b"gfx950" in blob and b"gfx906" not in blob
     src = tmp_path / "consumer"
     src.mkdir()
     (src / "main.cpp").write_text(_CONSUMER)
