@@ -57,9 +57,9 @@ struct BatchLds {
     RoundLds round;
     struct {
       double sh[kMaxRoundWaves * P4_CORR];
-      double corr[kMaxRoundWaves][136];
+      double gram[kMaxRoundWaves][256];  // the Cauchy-point pass's signed Grams
       long long shi[kMaxRoundWaves];
-      double wrow[kMaxRoundWaves][2 * kMaxM];  // the Cauchy-point pass's W rows (corr_step)
+      double wrow[kMaxRoundWaves][kRowBuf];  // its W rows (corr_step)
     } r;
   } u;
 };
@@ -344,6 +344,37 @@ __device__ __forceinline__ void b_active(const Prob<T>& P, BatchLds& L, Gang<GO>
   __syncthreads();
 }
 
+// This thread's share of f over terms [k0, k1) of an objective of kind
+// KIND, with the gradient stored: kLightU terms per thread per step, every
+// one evaluated before the first gradient store (the terms' loads go out
+// together), added to the sum in the order a one-term loop adds them.  Out
+// of line, like the control routines: inlined, the five kinds' loops shared
+// the kernel's register allocation (20.1k against 18.6k solves/s, fp32).
+template <int KIND, typename T>
+__device__ __noinline__ double obj_terms_sum(const ObjDesc& o, long long n, const T* x, T* g,
+                                                long long k0, long long k1) {
+  const long long nt = blockDim.x;
+  double acc = 0.0;
+  for (long long kb = k0 + threadIdx.x; kb < k1; kb += kLightU * nt) {
+    ObjOut<T> r[kLightU];
+    double fk[kLightU];
+#pragma unroll
+    for (int q = 0; q < kLightU; q++) {
+      const long long k = kb + q * nt;
+      fk[q] = obj_eval_kind<KIND>(o, n, x, k < k1 ? k : k1 - 1, r[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kLightU; q++) {
+      const long long k = kb + q * nt;
+      if (k < k1) {
+        obj_store_kind<KIND>(g, k, r[q]);
+        acc += fk[q];
+      }
+    }
+  }
+  return acc;
+}
+
 // Objective at X[which] into G[which]; L.f = f rounded to the bulk type, as
 // a callback returning `real& f` would hand it over.
 // In a gang, member j evaluates the terms of its segment (ROSEN: the pairs
@@ -364,26 +395,24 @@ __device__ __forceinline__ void b_objective(const Prob<T>& P, BatchLds& L, int w
   const bool pairs = P.obj.kind == OBJK_ROSEN;
   const long long k0 = pairs ? P.lo / 2 : P.lo;
   const long long k1 = pairs ? P.hi / 2 : P.hi;
-  const long long nt = blockDim.x;
-  // kLightU terms per thread per step, every one evaluated before the first
-  // gradient store (the terms' loads go out together), added to the sum in
-  // the order a one-term loop adds them
-  for (long long kb = k0 + threadIdx.x; kb < k1; kb += kLightU * nt) {
-    ObjOut<T> r[kLightU];
-    double fk[kLightU];
-#pragma unroll
-    for (int q = 0; q < kLightU; q++) {
-      const long long k = kb + q * nt;
-      fk[q] = obj_eval(P.obj, P.n, x, k < k1 ? k : k1 - 1, r[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < kLightU; q++) {
-      const long long k = kb + q * nt;
-      if (k < k1) {
-        obj_store(P.obj, g, k, r[q]);
-        acc[0] += fk[q];
-      }
-    }
+  // one loop per kind: its body is straight-line code, so the terms' loads
+  // can go out together
+  switch (P.obj.kind) {
+    case OBJK_GRID_QUAD:
+      acc[0] = obj_terms_sum<OBJK_GRID_QUAD>(P.obj, P.n, x, g, k0, k1);
+      break;
+    case OBJK_GRID_COS:
+      acc[0] = obj_terms_sum<OBJK_GRID_COS>(P.obj, P.n, x, g, k0, k1);
+      break;
+    case OBJK_DSSCFG:
+      acc[0] = obj_terms_sum<OBJK_DSSCFG>(P.obj, P.n, x, g, k0, k1);
+      break;
+    case OBJK_ROSEN:
+      acc[0] = obj_terms_sum<OBJK_ROSEN>(P.obj, P.n, x, g, k0, k1);
+      break;
+    default:
+      acc[0] = obj_terms_sum<OBJK_PENTA>(P.obj, P.n, x, g, k0, k1);
+      break;
   }
   bsum<1>(acc, L.u.r.sh, L.red);
   gang_combine(G, L, 1, 0, 0, 0, 0, &L.gflag);
@@ -453,20 +482,22 @@ __device__ __forceinline__ void b_p4(const Prob<T>& P, BatchLds& L, Gang<GO>& G)
     op.corr_step(i, i < hi ? op.element(i, e) : 0, e, rowbuf);
   }
   bsum<P4_CORR>(op.acc, L.u.r.sh, L.red);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const bool tracked = op.track && op.col > 0;
   if (tracked) {
-#pragma unroll
-    for (int q = 0; q < 3; q++)
-      if (lane + 64 * q < 136) L.u.r.corr[wid][lane + 64 * q] = op.corr[q];
+    op.corr_finish(rowbuf);
+    op.corr_store(L.u.r.gram[threadIdx.x >> 6]);
+    __syncthreads();
+    // the waves' Grams summed in wave order
+    for (int q = threadIdx.x; q < 256; q += blockDim.x) {
+      double s = L.u.r.gram[0][q];
+      for (int w = 1; w < nwaves(); w++) s += L.u.r.gram[w][q];
+      L.u.r.gram[0][q] = s;
+    }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < 136; t += blockDim.x) {
-    double s = 0.0;
-    if (tracked)
-      for (int w = 0; w < nwaves(); w++) s += L.u.r.corr[w][t];
-    L.red[P4_CORR + t] = s;
-  }
+  const int upcl = L.s.updatd ? op.col - 1 : op.col;
+  for (int t = threadIdx.x; t < 136; t += blockDim.x)
+    L.red[P4_CORR + t] = tracked ? corr_pair_value(t, L.u.r.gram[0], upcl) : 0.0;
   __syncthreads();
   gang_combine(G, L, P4_COUNT, 0, 0, 0, 0, &L.gflag);
 }

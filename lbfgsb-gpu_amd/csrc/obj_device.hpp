@@ -29,12 +29,13 @@ struct ObjOut {
   T g0, g1;
 };
 
-// Term k: its share of f, with its gradient entries in out (not stored, so
-// that a caller can evaluate several terms before the first store).
-template <typename T>
-__device__ inline double obj_eval(const ObjDesc& o, long long n, const T* __restrict__ x,
-                                  long long k, ObjOut<T>& out) {
-  if (o.kind == OBJK_GRID_QUAD || o.kind == OBJK_GRID_COS) {
+// Term k of an objective of kind KIND: its share of f, with its gradient
+// entries in out (not stored, so that a caller can evaluate several terms
+// before the first store).
+template <int KIND, typename T>
+__device__ inline double obj_eval_kind(const ObjDesc& o, long long n, const T* __restrict__ x,
+                                       long long k, ObjOut<T>& out) {
+  if constexpr (KIND == OBJK_GRID_QUAD || KIND == OBJK_GRID_COS) {
     const T* b = (const T*)o.b;
     const double h2 = o.h * o.h;
     const long long nx = o.nx, ny = o.ny;
@@ -51,15 +52,14 @@ __device__ inline double obj_eval(const ObjDesc& o, long long n, const T* __rest
     if (ix == 0) e += v * v;
     if (iy == 0) e += v * v;
     double fk = 0.5 * e - h2 * bk * v;
-    if (o.kind == OBJK_GRID_COS) {
+    if constexpr (KIND == OBJK_GRID_COS) {
       const double ck = (double)((const T*)o.c)[k];
       fk += h2 * ck * cos(v);
       gk -= h2 * ck * sin(v);
     }
     out.g0 = (T)gk;
     return fk;
-  }
-  if (o.kind == OBJK_DSSCFG) {
+  } else if constexpr (KIND == OBJK_DSSCFG) {
     // Point k owns its lower triangle (v, right, top) and its upper one
     // (v, left, bottom).  Its gradient gathers the five contributions the
     // reference example scatters (its own, then from the bottom, left,
@@ -88,8 +88,7 @@ __device__ inline double obj_eval(const ObjDesc& o, long long n, const T* __rest
     const double fq = dxl * dxl + dyl * dyl + (dxu * dxu + dyu * dyu);
     const double fe = lam * (ev + exp(vr) + exp(vt)) / 3.0 + lam * (exp(vb) + exp(vl) + ev) / 3.0;
     return area * (0.5 * fq - fe);
-  }
-  if (o.kind == OBJK_ROSEN) {
+  } else if constexpr (KIND == OBJK_ROSEN) {
     // the pair (x_{2k}, x_{2k+1}) as one 2-vector load: a wave reads one
     // contiguous run instead of two strided ones
     using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
@@ -99,19 +98,52 @@ __device__ inline double obj_eval(const ObjDesc& o, long long n, const T* __rest
     out.g0 = (T)(-400.0 * t1 * a - 2.0 * t2);
     out.g1 = (T)(200.0 * t1);
     return 100.0 * t1 * t1 + t2 * t2;
+  } else {
+    const T* a0 = (const T*)o.a0;
+    const T* a1 = (const T*)o.a1;
+    const T* a2 = (const T*)o.a2;
+    const T* b = (const T*)o.b;
+    const double xi = (double)x[k];
+    double ax = (double)a0[k] * xi;
+    if (k + 1 < n) ax += (double)a1[k] * (double)x[k + 1];
+    if (k + 2 < n) ax += (double)a2[k] * (double)x[k + 2];
+    if (k >= 1) ax += (double)a1[k - 1] * (double)x[k - 1];
+    if (k >= 2) ax += (double)a2[k - 2] * (double)x[k - 2];
+    out.g0 = (T)(ax - (double)b[k]);
+    return 0.5 * ax * xi - (double)b[k] * xi;
   }
-  const T* a0 = (const T*)o.a0;
-  const T* a1 = (const T*)o.a1;
-  const T* a2 = (const T*)o.a2;
-  const T* b = (const T*)o.b;
-  const double xi = (double)x[k];
-  double ax = (double)a0[k] * xi;
-  if (k + 1 < n) ax += (double)a1[k] * (double)x[k + 1];
-  if (k + 2 < n) ax += (double)a2[k] * (double)x[k + 2];
-  if (k >= 1) ax += (double)a1[k - 1] * (double)x[k - 1];
-  if (k >= 2) ax += (double)a2[k - 2] * (double)x[k - 2];
-  out.g0 = (T)(ax - (double)b[k]);
-  return 0.5 * ax * xi - (double)b[k] * xi;
+}
+
+// Term k of the objective o (any kind).
+template <typename T>
+__device__ inline double obj_eval(const ObjDesc& o, long long n, const T* __restrict__ x,
+                                  long long k, ObjOut<T>& out) {
+  switch (o.kind) {
+    case OBJK_GRID_QUAD:
+      return obj_eval_kind<OBJK_GRID_QUAD>(o, n, x, k, out);
+    case OBJK_GRID_COS:
+      return obj_eval_kind<OBJK_GRID_COS>(o, n, x, k, out);
+    case OBJK_DSSCFG:
+      return obj_eval_kind<OBJK_DSSCFG>(o, n, x, k, out);
+    case OBJK_ROSEN:
+      return obj_eval_kind<OBJK_ROSEN>(o, n, x, k, out);
+    default:
+      return obj_eval_kind<OBJK_PENTA>(o, n, x, k, out);
+  }
+}
+
+// Stores the gradient entries of term k (kind KIND).
+template <int KIND, typename T>
+__device__ inline void obj_store_kind(T* __restrict__ g, long long k, const ObjOut<T>& out) {
+  if constexpr (KIND == OBJK_ROSEN) {
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+    V2 gv;
+    gv.x = out.g0;
+    gv.y = out.g1;
+    reinterpret_cast<V2*>(g)[k] = gv;
+  } else {
+    g[k] = out.g0;
+  }
 }
 
 // Stores the gradient entries of term k.

@@ -13,6 +13,24 @@
 
 namespace lbfgsb {
 
+// FP64 MFMA accumulator (four doubles per lane).
+typedef double GramAcc __attribute__((ext_vector_type(4)));
+// doubles of LDS per wave for the Cauchy-point pass's Gram rows
+constexpr int kRowBuf = 68;
+
+// Value of signed Gram pair t (< 136: YY lower, SS lower, SY; corr_pair)
+// from the summed 16 x 16 Gram g (row-major), over the memory positions
+// below upcl (0 for the others).
+__device__ __forceinline__ double corr_pair_value(int t, const double* g, int upcl) {
+  int cat = 0, a = 0, b = 0;
+  corr_pair(t, cat, a, b);
+  if (a >= upcl || b >= upcl) return 0.0;
+  // YY (cat 0), SS (cat 1), SY (cat 2: a the S position, b the Y position)
+  const int ra = cat == 0 ? a : kMaxM + a;
+  const int rb = cat == 1 ? kMaxM + b : b;
+  return g[ra * 16 + rb];
+}
+
 // ---------------------------------------------------------------------------
 // Pass A: the iteration tail (projected gradient norm, ||x - x_old||^2,
 // y = g - g_old, y'y, the new S/Y pair written at ds->itail and its Gram row
@@ -317,12 +335,12 @@ struct P4Op {
   CauchyOutcome<T> co;
   RingReads rr;
   double acc[kAcc];
-  // signed Gram pairs over entering (+1) / leaving (-1) variables: lane L of
-  // every wave owns pairs L, L+64, L+128 (< 136); wa/wb are the W columns
-  // (0..7 = Y slots, 8..15 = S slots) of the pair's two factors
-  double corr[3];
-  int ra[3], rb[3];  // positions in rowbuf of the pair's two factors
-  bool pact[3];
+  // signed Gram of this wave's entering (+1) / leaving (-1) variables,
+  // G = sum_f sign_f w_f w_f' over their rows w_f = [y_0..7, s_0..7]
+  // (16 x 16, an FP64 MFMA accumulator: G[i][j] in lane j + 16 (i % 4),
+  // element i / 4), fed four rows at a time (npend rows waiting in rowbuf)
+  GramAcc gacc;
+  int npend;
 
   struct In {
     T x, g, l, u;
@@ -347,23 +365,11 @@ struct P4Op {
     co.init(ds);
     track = ds->iter > 0 && ds->cnstnd && !shortcut;
     newcol = ds->updatd && col > 0;
-    const int upcl = ds->updatd ? col - 1 : col;
     rr = ring_reads(head, m, col, -1, ds->pslot_x);
 #pragma unroll
     for (int v = 0; v < kAcc; v++) acc[v] = 0.0;
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-      corr[q] = 0.0;
-      const int t = lane + 64 * q;
-      int cat = 0, a = 0, b = 0;
-      if (t < 136) corr_pair(t, cat, a, b);
-      pact[q] = t < 136 && a < upcl && b < upcl;
-      // the pair's two factors by memory position: YY (cat 0), SS (cat 1),
-      // SY (cat 2, a the S position, b the Y position)
-      ra[q] = cat == 0 ? a : kMaxM + a;
-      rb[q] = cat == 1 ? kMaxM + b : b;
-    }
+    gacc = GramAcc{0.0, 0.0, 0.0, 0.0};
+    npend = 0;
   }
   __device__ __forceinline__ void load(In& e, long long i) const {
     e.x = x[i];
@@ -431,13 +437,12 @@ struct P4Op {
     }
     return flag;
   }
-  // Signed Gram over this wave's entering / leaving variables, one flagged
-  // variable at a time in lane order (all lanes of the wave take part).
-  // The flagged variable's lane has its W row in registers (e, masked by
-  // element()); it puts the row into the wave's 16 doubles of LDS (rowbuf:
-  // Y positions 0..7, S positions 8..15) and every lane reads the two
-  // factors of its pairs from there, so no flagged variable waits for a
-  // load from memory.
+  // Signed Gram over this wave's entering / leaving variables (all lanes of
+  // the wave take part).  The flagged variables' lanes have their W rows in
+  // registers (e, masked by element()); each puts its row and sign into the
+  // wave's rowbuf (kRowBuf doubles of LDS: four rows of 16, then the four
+  // signs), and every four rows go into the accumulator with one FP64 MFMA
+  // (16 x 16 x 4): no flagged variable waits for a load from memory.
   __device__ __forceinline__ void corr_step(long long i, int flag, const In& e, double* rowbuf) {
     if (!(track && col > 0)) return;
     const int lane = threadIdx.x & 63;
@@ -446,25 +451,49 @@ struct P4Op {
     while (fm) {
       const int src = __ffsll((long long)fm) - 1;
       fm &= fm - 1ull;
-      const double sg = (double)__shfl(flag, src, kWave);
       if (lane == src) {
 #pragma unroll
         for (int j = 0; j < kMaxM; j++) {
-          rowbuf[j] = (double)e.yv[j];
-          rowbuf[kMaxM + j] = (double)e.sv[j];
+          rowbuf[npend * 16 + j] = (double)e.yv[j];
+          rowbuf[npend * 16 + kMaxM + j] = (double)e.sv[j];
         }
+        rowbuf[64 + npend] = (double)flag;
       }
-      // the wave's LDS operations complete in order: the reads below see the
-      // row (the wave barrier keeps the compiler from moving them up)
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        const double va = rowbuf[ra[q]];
-        const double vb = rowbuf[rb[q]];
-        if (pact[q]) corr[q] += sg * va * vb;
+      if (++npend == 4) {
+        gram_feed(rowbuf, 4);
+        npend = 0;
       }
-      __builtin_amdgcn_wave_barrier();
     }
+  }
+  // The rows still waiting, with zero rows for the rest of the four.
+  __device__ __forceinline__ void corr_finish(double* rowbuf) {
+    if (!(track && col > 0) || npend == 0) return;
+    gram_feed(rowbuf, npend);
+    npend = 0;
+  }
+  // Four rows of rowbuf (the first `rows` of them) into the accumulator:
+  // A[i][k] = w_k[i] and B[k][j] = sign_k w_k[j] sit in lane i + 16 k and
+  // j + 16 k alike, so lane l takes rowbuf[l] (exact in FP64: the products
+  // of two bulk values are).
+  __device__ __forceinline__ void gram_feed(const double* rowbuf, int rows) {
+    const int lane = threadIdx.x & 63, k = lane >> 4;
+    // the wave's LDS operations complete in order: the rows are in place
+    // (the wave barriers keep the compiler from moving the reads up, or the
+    // next rows' writes above them)
+    __builtin_amdgcn_wave_barrier();
+    // rows past `rows` are zero in both operands (their LDS is stale: the
+    // sign slot may hold anything, a NaN included)
+    const bool live = k < rows;
+    const double a = live ? rowbuf[lane] : 0.0;
+    const double b = live ? a * rowbuf[64 + k] : 0.0;
+    gacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, gacc, 0, 0, 0);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // This wave's Gram into out[16 * i + j] (256 doubles of LDS).
+  __device__ __forceinline__ void corr_store(double* out) const {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int r = 0; r < 4; r++) out[((lane >> 4) + 4 * r) * 16 + (lane & 15)] = gacc[r];
   }
 };
 
