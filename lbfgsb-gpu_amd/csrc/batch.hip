@@ -57,7 +57,7 @@ struct BatchLds {
     RoundLds round;
     struct {
       double sh[kMaxRoundWaves * P4_CORR];
-      double gram[kMaxRoundWaves][256];  // the Cauchy-point pass's signed Grams
+      double gram[256];  // the Cauchy-point pass's signed Gram
       long long shi[kMaxRoundWaves];
       double wrow[kMaxRoundWaves][kRowBuf];  // its W rows (corr_step)
     } r;
@@ -483,21 +483,11 @@ __device__ __forceinline__ void b_p4(const Prob<T>& P, BatchLds& L, Gang<GO>& G)
   }
   bsum<P4_CORR>(op.acc, L.u.r.sh, L.red);
   const bool tracked = op.track && op.col > 0;
-  if (tracked) {
-    op.corr_finish(rowbuf);
-    op.corr_store(L.u.r.gram[threadIdx.x >> 6]);
-    __syncthreads();
-    // the waves' Grams summed in wave order
-    for (int q = threadIdx.x; q < 256; q += blockDim.x) {
-      double s = L.u.r.gram[0][q];
-      for (int w = 1; w < nwaves(); w++) s += L.u.r.gram[w][q];
-      L.u.r.gram[0][q] = s;
-    }
-  }
+  if (tracked) op.corr_block_sum(L.u.r.gram, nwaves());
   __syncthreads();
   const int upcl = L.s.updatd ? op.col - 1 : op.col;
   for (int t = threadIdx.x; t < 136; t += blockDim.x)
-    L.red[P4_CORR + t] = tracked ? corr_pair_value(t, L.u.r.gram[0], upcl) : 0.0;
+    L.red[P4_CORR + t] = tracked ? corr_pair_value(t, L.u.r.gram, upcl) : 0.0;
   __syncthreads();
   gang_combine(G, L, P4_COUNT, 0, 0, 0, 0, &L.gflag);
 }

@@ -221,7 +221,7 @@ k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
      const T* __restrict__ u, uint8_t* __restrict__ iwhere, const T* __restrict__ W, DevScalars* ds,
      double* part, int rev) {
   __shared__ double sh[kWavesPerBlock * P4_CORR];
-  __shared__ double s_gram[kWavesPerBlock][256];
+  __shared__ double s_gram[256];
   __shared__ double s_row[kWavesPerBlock][kRowBuf];
   if (ds->halt) return;
   double* rowbuf = s_row[threadIdx.x >> 6];
@@ -264,20 +264,10 @@ k_p4(long long n, Pair<T> X, Pair<T> Gv, const T* __restrict__ l,
   }
   block_sum_store<P4_CORR>(op.acc, sh, part + blockIdx.x, gridDim.x);
   if (op.track && op.col > 0) {
-    op.corr_finish(rowbuf);
-    op.corr_store(s_gram[threadIdx.x >> 6]);
-    __syncthreads();
-    // the waves' Grams summed in wave order, then the block's pairs
-    for (int q = threadIdx.x; q < 256; q += blockDim.x) {
-      double s = s_gram[0][q];
-#pragma unroll
-      for (int w = 1; w < kWavesPerBlock; w++) s += s_gram[w][q];
-      s_gram[0][q] = s;
-    }
-    __syncthreads();
+    op.corr_block_sum(s_gram, kWavesPerBlock);
     const int upcl = ds->updatd ? op.col - 1 : op.col;
     for (int t = threadIdx.x; t < 136; t += blockDim.x)
-      part[(size_t)(P4_CORR + t) * gridDim.x + blockIdx.x] = corr_pair_value(t, s_gram[0], upcl);
+      part[(size_t)(P4_CORR + t) * gridDim.x + blockIdx.x] = corr_pair_value(t, s_gram, upcl);
   }
 }
 

@@ -15,8 +15,10 @@ namespace lbfgsb {
 
 // FP64 MFMA accumulator (four doubles per lane).
 typedef double GramAcc __attribute__((ext_vector_type(4)));
-// doubles of LDS per wave for the Cauchy-point pass's Gram rows
-constexpr int kRowBuf = 68;
+// rows of 16 the Cauchy-point pass stages per wave for its Gram, and the
+// doubles of LDS per wave that holds them (the rows, then their signs)
+constexpr int kRowRows = 8;
+constexpr int kRowBuf = kRowRows * 16 + kRowRows;
 
 // Value of signed Gram pair t (< 136: YY lower, SS lower, SY; corr_pair)
 // from the summed 16 x 16 Gram g (row-major), over the memory positions
@@ -338,9 +340,8 @@ struct P4Op {
   // signed Gram of this wave's entering (+1) / leaving (-1) variables,
   // G = sum_f sign_f w_f w_f' over their rows w_f = [y_0..7, s_0..7]
   // (16 x 16, an FP64 MFMA accumulator: G[i][j] in lane j + 16 (i % 4),
-  // element i / 4), fed four rows at a time (npend rows waiting in rowbuf)
+  // element i / 4), fed four rows at a time
   GramAcc gacc;
-  int npend;
 
   struct In {
     T x, g, l, u;
@@ -369,7 +370,6 @@ struct P4Op {
 #pragma unroll
     for (int v = 0; v < kAcc; v++) acc[v] = 0.0;
     gacc = GramAcc{0.0, 0.0, 0.0, 0.0};
-    npend = 0;
   }
   __device__ __forceinline__ void load(In& e, long long i) const {
     e.x = x[i];
@@ -439,43 +439,40 @@ struct P4Op {
   }
   // Signed Gram over this wave's entering / leaving variables (all lanes of
   // the wave take part).  The flagged variables' lanes have their W rows in
-  // registers (e, masked by element()); each puts its row and sign into the
-  // wave's rowbuf (kRowBuf doubles of LDS: four rows of 16, then the four
-  // signs), and every four rows go into the accumulator with one FP64 MFMA
-  // (16 x 16 x 4): no flagged variable waits for a load from memory.
+  // registers (e, masked by element()); up to kRowRows of them at a time put
+  // their rows and signs into the wave's rowbuf (kRowBuf doubles of LDS) at
+  // their rank among the flagged lanes, and every four rows go into the
+  // accumulator with one FP64 MFMA (16 x 16 x 4): no flagged variable waits
+  // for a load from memory, and a wave whose lanes are all flagged (a free
+  // set that changes wholesale) stages its rows eight at a time.
   __device__ __forceinline__ void corr_step(long long i, int flag, const In& e, double* rowbuf) {
     if (!(track && col > 0)) return;
     const int lane = threadIdx.x & 63;
-    unsigned long long fm = __ballot(flag != 0);
-    if (fm && lane == 0) ds->p4_flagged = 1;
-    while (fm) {
-      const int src = __ffsll((long long)fm) - 1;
-      fm &= fm - 1ull;
-      if (lane == src) {
+    const unsigned long long fm = __ballot(flag != 0);
+    if (!fm) return;
+    if (lane == 0) ds->p4_flagged = 1;
+    const int nf = __popcll(fm);
+    const int rank = __popcll(fm & ((1ull << lane) - 1ull));
+    for (int base = 0; base < nf; base += kRowRows) {
+      if (flag != 0 && rank >= base && rank < base + kRowRows) {
+        const int r = rank - base;
 #pragma unroll
         for (int j = 0; j < kMaxM; j++) {
-          rowbuf[npend * 16 + j] = (double)e.yv[j];
-          rowbuf[npend * 16 + kMaxM + j] = (double)e.sv[j];
+          rowbuf[r * 16 + j] = (double)e.yv[j];
+          rowbuf[r * 16 + kMaxM + j] = (double)e.sv[j];
         }
-        rowbuf[64 + npend] = (double)flag;
+        rowbuf[kRowRows * 16 + r] = (double)flag;
       }
-      if (++npend == 4) {
-        gram_feed(rowbuf, 4);
-        npend = 0;
-      }
+      const int rows = nf - base < kRowRows ? nf - base : kRowRows;
+      for (int q = 0; q < rows; q += 4)
+        gram_feed(rowbuf + q * 16, rowbuf + kRowRows * 16 + q, rows - q);
     }
   }
-  // The rows still waiting, with zero rows for the rest of the four.
-  __device__ __forceinline__ void corr_finish(double* rowbuf) {
-    if (!(track && col > 0) || npend == 0) return;
-    gram_feed(rowbuf, npend);
-    npend = 0;
-  }
-  // Four rows of rowbuf (the first `rows` of them) into the accumulator:
-  // A[i][k] = w_k[i] and B[k][j] = sign_k w_k[j] sit in lane i + 16 k and
-  // j + 16 k alike, so lane l takes rowbuf[l] (exact in FP64: the products
-  // of two bulk values are).
-  __device__ __forceinline__ void gram_feed(const double* rowbuf, int rows) {
+  // Four rows at rows16 (16 doubles each; signs at sgn) into the accumulator,
+  // the rows from `rows` on taken as zero: A[i][k] = w_k[i] and
+  // B[k][j] = sign_k w_k[j] sit in lane i + 16 k and j + 16 k alike, so lane
+  // l takes rows16[l] (exact in FP64: the products of two bulk values are).
+  __device__ __forceinline__ void gram_feed(const double* rows16, const double* sgn, int rows) {
     const int lane = threadIdx.x & 63, k = lane >> 4;
     // the wave's LDS operations complete in order: the rows are in place
     // (the wave barriers keep the compiler from moving the reads up, or the
@@ -484,16 +481,26 @@ struct P4Op {
     // rows past `rows` are zero in both operands (their LDS is stale: the
     // sign slot may hold anything, a NaN included)
     const bool live = k < rows;
-    const double a = live ? rowbuf[lane] : 0.0;
-    const double b = live ? a * rowbuf[64 + k] : 0.0;
+    const double a = live ? rows16[lane] : 0.0;
+    const double b = live ? a * sgn[k] : 0.0;
     gacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, gacc, 0, 0, 0);
     __builtin_amdgcn_wave_barrier();
   }
-  // This wave's Gram into out[16 * i + j] (256 doubles of LDS).
-  __device__ __forceinline__ void corr_store(double* out) const {
-    const int lane = threadIdx.x & 63;
+  // The workgroup's Gram into gram[16 * i + j] (256 doubles of LDS): the
+  // waves' accumulators added in wave order, one wave after another.  All
+  // threads of the workgroup call it (it holds nwaves barriers).
+  __device__ __forceinline__ void corr_block_sum(double* gram, int nwaves) const {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int w = 0; w < nwaves; w++) {
+      if (wid == w) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) out[((lane >> 4) + 4 * r) * 16 + (lane & 15)] = gacc[r];
+        for (int r = 0; r < 4; r++) {
+          const int q = ((lane >> 4) + 4 * r) * 16 + (lane & 15);
+          gram[q] = w == 0 ? gacc[r] : gram[q] + gacc[r];
+        }
+      }
+      __syncthreads();
+    }
   }
 };
 
