@@ -375,6 +375,78 @@ __device__ __noinline__ double obj_terms_sum(const ObjDesc& o, long long n, cons
   return acc;
 }
 
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  typedef float4 type;
+};
+template <>
+struct Vec4<double> {
+  typedef double4 type;
+};
+
+// The pentadiagonal terms [k0, k1) (k0 a multiple of four), four consecutive
+// terms per thread: x, a1 and a2 at a term's neighbours come from the same
+// 4-vector loads as its own entries (one load of each vector instead of
+// eleven scalar loads per term; the arrays of a problem start 64-element
+// aligned and run to a multiple of 64).  Each term's gradient and f share
+// are formed as obj_eval_kind forms them, in the same order.
+template <typename T>
+__device__ __noinline__ double obj_penta_sum4(const ObjDesc& o, long long n,
+                                              const T* __restrict__ x, T* __restrict__ g,
+                                              long long k0, long long k1) {
+  using V = typename Vec4<T>::type;
+  const V* __restrict__ xv = (const V*)x;
+  const V* __restrict__ a0v = (const V*)o.a0;
+  const V* __restrict__ a1v = (const V*)o.a1;
+  const V* __restrict__ a2v = (const V*)o.a2;
+  const V* __restrict__ bv = (const V*)o.b;
+  const V zero = {(T)0, (T)0, (T)0, (T)0};
+  double acc = 0.0;
+  const long long j1 = (k1 + 3) / 4;
+  for (long long j = k0 / 4 + threadIdx.x; j < j1; j += blockDim.x) {
+    const long long k = 4 * j;
+    // the neighbour vectors only where they lie inside the problem
+    const bool prev = k >= 4, next = k + 4 < n;
+    const V xc = xv[j], xm = prev ? xv[j - 1] : zero, xp = next ? xv[j + 1] : zero;
+    const V A0 = a0v[j], A1 = a1v[j], A2 = a2v[j], B = bv[j];
+    const V A1m = prev ? a1v[j - 1] : zero, A2m = prev ? a2v[j - 1] : zero;
+    // x[k-2 .. k+5], a1[k-1 .. k+3], a2[k-2 .. k+3]
+    const T xs[8] = {xm.z, xm.w, xc.x, xc.y, xc.z, xc.w, xp.x, xp.y};
+    const T a1s[5] = {A1m.w, A1.x, A1.y, A1.z, A1.w};
+    const T a2s[6] = {A2m.z, A2m.w, A2.x, A2.y, A2.z, A2.w};
+    const T a0s[4] = {A0.x, A0.y, A0.z, A0.w};
+    const T bs[4] = {B.x, B.y, B.z, B.w};
+    T go[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const long long ke = k + e;
+      const double xi = (double)xs[2 + e];
+      double ax = (double)a0s[e] * xi;
+      if (ke + 1 < n) ax += (double)a1s[1 + e] * (double)xs[3 + e];
+      if (ke + 2 < n) ax += (double)a2s[2 + e] * (double)xs[4 + e];
+      if (ke >= 1) ax += (double)a1s[e] * (double)xs[1 + e];
+      if (ke >= 2) ax += (double)a2s[e] * (double)xs[e];
+      go[e] = (T)(ax - (double)bs[e]);
+      if (ke < k1) acc += 0.5 * ax * xi - (double)bs[e] * xi;
+    }
+    if (k + 4 <= k1) {
+      V gv;
+      gv.x = go[0];
+      gv.y = go[1];
+      gv.z = go[2];
+      gv.w = go[3];
+      reinterpret_cast<V*>(g)[j] = gv;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (k + e < k1) g[k + e] = go[e];
+    }
+  }
+  return acc;
+}
+
 // Objective at X[which] into G[which]; L.f = f rounded to the bulk type, as
 // a callback returning `real& f` would hand it over.
 // In a gang, member j evaluates the terms of its segment (ROSEN: the pairs
@@ -411,7 +483,8 @@ __device__ __forceinline__ void b_objective(const Prob<T>& P, BatchLds& L, int w
       acc[0] = obj_terms_sum<OBJK_ROSEN>(P.obj, P.n, x, g, k0, k1);
       break;
     default:
-      acc[0] = obj_terms_sum<OBJK_PENTA>(P.obj, P.n, x, g, k0, k1);
+      acc[0] = (k0 & 3) == 0 ? obj_penta_sum4<T>(P.obj, P.n, x, g, k0, k1)
+                             : obj_terms_sum<OBJK_PENTA>(P.obj, P.n, x, g, k0, k1);
       break;
   }
   bsum<1>(acc, L.u.r.sh, L.red);
