@@ -447,6 +447,95 @@ __device__ __noinline__ double obj_penta_sum4(const ObjDesc& o, long long n,
   return acc;
 }
 
+// The grid terms [k0, k1) (k0 a multiple of four) of a GRID_QUAD or GRID_COS
+// objective, four consecutive points of a row-major grid per thread: the
+// points' own x, b and c as 4-vectors, the left and right neighbours from
+// the same vector plus one load at each end, the rows below and above as
+// 4-vectors where rows start 4-aligned (nx a multiple of four), else one
+// load per point.  Each term's gradient and f share are formed as
+// obj_eval_kind forms them, in the same order.
+template <int KIND, typename T>
+__device__ __noinline__ double obj_grid_sum4(const ObjDesc& o, long long n,
+                                             const T* __restrict__ x, T* __restrict__ g,
+                                             long long k0, long long k1) {
+  using V = typename Vec4<T>::type;
+  const V* __restrict__ xv = (const V*)x;
+  const V* __restrict__ bv = (const V*)o.b;
+  const V* __restrict__ cv = (const V*)o.c;
+  const long long nx = o.nx, ny = o.ny;
+  const double h2 = o.h * o.h;
+  const bool rows4 = (nx & 3) == 0;
+  const V zero = {(T)0, (T)0, (T)0, (T)0};
+  double acc = 0.0;
+  const long long j1 = (k1 + 3) / 4;
+  for (long long j = k0 / 4 + threadIdx.x; j < j1; j += blockDim.x) {
+    const long long k = 4 * j;
+    const V xc = xv[j], B = bv[j];
+    V C = zero;
+    if constexpr (KIND == OBJK_GRID_COS) C = cv[j];
+    // x[k-1 .. k+4]
+    const T xs[6] = {k >= 1 ? x[k - 1] : (T)0, xc.x, xc.y, xc.z, xc.w, k + 4 < n ? x[k + 4] : (T)0};
+    T dn[4], up[4];
+    if (rows4) {
+      const V d = k >= nx ? xv[(k - nx) / 4] : zero;
+      const V u = k + nx < n ? xv[(k + nx) / 4] : zero;
+      dn[0] = d.x, dn[1] = d.y, dn[2] = d.z, dn[3] = d.w;
+      up[0] = u.x, up[1] = u.y, up[2] = u.z, up[3] = u.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        dn[e] = k + e >= nx && k + e < n ? x[k + e - nx] : (T)0;
+        up[e] = k + e + nx < n ? x[k + e + nx] : (T)0;
+      }
+    }
+    const T bs[4] = {B.x, B.y, B.z, B.w};
+    const T cs[4] = {C.x, C.y, C.z, C.w};
+    long long ix, iy;
+    grid_pos(k, n, nx, ix, iy);
+    T go[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      if (e > 0) ix++;
+      while (ix >= nx) {
+        ix -= nx;
+        iy++;
+      }
+      const double v = (double)xs[1 + e];
+      const double vl = ix > 0 ? (double)xs[e] : 0.0;
+      const double vr = ix < nx - 1 ? (double)xs[2 + e] : 0.0;
+      const double vd = iy > 0 ? (double)dn[e] : 0.0;
+      const double vu = iy < ny - 1 ? (double)up[e] : 0.0;
+      const double bk = (double)bs[e];
+      double gk = (4.0 * v - vl - vr - vd - vu) - h2 * bk;
+      const double er = vr - v, eu = vu - v;
+      double ee = er * er + eu * eu;
+      if (ix == 0) ee += v * v;
+      if (iy == 0) ee += v * v;
+      double fk = 0.5 * ee - h2 * bk * v;
+      if constexpr (KIND == OBJK_GRID_COS) {
+        const double ck = (double)cs[e];
+        fk += h2 * ck * cos(v);
+        gk -= h2 * ck * sin(v);
+      }
+      go[e] = (T)gk;
+      if (k + e < k1) acc += fk;
+    }
+    if (k + 4 <= k1) {
+      V gv;
+      gv.x = go[0];
+      gv.y = go[1];
+      gv.z = go[2];
+      gv.w = go[3];
+      reinterpret_cast<V*>(g)[j] = gv;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (k + e < k1) g[k + e] = go[e];
+    }
+  }
+  return acc;
+}
+
 // Objective at X[which] into G[which]; L.f = f rounded to the bulk type, as
 // a callback returning `real& f` would hand it over.
 // In a gang, member j evaluates the terms of its segment (ROSEN: the pairs
@@ -471,10 +560,12 @@ __device__ __forceinline__ void b_objective(const Prob<T>& P, BatchLds& L, int w
   // can go out together
   switch (P.obj.kind) {
     case OBJK_GRID_QUAD:
-      acc[0] = obj_terms_sum<OBJK_GRID_QUAD>(P.obj, P.n, x, g, k0, k1);
+      acc[0] = (k0 & 3) == 0 ? obj_grid_sum4<OBJK_GRID_QUAD, T>(P.obj, P.n, x, g, k0, k1)
+                             : obj_terms_sum<OBJK_GRID_QUAD>(P.obj, P.n, x, g, k0, k1);
       break;
     case OBJK_GRID_COS:
-      acc[0] = obj_terms_sum<OBJK_GRID_COS>(P.obj, P.n, x, g, k0, k1);
+      acc[0] = (k0 & 3) == 0 ? obj_grid_sum4<OBJK_GRID_COS, T>(P.obj, P.n, x, g, k0, k1)
+                             : obj_terms_sum<OBJK_GRID_COS>(P.obj, P.n, x, g, k0, k1);
       break;
     case OBJK_DSSCFG:
       acc[0] = obj_terms_sum<OBJK_DSSCFG>(P.obj, P.n, x, g, k0, k1);

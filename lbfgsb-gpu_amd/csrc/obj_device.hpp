@@ -29,6 +29,20 @@ struct ObjOut {
   T g0, g1;
 };
 
+// Column and row of grid point k (row-major, nx per row): a 32-bit division
+// where the grid allows one (the 64-bit one is a long instruction sequence).
+__device__ __forceinline__ void grid_pos(long long k, long long n, long long nx, long long& ix,
+                                         long long& iy) {
+  if (n <= 0x7fffffffLL) {
+    const unsigned q = (unsigned)k / (unsigned)nx;
+    iy = q;
+    ix = k - (long long)q * nx;
+  } else {
+    iy = k / nx;
+    ix = k - iy * nx;
+  }
+}
+
 // Term k of an objective of kind KIND: its share of f, with its gradient
 // entries in out (not stored, so that a caller can evaluate several terms
 // before the first store).
@@ -39,7 +53,8 @@ __device__ inline double obj_eval_kind(const ObjDesc& o, long long n, const T* _
     const T* b = (const T*)o.b;
     const double h2 = o.h * o.h;
     const long long nx = o.nx, ny = o.ny;
-    const long long ix = k % nx, iy = k / nx;
+    long long ix, iy;
+    grid_pos(k, n, nx, ix, iy);
     const double v = (double)x[k];
     const double vl = ix > 0 ? (double)x[k - 1] : 0.0;
     const double vr = ix < nx - 1 ? (double)x[k + 1] : 0.0;
@@ -65,7 +80,8 @@ __device__ inline double obj_eval_kind(const ObjDesc& o, long long n, const T* _
     // reference example scatters (its own, then from the bottom, left,
     // right and top neighbours' triangles), in that order.
     const long long nx = o.nx, ny = o.ny;
-    const long long ix = k % nx, iy = k / nx;
+    long long ix, iy;
+    grid_pos(k, n, nx, ix, iy);
     const double lam = o.h;
     const double hx = 1.0 / (double)(nx + 1), hy = 1.0 / (double)(ny + 1);
     const double area = 0.5 * hx * hy;
