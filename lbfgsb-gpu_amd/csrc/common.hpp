@@ -106,6 +106,30 @@ __device__ inline int wave_or_int(int v) {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, kWave);
   return v;
 }
+// One DPP move of a double (CTRL a DPP control, ROWS the rows of 16 lanes
+// that take part): a lane whose source lane lies outside its row, or whose
+// row is not in ROWS, receives 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// Inclusive prefix sum of v over the 64 lanes of a wave, by DPP moves (VALU
+// only, no LDS crossbar): within each row of 16 lanes by shifts of 1, 2, 4
+// and 8 (row_shr), then lane 15's running sum into the next row for rows 1
+// and 3 (row_bcast:15), then lane 31's into rows 2 and 3 (row_bcast:31).
+// Every lane of the wave takes part.
+__device__ __forceinline__ double wave_scan_incl(double v) {
+  v += dpp_f64<0x111, 0xf>(v);
+  v += dpp_f64<0x112, 0xf>(v);
+  v += dpp_f64<0x114, 0xf>(v);
+  v += dpp_f64<0x118, 0xf>(v);
+  v += dpp_f64<0x142, 0xa>(v);
+  v += dpp_f64<0x143, 0xc>(v);
+  return v;
+}
 // (value, index) minimum with ties to the smallest index.
 __device__ inline void wave_argmin(double& v, long long& idx) {
 #pragma unroll

@@ -516,12 +516,7 @@ k_gcp_tile_scan(const DevScalars* ds, long long cap, const double* __restrict__ 
   double run = 0.0;
   for (long long t = t0; t < t1; t++) run += row[t];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double incl = run;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double up = __shfl_up(incl, o, kWave);
-    if (lane >= o) incl += up;
-  }
+  const double incl = wave_scan_incl(run);
   if (lane == 63) swave[w] = incl;
   __syncthreads();
   double off = ds->gcp_carry[q];
@@ -567,23 +562,25 @@ k_gcp_tile_eval(const Cand* __restrict__ cand, Pair<T> Gv, const T* __restrict__
 #pragma unroll
   for (int q = 0; q < kNV; q++) {
     double incl = v[q];
-    if (nv_used(q, col)) {
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const double up = __shfl_up(incl, o, kWave);
-        if (lane >= o) incl += up;
-      }
-    }
+    if (nv_used(q, col)) incl = wave_scan_incl(incl);
     ex[q] = incl - v[q];
     if (lane == 63) sh[wid * kNV + q] = incl;
   }
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < kNV; q++) {
-    double off = tpre[(size_t)q * tstride + blockIdx.x];
-    for (int w = 0; w < wid; w++) off += sh[w * kNV + q];
-    ex[q] += off;
+  // each component's wave offsets (the tile's prefix plus the totals of the
+  // waves before, in wave order) in place of the wave totals
+  if (threadIdx.x < kNV) {
+    const int q = threadIdx.x;
+    double run = tpre[(size_t)q * tstride + blockIdx.x];
+    for (int w = 0; w < kBlock / kWave; w++) {
+      const double t = sh[w * kNV + q];
+      sh[w * kNV + q] = run;
+      run += t;
+    }
   }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kNV; q++) ex[q] += sh[wid * kNV + q];
   if (pos < K) {
     const long long k = ds->gcp_kpassed + pos;
     if (k > 0) {

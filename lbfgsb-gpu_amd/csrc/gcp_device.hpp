@@ -418,23 +418,28 @@ __device__ __forceinline__ void block_scan_range(const DevScalars* ds, RoundLds&
 #pragma unroll
     for (int q = 0; q < kNV; q++) {
       double incl = v[q];
-      if (nv_used(q, col)) {
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const double up = __shfl_up(incl, o, kWave);
-          if (lane >= o) incl += up;
-        }
-      }
+      if (nv_used(q, col)) incl = wave_scan_incl(incl);
       ex[q] = incl - v[q];
       if (lane == 63) L.swave[wid * kNV + q] = incl;
     }
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kNV; q++) {
-      double off = L.scarry[q];
-      for (int w = 0; w < wid; w++) off += L.swave[w * kNV + q];
-      ex[q] += off;
+    // each component's wave offsets (the carried state plus the totals of the
+    // waves before, added in wave order) in place of the wave totals, by one
+    // thread per component, and the chunk's total into the carry: the other
+    // threads then read their offsets independently
+    if (threadIdx.x < kNV) {
+      const int q = threadIdx.x;
+      double run = L.scarry[q];
+      for (int w = 0; w < nw; w++) {
+        const double t = L.swave[w * kNV + q];
+        L.swave[w * kNV + q] = run;
+        run += t;
+      }
+      L.scarry[q] = run;
     }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kNV; q++) ex[q] += L.swave[wid * kNV + q];
     bool stop = false;
     if (test && pos < c_hi && kpassed + pos > 0) {
       const double tp = pos > 0 ? cands[pos - 1].t : t_prev0;
@@ -450,13 +455,6 @@ __device__ __forceinline__ void block_scan_range(const DevScalars* ds, RoundLds&
         for (int q = 0; q < kNV; q++) L.sstop_state[q] = ex[q];
       break;
     }
-    // carry the chunk total
-    if (threadIdx.x < kNV) {
-      double t = L.scarry[threadIdx.x];
-      for (int w = 0; w < nw; w++) t += L.swave[w * kNV + threadIdx.x];
-      L.scarry[threadIdx.x] = t;
-    }
-    __syncthreads();
   }
   __syncthreads();
 }
