@@ -84,15 +84,16 @@ struct Prob {
 __device__ __forceinline__ int nwaves() { return blockDim.x >> 6; }
 
 // Sums of the first NR of NV per-thread values over the workgroup into
-// out[0 .. NR).
+// out[0 .. NR): each wave's sum is its DPP prefix sum's last lane (VALU
+// moves, no LDS crossbar), then the waves' sums in wave order.
 template <int NR, int NV>
 __device__ __forceinline__ void bsum_first(const double (&v)[NV], double* sh, double* out) {
   static_assert(NR <= NV, "more sums than values");
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = nwaves();
 #pragma unroll
   for (int q = 0; q < NR; q++) {
-    const double s = wave_sum(v[q]);
-    if (lane == 0) sh[wid * NR + q] = s;
+    const double s = wave_scan_incl(v[q]);
+    if (lane == 63) sh[wid * NR + q] = s;
   }
   __syncthreads();
   for (int q = threadIdx.x; q < NR; q += blockDim.x) {
