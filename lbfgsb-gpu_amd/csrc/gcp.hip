@@ -412,9 +412,12 @@ k_sort_tile(Cand* a, const DevScalars* ds, long long cap, int mid) {
   }
   __syncthreads();
   for (int k = 2; k <= kSortTile; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
+    // pair p of the stage with distance j = 2^lj: (i, i + j), i the p-th
+    // index whose bit lj is clear (shifts: no integer division)
+    for (int lj = __ffs(k) - 2; lj >= 0; lj--) {
+      const int j = 1 << lj;
       for (int p = threadIdx.x; p < kSortTile / 2; p += blockDim.x) {
-        const int i = (p / j) * 2 * j + (p % j), l = i + j;
+        const int i = ((p >> lj) << (lj + 1)) | (p & (j - 1)), l = i + j;
         const bool up = (i & k) == 0;
         if (cand_less(sh[l], sh[i]) == up) {
           Cand tmp = sh[i];

@@ -258,11 +258,14 @@ struct RoundLds {
 };
 
 // Bitonic sort of sc[0 .. P) in LDS, P a power of two.
+// Pair p of the stage with distance j = 2^lj is (i, i + j), i the p-th
+// index whose bit lj is clear (shifts: no integer division).
 __device__ inline void block_bitonic(Cand* sc, int P) {
   for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
+    for (int lj = __ffs(k) - 2; lj >= 0; lj--) {
+      const int j = 1 << lj;
       for (int p = threadIdx.x; p < P / 2; p += blockDim.x) {
-        const int i = (p / j) * 2 * j + (p % j), l = i + j;
+        const int i = ((p >> lj) << (lj + 1)) | (p & (j - 1)), l = i + j;
         const bool up = (i & k) == 0;
         if (cand_less(sc[l], sc[i]) == up) {
           Cand tmp = sc[i];
