@@ -549,8 +549,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-count", type=int, default=4096)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serving-count", type=int, default=600)
-    ap.add_argument("--serving-qps", type=float, default=15.0)
+    ap.add_argument("--serving-count", type=int, default=1200)
+    ap.add_argument("--serving-qps", type=float, default=30.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

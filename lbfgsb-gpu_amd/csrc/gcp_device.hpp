@@ -96,30 +96,52 @@ __device__ inline void stage_gcp_const(const DevScalars* ds, GcpConst& gc) {
 
 // Layout of a state vector: [P_y(8) P_s(8) Q_y(8) Q_s(8) D].
 
+// p'Mp and q'Mp at the state `st` (p = pp - P, q = Q) over the first C
+// positions of each half: the positions from col on hold exact zeros, so
+// leaving them out (C >= col) leaves every sum unchanged; the loops have
+// compile-time indices, so the temporaries stay in registers.
+template <int C>
+__device__ __forceinline__ void stop_quad(const GcpConst& gc, const double (&st)[kNV], double& pmp,
+                                          double& qmp) {
+  double p[2 * C], mp[2 * C];
+#pragma unroll
+  for (int a = 0; a < 2 * C; a++) {
+    const int q = a < C ? a : kMaxM + a - C;
+    p[a] = gc.pp[q] - st[q];
+  }
+#pragma unroll
+  for (int ra = 0; ra < 2 * C; ra++) {
+    const int r = ra < C ? ra : kMaxM + ra - C;
+    double s = 0.0;
+#pragma unroll
+    for (int ka = 0; ka < 2 * C; ka++) {
+      const int k = ka < C ? ka : kMaxM + ka - C;
+      s += gc.Mp[r * 16 + k] * p[ka];
+    }
+    mp[ra] = s;
+  }
+  pmp = 0.0;
+  qmp = 0.0;
+#pragma unroll
+  for (int j = 0; j < C; j++) {
+    pmp += p[j] * mp[j] + p[C + j] * mp[C + j];
+    qmp += st[16 + j] * mp[j] + st[24 + j] * mp[C + j];
+  }
+}
+
 // Stop test at the state `st` (passed breakpoints before t_next).
-// All loops run over the padded 16 positions with compile-time indices (the
-// entries beyond col are exact zeros, which leaves every sum unchanged), so
-// the temporaries stay in registers.
 __device__ inline bool stop_test(const DevScalars* ds, const GcpConst& gc,
                                  const double (&st)[kNV], double t_prev, double t_next,
                                  bool floor_on, double (&out)[3]) {
   const double theta = ds->theta;
-  double p[16], mp[16];
-#pragma unroll
-  for (int q = 0; q < 16; q++) p[q] = gc.pp[q] - st[q];
-#pragma unroll
-  for (int r = 0; r < 16; r++) {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) s += gc.Mp[r * 16 + k] * p[k];
-    mp[r] = s;
-  }
-  double pmp = 0.0, qmp = 0.0;
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    pmp += p[j] * mp[j] + p[8 + j] * mp[8 + j];
-    qmp += st[16 + j] * mp[j] + st[24 + j] * mp[8 + j];
-  }
+  const int col = ds->col;
+  double pmp, qmp;
+  if (col <= 2)
+    stop_quad<2>(gc, st, pmp, qmp);
+  else if (col <= 4)
+    stop_quad<4>(gc, st, pmp, qmp);
+  else
+    stop_quad<kMaxM>(gc, st, pmp, qmp);
   const double D = ds->gcp_D0 - st[32];
   const double f2u = theta * D - pmp;
   const double f1 = -D + t_prev * f2u - qmp;
