@@ -84,8 +84,9 @@ __device__ __host__ inline uint8_t iw_make(int iw, bool freelast, bool freeprev,
 }
 
 // ---------------------------------------------------------------------------
-// Wave / block reductions.  All cross-lane traffic goes through
-// __shfl_xor (ds_swizzle / dpp on gfx950); 64 lanes.
+// Wave / block reductions over 64 lanes.  __shfl_xor compiles to
+// ds_bpermute (the LDS crossbar); the prefix sums below use DPP moves instead
+// (VALU only), where a reduction runs once per candidate or pass step.
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
