@@ -52,6 +52,7 @@ struct BatchLds {
   double rv[4];              // [sbgnrm, bkmin, amin, smin] of the last passes
   long long ri[2];           // [ibkmin, ibd]
   int gflag;                 // gang error flag of the last barrier
+  unsigned near_n;           // pass A's count of listed breakpoints
   BucketLds bk;              // bucket counts of a large breakpoint round
   union {
     RoundLds round;
@@ -82,6 +83,14 @@ struct Prob {
 };
 
 __device__ __forceinline__ int nwaves() { return blockDim.x >> 6; }
+
+// Entries of a problem's breakpoint list (pass A, one workgroup per
+// problem): the list lives in the problem's second candidate buffer.
+constexpr long long kListCap = 8192;
+template <typename T>
+__device__ __forceinline__ unsigned list_cap(const Prob<T>& P) {
+  return (unsigned)(P.n < kListCap ? P.n : kListCap);
+}
 
 // Sums of the first NR of NV per-thread values over the workgroup into
 // out[0 .. NR): each wave's sum is its DPP prefix sum's last lane (VALU
@@ -607,7 +616,18 @@ __device__ __forceinline__ void b_pass_a(const Prob<T>& P, BatchLds& L, Gang<GO>
   if (L.s.halt) return;
   using Op = PassAOp<T, FUSED>;
   Op op;
-  op.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.W, P.d);
+  // one workgroup per problem: the breakpoints below the predicted bound go
+  // to the problem's list in cb (the class is then not stored: the search
+  // and the Cauchy-point pass reclassify from the stored one); a gang
+  // stores the class and selects from the variables
+  Cand* lst = nullptr;
+  const unsigned cap = list_cap(P);
+  if constexpr (!GO) {
+    if (L.s.gcp_near_tau > 0.0) lst = P.cb;
+    if (threadIdx.x == 0) L.near_n = 0u;
+    __syncthreads();
+  }
+  op.init(&L.s, P.X, P.G, P.l, P.u, P.iwhere, P.W, P.d, lst, &L.near_n, cap);
   for (long long i = P.lo + threadIdx.x; i < P.hi; i += blockDim.x) {
     typename Op::In e;
     op.load(e, i);
@@ -627,6 +647,7 @@ __device__ __forceinline__ void b_pass_a(const Prob<T>& P, BatchLds& L, Gang<GO>
     L.rv[1] = bk;
     L.ri[0] = ibk;
     L.red[PA_GD] = 0.0;
+    L.s.gcp_list_n = lst ? (long long)L.near_n : -1;
   }
   __syncthreads();
   gang_combine(G, L, PA_GD, 1, 3, 0, 0, &L.gflag);
@@ -736,6 +757,68 @@ __device__ __forceinline__ void b_gd(const Prob<T>& P, BatchLds& L, Gang<GO>& G)
   gang_combine(G, L, 1, 0, 0, 0, 0, &L.gflag);
 }
 
+// The round's candidates from pass A's list (cb, one workgroup per problem)
+// when its t-range lies below the bound the list was made under and the
+// list did not overflow: the entries in [tau_lo, tau_hi) into P.ca, as the
+// selection from the variables would give them (the same t, in another
+// order: the round sorts them).  Otherwise (false) the bound adapts as in
+// the plain form (x4 for a range beyond it, /4 after an overflow) and the
+// caller selects from the variables.
+template <typename T>
+__device__ LBFGSB_COLD bool b_select_listed(const Prob<T>& P, BatchLds& L) {
+  DevScalars* s = &L.s;
+  const long long ln = s->gcp_list_n;
+  const double ntau = s->gcp_near_tau;
+  const bool covers = ln >= 0 && ntau > 0.0 && s->gcp_tau_hi <= ntau && ln <= (long long)list_cap(P);
+  __syncthreads();
+  if (!covers) {
+    if (threadIdx.x == 0 && ln >= 0 && ntau > 0.0) {
+      s->gcp_near_misses++;
+      const double gr = s->gcp_near_grow > 0.0 ? s->gcp_near_grow : kNearGrow0;
+      s->gcp_near_grow = ln > (long long)list_cap(P) ? fmax(gr * 0.25, 1.0) : fmin(gr * 4.0, 1024.0);
+    }
+    if (threadIdx.x == 0) s->gcp_sel_near = 0;
+    __syncthreads();
+    return false;
+  }
+  const double lo = s->gcp_tau_lo, hi = s->gcp_tau_hi;
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) {
+    s->gcp_need_select = 0;
+    s->gcp_sel_near = 1;
+    s->gcp_near_listed = ln;
+  }
+  for (long long j0 = 0; j0 < ln; j0 += blockDim.x) {
+    const long long j = j0 + threadIdx.x;
+    Cand c;
+    c.t = -1.0;
+    if (j < ln) c = P.cb[j];
+    const bool take = j < ln && c.t >= lo && c.t < hi;
+    const unsigned long long mask = __ballot(take);
+    if (mask) {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(&s->cand_count, (unsigned long long)__popcll(mask));
+      base = __shfl(base, 0, kWave);
+      const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+      if (take && slot < (unsigned long long)P.n) P.ca[slot] = c;
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
+template <typename T, bool GO>
+__device__ __forceinline__ void b_select(const Prob<T>& P, BatchLds& L, Gang<GO>& G);
+
+// b_select out of line: the fp32 one-workgroup kernel's register allocation
+// came out better with it (4096-QP workload 31.3k -> 33.0k solves/s, the
+// 512-problem grid workload 564 -> 611), the fp64 one's worse (24.3k ->
+// 23.7k), so fp64 keeps it inline.
+template <typename T, bool GO>
+__device__ LBFGSB_COLD void b_select_cold(const Prob<T>& P, BatchLds& L, Gang<GO>& G) {
+  b_select(P, L, G);
+}
+
 // Breakpoint candidates with t in [tau_lo, tau_hi) of this workgroup's
 // segment into P.ca (slots from the LDS count, or in a gang from the gang's
 // HBM count).
@@ -770,9 +853,15 @@ __device__ __forceinline__ void b_select(const Prob<T>& P, BatchLds& L, Gang<GO>
       bool take = false;
       T tb = 0;
       const uint8_t iwb = bv[q];
-      if (i < ph && breakpoint_of(xv[q], gv[q], lv[q], uv[q], iw_nbd(iwb), iw_get(iwb), tb)) {
+      if (i < ph) {
+        // the stored class is pass A's or the one before it (classify is
+        // idempotent): the t pass A forms
+        int iw;
+        T neggi;
+        bool isfree, isbrk;
+        classify(xv[q], gv[q], lv[q], uv[q], iw_nbd(iwb), iw_get(iwb), iw, neggi, isfree, isbrk, tb);
         const double t = (double)tb;
-        take = t >= lo && t < hi;
+        take = isbrk && t >= lo && t < hi;
       }
       const unsigned long long mask = __ballot(take);
       if (mask) {
@@ -830,6 +919,8 @@ __device__ __forceinline__ void b_round(RoundArgs<T> P, BatchLds& L) {
     const Cand* sorted = block_sort_large(P.ca, P.cb, K, L.u.round.sc);
     block_round_scan(s, L.u.round, sorted, K, g, P.W);
   }
+  // the larger rounds work through cb, which held pass A's list
+  if (K > kSmallRound && threadIdx.x == 0) s->gcp_list_n = -1;
   if (s->gcp_pending) {
     // every candidate of the t-range passed: the next range
     if (threadIdx.x == 0) logic_resume(s, 0);
@@ -1023,10 +1114,16 @@ __device__ __forceinline__ void b_gcp(const Prob<T>& P, BatchLds& L, Gang<GO>& G
   DevScalars* s = &L.s;
   if (!s->gcp_pending || s->halt) return;
   for (int guard = 0; guard < 64 && s->gcp_pending; guard++) {
-    b_select(P, L, G);
     if constexpr (!GO) {
+      if (!b_select_listed(P, L)) {
+        if constexpr (sizeof(T) == 4)
+          b_select_cold(P, L, G);
+        else
+          b_select(P, L, G);
+      }
       b_round(round_args(P, s), L);
     } else {
+      b_select(P, L, G);
       if (!gang_sync(G, &L.gflag)) {
         if (threadIdx.x == 0) set_done(s, -2);
         __syncthreads();

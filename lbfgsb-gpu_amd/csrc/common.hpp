@@ -61,7 +61,10 @@ struct Pair {
 };
 
 // Breakpoint candidate of the Cauchy search: sorted by (t, idx).
-struct Cand {
+// 16-byte aligned wherever it is placed (LDS arrays of it included), so that
+// an entry moves with one 128-bit LDS access: the arrays' offsets inside the
+// workgroup's LDS structures do not depend on the sizes of what precedes them.
+struct alignas(16) Cand {
   double t;
   int idx;
   int pad;

@@ -212,9 +212,17 @@ struct PassAOp {
       // Cauchy-point pass reclassifies from the stored one (classification
       // is idempotent), so this pass writes no per-variable stream for the
       // search
-      if (isbrk && (double)tbrk < ntau) {
-        const unsigned k = atomicAdd(ncnt, 1u);
-        if (k < ncap) {
+      // (one LDS atomic per wave: the wave's listed lanes take consecutive
+      // slots; lanes of the wave that left the loop take no part)
+      const bool lst = isbrk && (double)tbrk < ntau;
+      const unsigned long long lm = __ballot(lst);
+      if (lm) {
+        const int lane = threadIdx.x & 63, lead = __ffsll((long long)lm) - 1;
+        unsigned base = 0u;
+        if (lane == lead) base = atomicAdd(ncnt, (unsigned)__popcll(lm));
+        base = __shfl(base, lead, kWave);
+        const unsigned k = base + (unsigned)__popcll(lm & ((1ull << lane) - 1ull));
+        if (lst && k < ncap) {
           Cand c;
           c.t = (double)tbrk;
           c.idx = (int)i;

@@ -138,14 +138,16 @@ struct DevScalars {
   int gcp_cand_in_b;       // sorted candidates ended in the second buffer
   double gcp_t_prev;       // t of the last passed breakpoint (0 before any)
   long long gcp_idx_prev;
-  // Breakpoints pass A lists (plain form): every variable with t below
-  // gcp_near_tau (0: none listed), per block of pass A; a round whose
-  // t-range lies below it takes its candidates from that list instead of
-  // classifying every variable again (gcp_sel_near: the current round did)
+  // Breakpoints pass A lists: every variable with t below gcp_near_tau (0:
+  // none listed), per block of pass A (plain form) or per problem (batched
+  // form); a round whose t-range lies below it takes its candidates from
+  // that list instead of classifying every variable again (gcp_sel_near:
+  // the current round did)
   double gcp_near_tau;
   double gcp_near_grow;    // near_tau over the t-range the last search ended in
   int gcp_sel_near;
   int gcp_near_misses;     // statistics: rounds the list did not cover
+  long long gcp_list_n;    // batched: entries pass A counted into the problem's list (-1: none)
 
   // ---- small dense algebra (row-major, leading dimension 8 or 16)
   double sy[64], ss[64], wt[64];
