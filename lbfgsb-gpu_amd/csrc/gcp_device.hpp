@@ -21,22 +21,6 @@ __device__ inline bool cand_less(const Cand& a, const Cand& b) {
   return a.t < b.t || (a.t == b.t && a.idx < b.idx);
 }
 
-// Same classification as the bulk passes (lbfgsbcpu.cpp:634-700).
-template <typename T>
-__device__ inline bool breakpoint_of(T x, T g, T l, T u, int nb, int iw, T& t) {
-  const T neggi = -g;
-  if (iw != 0 && iw != -1) return false;
-  if (nb <= 2 && nb != 0 && neggi < 0) {
-    t = (x - l) / (-neggi);
-    return true;
-  }
-  if (nb >= 2 && neggi > 0) {
-    t = (u - x) / neggi;
-    return true;
-  }
-  return false;
-}
-
 // Whether component q of a state vector [P_y P_s Q_y Q_s D] can be nonzero
 // with col pairs in the memory: the positions j < col of each block, and D.
 // The others are exact zeros, so their sums and scans are skipped.
