@@ -263,13 +263,22 @@ struct RoundLds {
   GcpConst gc;
 };
 
-// Bitonic sort of sc[0 .. P) in LDS, P a power of two.
-// Pair p of the stage with distance j = 2^lj is (i, i + j), i the p-th
-// index whose bit lj is clear (shifts: no integer division).
+// Bitonic sort of sc[0 .. P) in LDS, P a power of two.  Pair p of the stage
+// with distance j = 2^lj is (i, i + j), i the p-th index whose bit lj is
+// clear (shifts: no integer division).  The pairs a wave takes (p in
+// 64-aligned runs) lie in 128-element blocks of its own for every stage with
+// j <= 32, so such a stage waits for the workgroup only when the stage
+// before it had j >= 64; between two of them the wave's own LDS operations,
+// which complete in order, suffice.
 __device__ inline void block_bitonic(Cand* sc, int P) {
   for (int k = 2; k <= P; k <<= 1) {
-    for (int lj = __ffs(k) - 2; lj >= 0; lj--) {
+    const int lj0 = __ffs(k) - 2;
+    for (int lj = lj0; lj >= 0; lj--) {
       const int j = 1 << lj;
+      if (j >= 64 || (j == 32 && lj != lj0))
+        __syncthreads();
+      else
+        __builtin_amdgcn_wave_barrier();
       for (int p = threadIdx.x; p < P / 2; p += blockDim.x) {
         const int i = ((p >> lj) << (lj + 1)) | (p & (j - 1)), l = i + j;
         const bool up = (i & k) == 0;
@@ -279,9 +288,9 @@ __device__ inline void block_bitonic(Cand* sc, int P) {
           sc[l] = tmp;
         }
       }
-      __syncthreads();
     }
   }
+  __syncthreads();
 }
 
 // Sort K <= 64 candidates with one wave: lane-to-lane bitonic network in
