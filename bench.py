@@ -452,7 +452,9 @@ def run_serving(count: int, qps: float, check: int = 48, polish: bool = True):
     from lbfgsb_amd.serve import latency_summary, serve
 
     reqs = pb.serving_stream(20000, qps=qps)[:count]
-    cfg = dict(budget_bytes=200e9, fp32_share=0.7, max_slots=4096, threads=256, segment=65536,
+    # the shape: the best of a sweep over (threads, segment, max_gang) on this
+    # stream at 30/s (512 threads, 32,768-element segments, gangs of up to 64)
+    cfg = dict(budget_bytes=200e9, fp32_share=0.7, max_slots=4096, threads=512, segment=32768,
                max_gang=64, quantum=20, use_graph=True, polish=polish, polish_eps_scale=1e-2)
     print(f"serving: {count} requests at {qps} per second", file=sys.stderr, flush=True)
     res, st = serve(reqs, **cfg)
