@@ -130,24 +130,6 @@ __device__ inline double obj_eval_kind(const ObjDesc& o, long long n, const T* _
   }
 }
 
-// Term k of the objective o (any kind).
-template <typename T>
-__device__ inline double obj_eval(const ObjDesc& o, long long n, const T* __restrict__ x,
-                                  long long k, ObjOut<T>& out) {
-  switch (o.kind) {
-    case OBJK_GRID_QUAD:
-      return obj_eval_kind<OBJK_GRID_QUAD>(o, n, x, k, out);
-    case OBJK_GRID_COS:
-      return obj_eval_kind<OBJK_GRID_COS>(o, n, x, k, out);
-    case OBJK_DSSCFG:
-      return obj_eval_kind<OBJK_DSSCFG>(o, n, x, k, out);
-    case OBJK_ROSEN:
-      return obj_eval_kind<OBJK_ROSEN>(o, n, x, k, out);
-    default:
-      return obj_eval_kind<OBJK_PENTA>(o, n, x, k, out);
-  }
-}
-
 // Stores the gradient entries of term k (kind KIND).
 template <int KIND, typename T>
 __device__ inline void obj_store_kind(T* __restrict__ g, long long k, const ObjOut<T>& out) {
@@ -160,32 +142,6 @@ __device__ inline void obj_store_kind(T* __restrict__ g, long long k, const ObjO
   } else {
     g[k] = out.g0;
   }
-}
-
-// Stores the gradient entries of term k.
-template <typename T>
-__device__ inline void obj_store(const ObjDesc& o, T* __restrict__ g, long long k,
-                                 const ObjOut<T>& out) {
-  if (o.kind == OBJK_ROSEN) {
-    // the pair as one 2-vector store: a wave writes one contiguous run
-    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
-    V2 gv;
-    gv.x = out.g0;
-    gv.y = out.g1;
-    reinterpret_cast<V2*>(g)[k] = gv;
-  } else {
-    g[k] = out.g0;
-  }
-}
-
-// Term k: writes the gradient entries it owns and returns its share of f.
-template <typename T>
-__device__ inline double obj_term(const ObjDesc& o, long long n, const T* __restrict__ x,
-                                  T* __restrict__ g, long long k) {
-  ObjOut<T> r;
-  const double f = obj_eval(o, n, x, k, r);
-  obj_store(o, g, k, r);
-  return f;
 }
 
 }  // namespace lbfgsb
