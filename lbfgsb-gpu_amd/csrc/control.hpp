@@ -584,7 +584,7 @@ __device__ inline void post_update_block(DevScalars* s) {
 // read sy and wt; the controller changes col only when it discards the
 // memory, and then the columns are formed again for the new col).  The same
 // results as the two calls in sequence; needs at least 64 + 16 threads.
-__device__ inline void post_update_middle(DevScalars* s) {
+__device__ __forceinline__ void post_update_middle(DevScalars* s) {
   const int mode = s->post_update;
   if (!mode) {
     build_middle(s);
