@@ -505,13 +505,19 @@ __device__ inline void block_round_scan(DevScalars* ds, RoundLds& L, const Cand*
 // breakpoint advance as the scan's positions do.  Returns false, having
 // changed nothing, when one bucket holds more than kSmallRound candidates
 // (the caller sorts the round whole).
-static constexpr int kBuckets = 2048;
+static constexpr int kBuckets = 8192;
 
 struct BucketLds {
   unsigned cnt[kBuckets];  // counts, then (scanned) each bucket's end in b
   unsigned long long kmin, kmax;
   unsigned big;
+  unsigned gmax;  // the current group's largest bucket
 };
+
+// A group whose buckets all hold at most this many candidates is sorted one
+// bucket per thread (insertion sort in LDS) instead of by the bitonic
+// network over the whole group.
+static constexpr unsigned kBucketInsert = 32;
 
 __device__ __forceinline__ unsigned long long cand_key(double t) {
   return (unsigned long long)__double_as_longlong(t);  // t >= 0: ordered as t
@@ -630,9 +636,37 @@ __device__ LBFGSB_COLD bool block_round_bucketed(DevScalars* ds, RoundLds& L, Bu
     const long long len = (long long)(end - start);
     const bool last = lo == kBuckets - 1;
     if (len > 0) {
+      if (threadIdx.x == 0) B.gmax = 0u;
       for (long long j = threadIdx.x; j < len; j += nt) L.sc[j] = b[start + j];
       __syncthreads();
-      block_sort_small(L.sc, len, L.sc);
+      // bucket q of the group spans [end of bucket q - 1, end of q) of b
+      unsigned mx = 0u;
+      for (int q = b0 + (int)threadIdx.x; q <= lo; q += (int)nt) {
+        const unsigned sz = B.cnt[q] - (q > 0 ? B.cnt[q - 1] : 0u);
+        mx = sz > mx ? sz : mx;
+      }
+      if (mx) atomicMax(&B.gmax, mx);
+      __syncthreads();
+      if (B.gmax <= kBucketInsert) {
+        // the buckets' t-ranges are disjoint and ascending, so sorting each
+        // bucket in place sorts the group
+        for (int q = b0 + (int)threadIdx.x; q <= lo; q += (int)nt) {
+          const int a = (int)((q > 0 ? B.cnt[q - 1] : 0u) - start);
+          const int e = (int)(B.cnt[q] - start);
+          for (int x = a + 1; x < e; x++) {
+            const Cand key = L.sc[x];
+            int y = x - 1;
+            while (y >= a && cand_less(key, L.sc[y])) {
+              L.sc[y + 1] = L.sc[y];
+              y--;
+            }
+            L.sc[y + 1] = key;
+          }
+        }
+        __syncthreads();
+      } else {
+        block_sort_small(L.sc, len, L.sc);
+      }
       if (threadIdx.x < kNV) L.scarry[threadIdx.x] = ds->gcp_carry[threadIdx.x];
       block_scan_range(ds, L, L.sc, 0, len, len, true, g, W);
       const bool stopped = L.sstop != 0x7fffffffffffffffLL;
