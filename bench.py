@@ -73,7 +73,7 @@ STEADY_STATE = ("pass_a_spec", "p4", "p5a", "p5b", "trial")
 # fp32), from two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE, KiB per
 # dispatch) scaled by what the same counters read for a 64 MiB copy kernel
 # (fetch x1.999, write x1.000); see profiles/plain_rosenbrock_16M_fp32_hbm_traffic.md.
-MEASURED_TRAFFIC_16M_FP32 = {"pass_a_spec": 1496496149, "p4": 1389922639,
+MEASURED_TRAFFIC_16M_FP32 = {"pass_a_spec": 1496419441, "p4": 1389922639,
                              "p5a": 1426190010, "p5b": 419491166}
 
 
