@@ -27,7 +27,10 @@ def _mix(seed=7):
             # Rosenbrock runs to its minimiser (f* = 0): stopped early by a
             # loose eps_f, two rounding-different paths stop at different f
             ("rosen_3000", pb.rosenbrock(3000, seed=seed + 4), 1e-13, 3000),
-            ("penta_777", pb.penta_qp(777, seed=seed + 5), 1e-6, 1000)]
+            ("penta_777", pb.penta_qp(777, seed=seed + 5), 1e-6, 1000),
+            # first Cauchy rounds of ~10^4 candidates: the bucketed rounds,
+            # several groups, sorted bucket by bucket or by the network
+            ("penta_24000", pb.penta_qp(24000, seed=seed + 6), 1e-6, 1000)]
 
 
 def _solve(cases, dtype, threads=256, budget=0, segment=0, max_gang=64, gangs=None):
