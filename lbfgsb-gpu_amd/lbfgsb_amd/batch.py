@@ -171,7 +171,7 @@ class Batch:
 # (threads per workgroup, resident workgroups as a multiple of the CU count,
 # segment length of the gangs (0 = one workgroup per problem))
 TUNE_CANDIDATES = ((256, 2, 0), (512, 1, 0), (512, 1, 131072), (512, 1, 65536),
-                   (512, 1, 16384))
+                   (512, 1, 32768), (512, 1, 16384), (256, 2, 65536), (256, 2, 32768))
 MAX_GANG = 64
 
 
