@@ -82,6 +82,26 @@ double now_seconds() {
       .count();
 }
 
+// Launches the library objective at x with no wait: the gradient into g,
+// the value into p->fdev (and published to p->fhost under a new sequence
+// number).
+template <typename R>
+int problem_launch(lbfgsb_problem* p, R* x, R* g, hipStream_t st) {
+  launch_objective<R>(st, p->desc, p->n, x, g, p->part, p->G, p->fdev, p->fhost, ++p->fseq);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// The problem when funcgrad is the library's own objective of this
+// precision (lbfgsb_problem_funcgrad_*), else null.
+template <typename R, typename Cb>
+lbfgsb_problem* own_objective(const Cb* cb) {
+  const void* own = sizeof(R) == 8 ? reinterpret_cast<const void*>(&lbfgsb_problem_funcgrad_f64)
+                                   : reinterpret_cast<const void*>(&lbfgsb_problem_funcgrad_f32);
+  lbfgsb_problem* p = (lbfgsb_problem*)cb->funcgrad_ctx;
+  if (reinterpret_cast<const void*>(cb->funcgrad) != own || !p) return nullptr;
+  return p->dtype == (sizeof(R) == 8 ? 1 : 0) ? p : nullptr;
+}
+
 // Host loop of a plain solve on an existing engine (lbfgsbcpu.cpp:117-465
 // control flow; the per-iteration decisions run on the device, see
 // small.hip).  iter_clock[k], if given, receives the host time at which the
@@ -121,6 +141,13 @@ int drive(EngineBase* e, const Cb* cb, const Opt* opt, Summary* summary, void* s
   }
   int prog = PROG_ITER_START;
   double fval = 0.0;
+  // With the library's own objective the line search reads each trial's
+  // value from device memory, so the chain after an evaluation is launched
+  // right behind it, with no host wait for f in between (the same values:
+  // the callback returns that double rounded to R).  The first evaluation
+  // above goes through the callback.
+  lbfgsb_problem* own = own_objective<R>(cb);
+  e->set_value_source(own ? own->fdev : nullptr);
   // every host round trip either evaluates f (at most 20 per iteration) or
   // restarts; this bound is never reached by a correct chain
   long long budget = 64LL * ((long long)opt->max_iteration + 2) + 4096;
@@ -135,9 +162,16 @@ int drive(EngineBase* e, const Cb* cb, const Opt* opt, Summary* summary, void* s
     clock_iter();
     fill_summary(h, summary);
     if (status == ST_NEED_EVAL) {
-      f = 0;
-      cb->funcgrad(cb->funcgrad_ctx, (R*)e->x_ptr(1), &f, (R*)e->g_ptr(1), stream, summary);
-      fval = (double)f;
+      if (own) {
+        if (problem_launch<R>(own, (R*)e->x_ptr(1), (R*)e->g_ptr(1), e->stream())) {
+          status = -1;
+          break;
+        }
+      } else {
+        f = 0;
+        cb->funcgrad(cb->funcgrad_ctx, (R*)e->x_ptr(1), &f, (R*)e->g_ptr(1), stream, summary);
+        fval = (double)f;
+      }
       prog = PROG_AFTER_EVAL;
     } else if (status == ST_ACCEPTED) {
       R fc = (R)h.f;
@@ -163,6 +197,7 @@ int drive(EngineBase* e, const Cb* cb, const Opt* opt, Summary* summary, void* s
       status = -1;
     }
   }
+  e->set_value_source(nullptr);
   int info;
   if (status < 0) {
     std::fprintf(stderr, "lbfgsb_amd: device error %s\n", hipGetErrorString(e->last_error));
@@ -270,8 +305,7 @@ int minimize_noalloc_impl(int dtype, int n, const Cb* cb, const Opt* opt, R* x,
 template <typename R>
 int problem_funcgrad(lbfgsb_problem* p, R* x, R* f, R* g, void* stream) {
   hipStream_t st = (hipStream_t)stream;
-  launch_objective<R>(st, p->desc, p->n, x, g, p->part, p->G, p->fdev, p->fhost, ++p->fseq);
-  if (hipGetLastError() != hipSuccess) return -1;
+  if (problem_launch<R>(p, x, g, st)) return -1;
   // f is final on return, as the reference's callback contract requires
   if (mail_wait(st, &p->fhost->s, p->fseq) != hipSuccess) return -1;
   *f = (R)p->fhost->f;

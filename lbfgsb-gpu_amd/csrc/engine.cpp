@@ -411,7 +411,7 @@ struct Engine final : EngineBase {
         break;
       case K_LSSTOP:
         launch_rowsum(st, part, G, 0, PA_COUNT, ds, 0, 1, pmax, pmin, pmini);
-        launch_ls_stop(st, ds, pmax, pmin, pmini, G, f);
+        launch_ls_stop(st, ds, pmax, pmin, pmini, G, f, fdev_src, sizeof(T) == 4);
         break;
       case K_UPD:
         launch_update_gcp(st, ds, part, pmin, pmini, G, 0);
@@ -442,7 +442,7 @@ struct Engine final : EngineBase {
         launch_gd<T>(st, G, n, dX, dG, d, ds, part, 1);
         break;
       case K_LSSTEP:
-        launch_ls_step(st, ds, part, G, f);
+        launch_ls_step(st, ds, part, G, f, fdev_src, sizeof(T) == 4);
         break;
       default:
         break;

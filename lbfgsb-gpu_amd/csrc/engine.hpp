@@ -22,6 +22,11 @@ struct EngineBase {
   virtual int setup_begin() = 0;  // errclb + active; returns status
   virtual int setup_finish(double f) = 0;
   virtual int run(int program, int start_step, int stop_step, double f) = 0;
+  // While set, the line-search steps of run() read the evaluated value from
+  // this device double (written on the engine's stream by the objective;
+  // rounded to fp32 for an fp32 engine, as a callback would return it)
+  // instead of taking f.
+  void set_value_source(const double* fdev) { fdev_src = fdev; }
   virtual DevScalars& host_scalars() = 0;
   virtual int read_scalars() = 0;
   virtual int write_scalars() = 0;
@@ -36,6 +41,7 @@ struct EngineBase {
   virtual void set_profiling(int from_iter, unsigned step_mask) = 0;  // -1 = off
   virtual void step_time(int step, double* ms, long long* count) = 0;
   hipError_t last_error = hipSuccess;
+  const double* fdev_src = nullptr;
   // statistics of the host-sized breakpoint rounds: the most candidates one
   // round sorted (tiles of kSortTile, then one merge-path pass per doubling)
   long long max_sorted_round = 0;

@@ -90,11 +90,14 @@ void launch_alpha(hipStream_t s, DevScalars* ds, const double* pmin, const long 
                   int G);
 void launch_ls_init(hipStream_t s, DevScalars* ds, const double* part, const double* pmin,
                     int G);
-void launch_ls_step(hipStream_t s, DevScalars* ds, const double* part, int G, double f);
+// The line-search kernels take the evaluated value as f, or when fdev is
+// not null from the FP64 double there (rounded to fp32 when single is set).
+void launch_ls_step(hipStream_t s, DevScalars* ds, const double* part, int G, double f,
+                    const double* fdev, int single);
 // Line-search step on the g'd of a speculative pass A (row PA_GD of the
 // row-summed pass-A partials), then on acceptance the stopping tests.
 void launch_ls_stop(hipStream_t s, DevScalars* ds, const double* pmax, const double* pmin,
-                    const long long* pmini, int G, double f);
+                    const long long* pmini, int G, double f, const double* fdev, int single);
 
 // gcp.hip
 struct GcpScratch {

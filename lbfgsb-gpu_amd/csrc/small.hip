@@ -352,16 +352,24 @@ s_ls_init(DevScalars* ds, const double* part, const double* pmin, int G) {
   ds_store(ds, s);
 }
 
-// Later calls of lnsrlb, after an evaluation with value f
-// (lbfgsbcpu.cpp:1312-1339 and the loop exit at :352).
+// The evaluated value: f, or when fdev is given the objective's FP64 value
+// there, rounded to fp32 for an fp32 solve as a callback returns it.
+__device__ __forceinline__ double eval_value(double f, const double* fdev, int single) {
+  if (!fdev) return f;
+  return single ? (double)(float)*fdev : *fdev;
+}
+
+// Later calls of lnsrlb, after an evaluation (lbfgsbcpu.cpp:1312-1339 and
+// the loop exit at :352).
 __global__ void __launch_bounds__(kBlock)
-s_ls_step(DevScalars* ds, const double* part, int G, double f) {
+s_ls_step(DevScalars* ds, const double* part, int G, double f, const double* fdev,
+          int single) {
   __shared__ DevScalars sds;
   __shared__ double r[1];
   reduce_rows(part, G, 0, 1, r);
   ds_load(&sds, ds);
   DevScalars* s = &sds;
-  if (threadIdx.x == 0) logic_ls_step(s, f, r[0]);
+  if (threadIdx.x == 0) logic_ls_step(s, eval_value(f, fdev, single), r[0]);
   ds_store(ds, s);
 }
 
@@ -374,7 +382,7 @@ s_ls_step(DevScalars* ds, const double* part, int G, double f) {
 // and the host runs the regular tail pass.
 __global__ void __launch_bounds__(kBlock)
 s_ls_stop(DevScalars* ds, const double* pmax, const double* pmin, const long long* pmini,
-          int G, double f) {
+          int G, double f, const double* fdev, int single) {
   __shared__ DevScalars sds;
   (void)pmax;
   (void)pmin;
@@ -385,7 +393,7 @@ s_ls_stop(DevScalars* ds, const double* pmax, const double* pmin, const long lon
   if (threadIdx.x == 0) {
     const int iter0 = s->iter;
     const double stp0 = s->stp;
-    logic_ls_step(s, f, s->red[PA_GD]);
+    logic_ls_step(s, eval_value(f, fdev, single), s->red[PA_GD]);
     if (s->iter != iter0 && s->status == ST_RUNNING) {
       if (s->updatd && !s->write_trial && s->stp == stp0) {
         ring_take_spare(s, s->itail);
@@ -454,12 +462,14 @@ void launch_ls_init(hipStream_t s, DevScalars* ds, const double* part, const dou
                     int G) {
   hipLaunchKernelGGL(s_ls_init, dim3(1), dim3(kBlock), 0, s, ds, part, pmin, G);
 }
-void launch_ls_step(hipStream_t s, DevScalars* ds, const double* part, int G, double f) {
-  hipLaunchKernelGGL(s_ls_step, dim3(1), dim3(kBlock), 0, s, ds, part, G, f);
+void launch_ls_step(hipStream_t s, DevScalars* ds, const double* part, int G, double f,
+                    const double* fdev, int single) {
+  hipLaunchKernelGGL(s_ls_step, dim3(1), dim3(kBlock), 0, s, ds, part, G, f, fdev, single);
 }
 void launch_ls_stop(hipStream_t s, DevScalars* ds, const double* pmax, const double* pmin,
-                    const long long* pmini, int G, double f) {
-  hipLaunchKernelGGL(s_ls_stop, dim3(1), dim3(kBlock), 0, s, ds, pmax, pmin, pmini, G, f);
+                    const long long* pmini, int G, double f, const double* fdev, int single) {
+  hipLaunchKernelGGL(s_ls_stop, dim3(1), dim3(kBlock), 0, s, ds, pmax, pmin, pmini, G, f,
+                     fdev, single);
 }
 
 #undef A1
