@@ -76,10 +76,16 @@ class DeviceProblem:
 
 
 def minimize(dp: DeviceProblem, eps=1e-8, max_iter=1000, m=8, x0=None, eps_g=None,
-             eps_f=None, eps_x=None, after_iteration=None, stop=None, noalloc=False):
+             eps_f=None, eps_x=None, after_iteration=None, stop=None, noalloc=False,
+             foreign_funcgrad=False):
     """Plain solve through lbfgsb_minimize_{f64,f32} (noalloc: through
     lbfgsb_minimize_noalloc_{f64,f32}) with the built-in objective as
-    funcgrad.  Returns (x tensor, summary dict with the return code)."""
+    funcgrad.  Returns (x tensor, summary dict with the return code).
+
+    The library recognises its own objective and takes each trial's value
+    from device memory; with foreign_funcgrad the objective is passed inside
+    a ctypes callback instead, which the library treats as a caller's
+    funcgrad (the value returned on the host after each evaluation)."""
     torch = _torch()
     L = _lib.lib()
     x = (dp.tensors["x0"] if x0 is None else x0).clone()
@@ -97,6 +103,13 @@ def minimize(dp: DeviceProblem, eps=1e-8, max_iter=1000, m=8, x0=None, eps_g=Non
     cbs.funcgrad_ctx = C.c_void_p(dp.handle)
     keep = []
     CB = _lib.CallbackF64 if f64 else _lib.CallbackF32
+    if foreign_funcgrad:
+        wrap = CB(lambda ctx, xp, fp, gp, s, summ: fg(ctx, C.cast(xp, C.c_void_p),
+                                                      C.cast(fp, C.c_void_p),
+                                                      C.cast(gp, C.c_void_p), s,
+                                                      C.cast(summ, C.c_void_p)))
+        keep.append(wrap)
+        cbs.funcgrad = C.cast(wrap, C.c_void_p)
     if after_iteration is not None:
         cb = CB(lambda ctx, xp, fp, gp, s, summ: int(after_iteration(summ.contents) or 0))
         keep.append(cb)

@@ -64,3 +64,22 @@ def test_solve_matches_oracle(case, dtype, oracle):
     else:
         assert summ["info"] in (1, 2, 4, 5), summ
         assert rel < tol32
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_device_value_path_matches_callback(dtype):
+    """The library's own objective as funcgrad (each trial's value read by
+    the line search from device memory) against the same objective inside a
+    foreign callback (the value returned on the host): the same solve, bit
+    for bit."""
+    from lbfgsb_amd import problems as pb
+    from lbfgsb_amd import solver
+    for p, eps in ((pb.rosenbrock(4096), 1e-7), (pb.grid_cos(64), 1e-7),
+                   (pb.penta_qp(5000), 1e-6)):
+        print(f"case kind {p.kind} n {p.n} {np.dtype(dtype).name}", flush=True)
+        dp = solver.DeviceProblem.create(p, dtype)
+        xa, sa = solver.minimize(dp, eps=eps, max_iter=2000)
+        xb, sb = solver.minimize(dp, eps=eps, max_iter=2000, foreign_funcgrad=True)
+        dp.close()
+        assert sa == sb
+        assert np.array_equal(xa.cpu().numpy(), xb.cpu().numpy())
