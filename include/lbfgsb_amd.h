@@ -191,7 +191,10 @@ lbfgsb_problem* lbfgsb_problem_create(int dtype, int kind, long long n, long lon
                                       long long ny, double h, const void* b, const void* c,
                                       const void* a0, const void* a1, const void* a2);
 void lbfgsb_problem_destroy(lbfgsb_problem* p);
-/* Usable directly as funcgrad (ctx = the problem). */
+/* Usable directly as funcgrad (ctx = the problem).  Passed by this address,
+ * lbfgsb_minimize_* and lbfgsb_engine_solve_* read each trial's value from
+ * the problem's device double instead of waiting for it on the host (the
+ * same value, rounded to the solve's precision). */
 int lbfgsb_problem_funcgrad_f64(void* ctx, double* x, double* f, double* g, void* stream,
                                 const lbfgsb_summary_f64* summary);
 int lbfgsb_problem_funcgrad_f32(void* ctx, float* x, float* f, float* g, void* stream,
