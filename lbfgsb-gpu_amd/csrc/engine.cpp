@@ -308,9 +308,28 @@ struct Engine final : EngineBase {
     if (!owns_memory()) return read_scalars();  // no pinned mailbox in a region
     launch_publish(st, ds, hs, mseq, ++mail_seq);
     HIPCHK(hipGetLastError());
+    return wait_status();
+  }
+  int wait_status() {
     HIPCHK(mail_wait(st, mseq, mail_seq));
     if (!pending.empty()) drain_events();
     return 0;
+  }
+  // Enqueues steps[a, b) and reads the status at their end.  k_trial only
+  // reads the device state, so when it is the last step the state is
+  // published before it: the host wakes and enqueues what follows (the
+  // objective on the trial point) while k_trial runs.
+  int enqueue_and_read(const std::vector<int>& steps, size_t a, size_t b, double f) {
+    const bool early = owns_memory() && b > a && steps[b - 1] == K_TRIAL;
+    for (size_t k = a; k < (early ? b - 1 : b); k++) enqueue(steps[k], f);
+    if (!early) {
+      HIPCHK(hipGetLastError());
+      return read_status();
+    }
+    launch_publish(st, ds, hs, mseq, ++mail_seq);
+    enqueue(K_TRIAL, f);
+    HIPCHK(hipGetLastError());
+    return wait_status();
   }
   int write_scalars() override {
     HIPCHK(hipMemcpyAsync(ds, hs, sizeof(DevScalars), hipMemcpyHostToDevice, st));
@@ -500,9 +519,7 @@ struct Engine final : EngineBase {
     }
     size_t j = i;
     while (j < steps.size() && steps[j] != stop_step) j++;
-    for (size_t k = i; k < j; k++) enqueue(steps[k], f);
-    HIPCHK(hipGetLastError());
-    if (read_status()) return -1;
+    if (enqueue_and_read(steps, i, j, f)) return -1;
     note_handoffs();
     if (hs->status == ST_GCP_ROUNDS) {
       hipEvent_t ea = (profile_from >= 0 && hs->iter >= profile_from &&
@@ -523,11 +540,7 @@ struct Engine final : EngineBase {
       // resume the tail from the Cauchy point pass, up to stop_step
       size_t p = 0;
       while (p < steps.size() && steps[p] != K_P4) p++;
-      if (p < j) {
-        for (size_t k = p; k < j; k++) enqueue(steps[k], f);
-        HIPCHK(hipGetLastError());
-        if (read_status()) return -1;
-      }
+      if (p < j && enqueue_and_read(steps, p, j, f)) return -1;
     }
     near_hint = hs->gcp_sel_near != 0;
     return hs->status;
