@@ -473,7 +473,7 @@ __device__ inline void ls_failure(DevScalars* ds) {
 // Per-step logic, thread 0, on reduced values.  Each control kernel and the
 // batched kernel call these after finishing their reductions.
 
-// errclb verdict (lbfgsbcpu.cpp:871-905): counts of bad nbd and of l > u.
+// errclb outcome (lbfgsbcpu.cpp:871-905): counts of bad nbd and of l > u.
 __device__ inline void logic_setup_errclb(DevScalars* ds, double bad_nbd, double bad_box) {
   if (bad_nbd > 0 || bad_box > 0) set_done(ds, -1);
 }
